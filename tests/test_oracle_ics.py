@@ -1,0 +1,58 @@
+"""Pin the oracle's initial-condition stage (ics/*.jl).
+
+Restates test/test_kepler_init.jl and test/test_init_nbody.jl (analytic
+jac_init against finite differences; binary128 stands in for BigFloat) and
+test/test_initial_conditions.jl (the fully nested hierarchy).
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from refcases import elements, coordinates, rel_maxabs
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_kepler_init_jacobian(seed):
+    # test_kepler_init.jl:10-40: random elements with e < 0.2, P = 1.5
+    rng = np.random.default_rng(seed)
+    period = 1.5
+    while True:
+        el = np.array([period, rng.random() * period, rng.standard_normal(), rng.standard_normal(),
+                       rng.random() * np.pi, rng.random() * np.pi])
+        if el[2] ** 2 + el[3] ** 2 < 0.2 ** 2:
+            break
+    for t in (2.4, 2.4 + 0.37 * period, 2.4 + 0.81 * period):
+        _, _, jac = O.kepler_init(t, 1.0, el)
+        num = O.fd_kepler_init(t, 1.0, el, dq=1e-15)
+        assert rel_maxabs(jac, num) < 1e-10
+
+
+def test_init_nbody_jacobian_trappist():
+    # test_init_nbody.jl: jac_init of the 8-body TRAPPIST-1 fixture
+    el = elements()
+    t0 = 7257.93115525
+    _, _, jac = O.init_nbody(t0, el)
+    num = O.fd_init_nbody(t0, el, dq=1e-15)
+    assert rel_maxabs(jac, num) < 1e-10
+
+
+def test_hierarchy_fully_nested():
+    # setup_hierarchy.jl:9-29 with [n, 1, 1, ...]; documented example H = 4
+    H = O.hierarchy_nested(4)
+    expect = np.array([[-1, 1, 0, 0], [-1, -1, 1, 0], [-1, -1, -1, 1], [-1, -1, -1, -1]], dtype=float)
+    assert np.array_equal(H, expect)
+    H2 = O.hierarchy_nested(2)
+    assert np.array_equal(H2, np.array([[-1, 1], [-1, -1]], dtype=float))
+
+
+def test_center_of_mass_at_rest_and_fixture_coordinates():
+    # The last hierarchy row is the centre of mass: it starts at the origin at
+    # rest.  The Cartesian fixture (test/coordinates.txt) was written from the
+    # same elements; positions agree to the file's printed precision.
+    el = elements()
+    x, v, _ = O.init_nbody(7257.93115525, el)
+    m = el[:, 0]
+    assert np.max(np.abs(x @ m)) < 1e-15 and np.max(np.abs(v @ m)) < 1e-15
+    co = coordinates()
+    assert co.shape == (8, 7)
+    assert np.allclose(co[:, 0], m)
