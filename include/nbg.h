@@ -1,0 +1,125 @@
+/* nbg.h -- C ABI of libnbg_hip.so: the AHL21 integrator step with Jacobian
+ * propagation and the transit-time search of NbodyGradient.jl, on gfx950.
+ *
+ * Conventions (shared by every entry point):
+ *  - All arrays are host memory in the reference's Julia layout
+ *    (column-major, 1-based indices become 0-based):
+ *      x, v, xerror, verror : 3 x n        m : n        pair : n x n (uint8)
+ *      jac_step, jac_error, jac_init : 7n x 7n        dqdt : 7n
+ *      tt : n x ntt (TransitTiming) or 3 x n x ntt (TransitParameters.ttbv)
+ *      dtdq0, dtdelements : n x ntt x 7 x n  (or 3 x n x ntt x 7 x n)
+ *      count : n (int64)
+ *  - Indices of bodies (ti) are 0-based.
+ *  - The library copies in and out and keeps no host pointer after return.
+ *  - Return value: 0 on success, negative on error (message in
+ *    nbg_last_error).  Numerical non-convergence is not an error: the
+ *    reference caps its iterations silently, and so does this library.
+ *  - One call at a time per context (calls are serialised by a mutex).
+ *  - Bodies per system: 2 <= n <= 12.  jac_step's mass rows must be unit rows
+ *    and jac_error's mass rows zero (as State(ic) creates them; every step
+ *    preserves this).
+ */
+#ifndef NBG_H
+#define NBG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nbg_ctx nbg_ctx;
+
+/* Context on one GPU (HIP device index). */
+int nbg_ctx_create(nbg_ctx** ctx, int device);
+void nbg_ctx_destroy(nbg_ctx* ctx);
+const char* nbg_last_error(const nbg_ctx* ctx);
+/* Device name, compute-unit count, LDS bytes per system for n bodies. */
+int nbg_device_info(nbg_ctx* ctx, char* name, int name_len, int* cu_count);
+size_t nbg_lds_bytes_per_system(int n);
+
+/* One AHL21 step, in place.
+ * Replaces ahl21!(s, d, h) (src/integrator/ahl21/ahl21.jl:5-95) when grad != 0
+ * and ahl21!(s, h) (src/integrator/ahl21/ahl21_no_grad.jl:6-18) when grad == 0;
+ * the Julia scheme plug-in Integrator.scheme calls it (Integrator.jl:17-22,
+ * Transits.jl:156-158).  With grad == 0, jac_step/jac_error/dqdt may be NULL. */
+int nbg_ahl21_step(nbg_ctx* ctx, int32_t n, double h, int32_t grad, double* x, double* v, double* xerror,
+                   double* verror, const double* m, const uint8_t* pair, double* jac_step, double* jac_error,
+                   double* dqdt);
+
+/* (intr::Integrator)(s, N; grad) -- Integrator.jl:211-234.  N may be
+ * negative (integrates backward); *t advances with Kahan summation. */
+int nbg_integrate_steps(nbg_ctx* ctx, int32_t n, double h, int64_t N, int32_t grad, double* t, double* x, double* v,
+                        double* xerror, double* verror, const double* m, const uint8_t* pair, double* jac_step,
+                        double* jac_error, double* dqdt);
+
+/* (intr::Integrator)(s, time; grad) -- Integrator.jl:159-197: full steps of h
+ * towards `time`, one partial step to land on it, *t = time. */
+int nbg_integrate_to(nbg_ctx* ctx, int32_t n, double h, double time, int32_t grad, double* t, double* x, double* v,
+                     double* xerror, double* verror, const double* m, const uint8_t* pair, double* jac_step,
+                     double* jac_error, double* dqdt);
+
+/* (intr::Integrator)(s, tt; grad) -- Transits.jl:140-180 with timing.jl:3-194:
+ * round(tmax/h) steps from *t, transit search after every step (findtransit!
+ * with dt/dq), calc_dtdelements! at the end when grad != 0.
+ * ntbv = 1: TransitTiming (tt, dtdq0, dtdelements);
+ * ntbv = 3: TransitParameters (ttbv, dtbvdq0, dtbvdelements).
+ * The output arrays are filled in place (entries beyond count stay as given). */
+int nbg_integrate_transits(nbg_ctx* ctx, int32_t n, double h, double tmax, int32_t ti, int32_t grad, int32_t ntbv,
+                           double* t, double* x, double* v, double* xerror, double* verror, const double* m,
+                           const uint8_t* pair, double* jac_step, double* jac_error, double* dqdt,
+                           const double* jac_init, int64_t ntt, double* tt, double* dtdq0, double* dtdelements,
+                           int64_t* count);
+
+/* ---- Batched form: many independent systems in one persistent launch ----
+ * Systems are packed back to back: system s uses n[s] bodies and ntt[s]
+ * transit slots; x/v/m/pair/jac_init/tt/dtdq0/dtdelements/count hold the
+ * per-system arrays above concatenated in system order (pair may be NULL =
+ * all pairs Kepler, i.e. State's default).  Every system starts from
+ * xerror = verror = 0, jac_step = I, jac_error = 0 at time t0[s]. */
+typedef struct nbg_batch nbg_batch;
+
+typedef struct {
+  int64_t system_steps;    /* main steps + one per recorded transit (final gradient step) */
+  int64_t transits;        /* transits recorded (count <= ntt) */
+  int64_t newton_iters;    /* findtransit! Newton iterations */
+  double kernel_ms;        /* device time of the last run (HIP events, all groups) */
+} nbg_batch_stats;
+
+int nbg_batch_create(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* tmax,
+                     const double* t0, const int32_t* ti, const int64_t* ntt, int32_t grad, int32_t ntbv,
+                     const double* x, const double* v, const double* m, const uint8_t* pair,
+                     const double* jac_init, nbg_batch** out);
+/* Restore every system to its initial state (device-side copy). */
+int nbg_batch_reset(nbg_batch* b);
+/* Integrate all systems to completion; systems longest first (LPT order),
+ * one persistent launch per body count, launches overlapped on streams. */
+int nbg_batch_run(nbg_batch* b, nbg_batch_stats* stats);
+/* Copy results to host (any pointer may be NULL). */
+int nbg_batch_download(nbg_batch* b, double* tt, double* dtdq0, double* dtdelements, int64_t* count, double* x,
+                       double* v);
+void nbg_batch_destroy(nbg_batch* b);
+
+/* One-shot convenience: create, run, download, destroy. */
+int nbg_batch_transits(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* tmax,
+                       const double* t0, const int32_t* ti, const int64_t* ntt, int32_t grad, int32_t ntbv,
+                       const double* x, const double* v, const double* m, const uint8_t* pair,
+                       const double* jac_init, double* tt, double* dtdq0, double* dtdelements, int64_t* count,
+                       nbg_batch_stats* stats);
+
+/* Tuning knobs for the persistent launches (0 = automatic). */
+int nbg_set_launch(nbg_ctx* ctx, int32_t max_groups_per_cu, int32_t streams);
+
+/* Diagnostic only: evaluates the device pair formulas (nbg_pair.h) on the
+ * host CPU so the test suite can compare them bit for bit with the CPU
+ * restatement without a GPU.  Not used by any integration path.
+ * out: delxv[6], jac_kepler[48] (6 x 8 column-major), jac_mass[6], iters. */
+int nbg_selftest_pair_host(const double* x0, const double* v0, double k, double h, int32_t drift_first,
+                           double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NBG_H */

@@ -1,0 +1,674 @@
+// nbg_capi.hip -- host side of libnbg_hip.so: the C ABI declared in
+// include/nbg.h.  Packs systems into the device records described in
+// nbg_types.h, launches one persistent kernel per body count (overlapped on
+// streams), and copies results back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/nbg.h"
+#include "nbg_pair.h"
+#include "nbg_types.h"
+
+extern "C" hipError_t nbg_launch_group(int nb, const nbg::SysDesc* descs, int nsys, int* queue, nbg::SysBufs bufs,
+                                       int64_t chunk, int grid, hipStream_t stream);
+extern "C" size_t nbg_lds_bytes(int nb);
+
+namespace {
+
+constexpr int NB_MIN = 2, NB_MAX = 12;
+
+template <class T> struct DevVec {
+  T* p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t count) {
+    if (count <= cap) return hipSuccess;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t c = std::max<size_t>(count, 1);
+    hipError_t e = hipMalloc(&p, c * sizeof(T));
+    if (e == hipSuccess) cap = c;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+// Device storage of a set of systems.
+struct Store {
+  DevVec<double> state, state0, jac, jac0, ws, tt, dq, del;
+  DevVec<int64_t> ints;
+  DevVec<uint8_t> pair;
+  DevVec<nbg::SysDesc> desc;
+  DevVec<int> queue;
+  void release() {
+    state.release(); state0.release(); jac.release(); jac0.release(); ws.release();
+    tt.release(); dq.release(); del.release(); ints.release(); pair.release(); desc.release(); queue.release();
+  }
+};
+
+struct Layout {
+  std::vector<nbg::SysDesc> desc;  // system order
+  int64_t n_state = 0, n_jac = 0, n_ws = 0, n_tt = 0, n_dq = 0, n_int = 0, n_pair = 0;
+  // launch groups: systems sorted by (n, cost desc)
+  std::vector<int> order;
+  std::vector<int> gn, goff, gcnt;
+};
+
+double check_step(double t0, double tmax) {  // Integrator.jl:249-259
+  auto sg = [](double x) { return x < 0 ? -1.0 : (x > 0 ? 1.0 : x); };
+  if (std::fabs(tmax) > std::fabs(t0)) return sg(tmax);
+  if (sg(tmax) != sg(t0)) return sg(tmax);
+  return -1.0 * sg(tmax);
+}
+
+long long jround(double x) { return (long long)std::nearbyint(x); }  // Julia round(Int64, x): half to even
+
+void finalize_layout(Layout& L) {
+  int64_t s = 0, j = 0, w = 0, t = 0, q = 0, in = 0, pr = 0;
+  for (auto& d : L.desc) {
+    const int n = d.n;
+    d.off_state = s; s += nbg::state_len(n);
+    d.off_jac = j; j += nbg::jac_len(n);
+    d.off_ws = w; w += nbg::ws_len(n);
+    d.off_tt = t; t += (int64_t)d.ntbv * n * d.ntt;
+    d.off_dq = q; q += (int64_t)d.ntbv * n * d.ntt * 7 * n;
+    d.off_int = in; in += nbg::int_len(n);
+    d.off_pair = pr; pr += (int64_t)n * n;
+  }
+  L.n_state = s; L.n_jac = j; L.n_ws = w; L.n_tt = t; L.n_dq = q; L.n_int = in; L.n_pair = pr;
+  const int ns = (int)L.desc.size();
+  L.order.resize(ns);
+  std::iota(L.order.begin(), L.order.end(), 0);
+  auto cost = [&](int i) {
+    const auto& d = L.desc[i];
+    const double n = d.n;
+    return (double)(d.nsteps + 1) * (2016.0 * n * n * (n - 1) + 126.0 * n * n * n + 3000.0 * n * (n - 1));
+  };
+  std::stable_sort(L.order.begin(), L.order.end(), [&](int a, int b) {
+    if (L.desc[a].n != L.desc[b].n) return L.desc[a].n > L.desc[b].n;
+    return cost(a) > cost(b);
+  });
+  L.gn.clear(); L.goff.clear(); L.gcnt.clear();
+  for (int k = 0; k < ns; ++k) {
+    const int n = L.desc[L.order[k]].n;
+    if (L.gn.empty() || L.gn.back() != n) {
+      L.gn.push_back(n);
+      L.goff.push_back(k);
+      L.gcnt.push_back(0);
+    }
+    L.gcnt.back() += 1;
+  }
+}
+
+bool mass_rows_ok(int n, const double* J, const double* JE) {
+  const int s = 7 * n;
+  for (int b = 0; b < n; ++b) {
+    const int r = 7 * b + 6;
+    for (int c = 0; c < s; ++c) {
+      if (J && J[r + (size_t)s * c] != (c == r ? 1.0 : 0.0)) return false;
+      if (JE && JE[r + (size_t)s * c] != 0.0) return false;
+    }
+  }
+  return true;
+}
+
+}  // namespace
+
+struct nbg_ctx {
+  int device = 0;
+  std::string err;
+  std::mutex mu;
+  int cu_count = 0;
+  std::string name;
+  int max_groups_per_cu = 0;
+  int nstreams = 4;
+  std::vector<hipStream_t> streams;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  Store single;  // reused by the single-system entry points
+};
+
+struct nbg_batch {
+  nbg_ctx* ctx = nullptr;
+  Layout L;
+  Store S;
+  int64_t steps_main = 0;
+};
+
+#define NBG_HIP(expr)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      ctx->err = std::string(#expr) + " failed: " + hipGetErrorString(e_);            \
+      return -2;                                                                      \
+    }                                                                                 \
+  } while (0)
+
+#define NBG_ARG(cond, msg)         \
+  do {                             \
+    if (!(cond)) {                 \
+      ctx->err = (msg);            \
+      return -1;                   \
+    }                              \
+  } while (0)
+
+namespace {
+
+int grid_for(nbg_ctx* ctx, int nb, int nsys) {
+  const size_t lds = nbg_lds_bytes(nb);
+  int per_cu = (int)(163840 / std::max<size_t>(lds, 1));
+  per_cu = std::max(1, std::min(per_cu, 8));
+  if (ctx->max_groups_per_cu > 0) per_cu = std::min(per_cu, ctx->max_groups_per_cu);
+  const int g = ctx->cu_count * per_cu;
+  return std::max(1, std::min(g, nsys));
+}
+
+// Allocate device storage for a layout and upload descriptors.
+int store_alloc(nbg_ctx* ctx, Store& S, const Layout& L, bool keep_initial) {
+  NBG_HIP(S.state.ensure(L.n_state));
+  NBG_HIP(S.jac.ensure(L.n_jac));
+  NBG_HIP(S.ws.ensure(L.n_ws));
+  NBG_HIP(S.tt.ensure(L.n_tt));
+  NBG_HIP(S.dq.ensure(L.n_dq));
+  NBG_HIP(S.del.ensure(L.n_dq));
+  NBG_HIP(S.ints.ensure(L.n_int));
+  NBG_HIP(S.pair.ensure(L.n_pair));
+  NBG_HIP(S.desc.ensure(L.desc.size()));
+  NBG_HIP(S.queue.ensure(L.gn.size() + 1));
+  if (keep_initial) {
+    NBG_HIP(S.state0.ensure(L.n_state));
+    NBG_HIP(S.jac0.ensure(L.n_jac));
+  }
+  std::vector<nbg::SysDesc> sorted(L.desc.size());
+  for (size_t k = 0; k < L.order.size(); ++k) sorted[k] = L.desc[L.order[k]];
+  NBG_HIP(hipMemcpy(S.desc.p, sorted.data(), sorted.size() * sizeof(nbg::SysDesc), hipMemcpyHostToDevice));
+  return 0;
+}
+
+nbg::SysBufs bufs_of(Store& S) {
+  nbg::SysBufs B;
+  B.state = S.state.p; B.jac = S.jac.p; B.ws = S.ws.p; B.tt = S.tt.p; B.dq = S.dq.p; B.del = S.del.p;
+  B.ints = S.ints.p; B.pair = S.pair.p;
+  return B;
+}
+
+// Launch every group (one persistent kernel per body count) and wait.
+int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* kernel_ms) {
+  NBG_HIP(hipSetDevice(ctx->device));
+  NBG_HIP(hipMemset(S.queue.p, 0, sizeof(int) * (L.gn.size() + 1)));
+  nbg::SysBufs B = bufs_of(S);
+  const int ns = (int)ctx->streams.size();
+  NBG_HIP(hipEventRecord(ctx->ev0, 0));
+  for (int s = 0; s < ns; ++s) NBG_HIP(hipStreamWaitEvent(ctx->streams[s], ctx->ev0, 0));
+  for (size_t g = 0; g < L.gn.size(); ++g) {
+    hipStream_t st = ctx->streams[g % ns];
+    const int grid = grid_for(ctx, L.gn[g], L.gcnt[g]);
+    NBG_HIP(nbg_launch_group(L.gn[g], S.desc.p + L.goff[g], L.gcnt[g], S.queue.p + g, B, chunk, grid, st));
+  }
+  for (int s = 0; s < ns; ++s) {
+    hipEvent_t e;
+    NBG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    NBG_HIP(hipEventRecord(e, ctx->streams[s]));
+    NBG_HIP(hipStreamWaitEvent(0, e, 0));
+    NBG_HIP(hipEventDestroy(e));
+  }
+  NBG_HIP(hipEventRecord(ctx->ev1, 0));
+  NBG_HIP(hipEventSynchronize(ctx->ev1));
+  NBG_HIP(hipGetLastError());
+  if (kernel_ms) {
+    float ms = 0;
+    NBG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    *kernel_ms = ms;
+  }
+  return 0;
+}
+
+// Host-side staging of one system's record.
+struct HostRec {
+  std::vector<double> state, jac;
+  std::vector<int64_t> ints;
+  std::vector<uint8_t> pair;
+};
+
+void pack_system(const nbg::SysDesc& d, const double* x, const double* v, const double* xe, const double* ve,
+                 const double* m, const uint8_t* pair, const double* J, const double* JE, const double* dqdt,
+                 const double* jinit, double t, const double* gsave, const int64_t* count, HostRec& R) {
+  const int n = d.n, s7 = 7 * n;
+  R.state.assign(nbg::state_len(n), 0.0);
+  for (int q = 0; q < 3 * n; ++q) {
+    R.state[nbg::off_x(n) + q] = x[q];
+    R.state[nbg::off_v(n) + q] = v[q];
+    R.state[nbg::off_xe(n) + q] = xe ? xe[q] : 0.0;
+    R.state[nbg::off_ve(n) + q] = ve ? ve[q] : 0.0;
+  }
+  for (int q = 0; q < n; ++q) {
+    R.state[nbg::off_m(n) + q] = m[q];
+    R.state[nbg::off_gsave(n) + q] = gsave ? gsave[q] : 0.0;
+  }
+  for (int q = 0; q < s7; ++q) R.state[nbg::off_dqdt(n) + q] = dqdt ? dqdt[q] : 0.0;
+  R.state[nbg::off_t(n)] = t;
+  const size_t NN = (size_t)s7 * s7;
+  R.jac.assign(3 * NN, 0.0);
+  for (size_t q = 0; q < NN; ++q) {
+    R.jac[q] = J ? J[q] : ((q % (s7 + 1)) == 0 ? 1.0 : 0.0);
+    R.jac[NN + q] = JE ? JE[q] : 0.0;
+    R.jac[2 * NN + q] = jinit ? jinit[q] : ((q % (s7 + 1)) == 0 ? 1.0 : 0.0);
+  }
+  R.ints.assign(nbg::int_len(n), 0);
+  if (count)
+    for (int q = 0; q < n; ++q) R.ints[q] = count[q];
+  R.pair.assign((size_t)n * n, 0);
+  if (pair)
+    for (int q = 0; q < n * n; ++q) R.pair[q] = pair[q] ? 1 : 0;
+}
+
+int any_kick(int n, const uint8_t* pair) {
+  if (!pair) return 0;
+  for (int i = 0; i < n - 1; ++i)
+    for (int j = i + 1; j < n; ++j)
+      if (pair[i + n * j]) return 1;
+  return 0;
+}
+
+// Run one system through the single-system store.
+int run_single(nbg_ctx* ctx, nbg::SysDesc d, double* x, double* v, double* xe, double* ve, const double* m,
+               const uint8_t* pair, double* J, double* JE, double* dqdt, const double* jinit, double* t,
+               double* tt, double* dtdq0, double* dtdel, int64_t* count) {
+  Layout L;
+  L.desc.push_back(d);
+  finalize_layout(L);
+  const nbg::SysDesc& D = L.desc[0];
+  HostRec R;
+  pack_system(D, x, v, xe, ve, m, pair, J, JE, dqdt, jinit, t ? *t : 0.0, nullptr, nullptr, R);
+  Store& S = ctx->single;
+  if (int rc = store_alloc(ctx, S, L, false)) return rc;
+  NBG_HIP(hipMemcpy(S.state.p, R.state.data(), R.state.size() * sizeof(double), hipMemcpyHostToDevice));
+  NBG_HIP(hipMemcpy(S.jac.p, R.jac.data(), R.jac.size() * sizeof(double), hipMemcpyHostToDevice));
+  NBG_HIP(hipMemcpy(S.ints.p, R.ints.data(), R.ints.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+  NBG_HIP(hipMemcpy(S.pair.p, R.pair.data(), R.pair.size(), hipMemcpyHostToDevice));
+  const size_t ntt_len = (size_t)L.n_tt, ndq = (size_t)L.n_dq;
+  if (D.mode == nbg::MODE_TRANSITS) {
+    // outputs start from the caller's arrays (entries past count stay as given)
+    NBG_HIP(hipMemcpy(S.tt.p, tt, ntt_len * sizeof(double), hipMemcpyHostToDevice));
+    if (dtdq0) NBG_HIP(hipMemcpy(S.dq.p, dtdq0, ndq * sizeof(double), hipMemcpyHostToDevice));
+    else NBG_HIP(hipMemset(S.dq.p, 0, ndq * sizeof(double)));
+    if (dtdel) NBG_HIP(hipMemcpy(S.del.p, dtdel, ndq * sizeof(double), hipMemcpyHostToDevice));
+    else NBG_HIP(hipMemset(S.del.p, 0, ndq * sizeof(double)));
+  }
+  if (int rc = run_groups(ctx, S, L, 0, nullptr)) return rc;
+  NBG_HIP(hipMemcpy(R.state.data(), S.state.p, R.state.size() * sizeof(double), hipMemcpyDeviceToHost));
+  NBG_HIP(hipMemcpy(R.ints.data(), S.ints.p, R.ints.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+  const int n = D.n, s7 = 7 * n;
+  for (int q = 0; q < 3 * n; ++q) {
+    x[q] = R.state[nbg::off_x(n) + q];
+    v[q] = R.state[nbg::off_v(n) + q];
+    if (xe) xe[q] = R.state[nbg::off_xe(n) + q];
+    if (ve) ve[q] = R.state[nbg::off_ve(n) + q];
+  }
+  if (D.grad) {
+    const size_t NN = (size_t)s7 * s7;
+    NBG_HIP(hipMemcpy(R.jac.data(), S.jac.p, 2 * NN * sizeof(double), hipMemcpyDeviceToHost));
+    if (J) std::memcpy(J, R.jac.data(), NN * sizeof(double));
+    if (JE) std::memcpy(JE, R.jac.data() + NN, NN * sizeof(double));
+    if (dqdt)
+      for (int q = 0; q < s7; ++q) dqdt[q] = R.state[nbg::off_dqdt(n) + q];
+  } else if (dqdt && D.mode == nbg::MODE_TRANSITS) {
+    for (int q = 0; q < s7; ++q) dqdt[q] = R.state[nbg::off_dqdt(n) + q];
+  }
+  if (D.mode == nbg::MODE_TRANSITS) {
+    if (t) *t = R.state[nbg::off_t(n)];
+    for (int q = 0; q < n; ++q) count[q] = R.ints[q];
+    NBG_HIP(hipMemcpy(tt, S.tt.p, ntt_len * sizeof(double), hipMemcpyDeviceToHost));
+    if (dtdq0) NBG_HIP(hipMemcpy(dtdq0, S.dq.p, ndq * sizeof(double), hipMemcpyDeviceToHost));
+    if (dtdel) NBG_HIP(hipMemcpy(dtdel, S.del.p, ndq * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  return 0;
+}
+
+int check_common(nbg_ctx* ctx, int32_t n, double h, const double* x, const double* v, const double* m) {
+  NBG_ARG(ctx != nullptr, "null context");
+  NBG_ARG(n >= NB_MIN && n <= NB_MAX, "n must be in [2, 12] for the GPU path");
+  NBG_ARG(x && v && m, "x, v and m are required");
+  NBG_ARG(std::isfinite(h) && h != 0.0, "step h must be finite and non-zero");
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int nbg_ctx_create(nbg_ctx** out, int device) {
+  if (!out) return -1;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return -3;
+  auto* ctx = new nbg_ctx();
+  ctx->device = device;
+  if (hipSetDevice(device) != hipSuccess) { delete ctx; return -3; }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) != hipSuccess) { delete ctx; return -3; }
+  ctx->cu_count = prop.multiProcessorCount;
+  ctx->name = prop.gcnArchName;
+  ctx->streams.resize(ctx->nstreams);
+  for (auto& s : ctx->streams)
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) { delete ctx; return -3; }
+  if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) { delete ctx; return -3; }
+  *out = ctx;
+  return 0;
+}
+
+void nbg_ctx_destroy(nbg_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();
+  ctx->single.release();
+  for (auto& s : ctx->streams) (void)hipStreamDestroy(s);
+  if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+  if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  delete ctx;
+}
+
+const char* nbg_last_error(const nbg_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int nbg_device_info(nbg_ctx* ctx, char* name, int name_len, int* cu_count) {
+  if (!ctx) return -1;
+  if (name && name_len > 0) {
+    std::snprintf(name, name_len, "%s", ctx->name.c_str());
+  }
+  if (cu_count) *cu_count = ctx->cu_count;
+  return 0;
+}
+
+size_t nbg_lds_bytes_per_system(int n) { return nbg_lds_bytes(n); }
+
+int nbg_set_launch(nbg_ctx* ctx, int32_t max_groups_per_cu, int32_t streams) {
+  if (!ctx) return -1;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->max_groups_per_cu = std::max(0, max_groups_per_cu);
+  if (streams > 0 && streams != (int)ctx->streams.size()) {
+    (void)hipDeviceSynchronize();
+    for (auto& s : ctx->streams) (void)hipStreamDestroy(s);
+    ctx->streams.assign(streams, nullptr);
+    for (auto& s : ctx->streams) NBG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  }
+  return 0;
+}
+
+int nbg_ahl21_step(nbg_ctx* ctx, int32_t n, double h, int32_t grad, double* x, double* v, double* xerror,
+                   double* verror, const double* m, const uint8_t* pair, double* jac_step, double* jac_error,
+                   double* dqdt) {
+  if (int rc = check_common(ctx, n, h, x, v, m)) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(!grad || (jac_step && jac_error && dqdt), "grad step needs jac_step, jac_error and dqdt");
+  NBG_ARG(!grad || mass_rows_ok(n, jac_step, jac_error), "jac_step mass rows must be unit rows, jac_error's zero");
+  nbg::SysDesc d{};
+  d.n = n; d.mode = nbg::MODE_STEPS; d.grad = grad ? 1 : 0; d.ti = 0; d.ntbv = 1; d.any_kick = any_kick(n, pair);
+  d.nsteps = 1; d.ntt = 0; d.h = h; d.hstep = h; d.hlast = 0.0; d.t0 = 0.0;
+  return run_single(ctx, d, x, v, xerror, verror, m, pair, jac_step, jac_error, dqdt, nullptr, nullptr, nullptr,
+                    nullptr, nullptr, nullptr);
+}
+
+int nbg_integrate_steps(nbg_ctx* ctx, int32_t n, double h, int64_t N, int32_t grad, double* t, double* x, double* v,
+                        double* xerror, double* verror, const double* m, const uint8_t* pair, double* jac_step,
+                        double* jac_error, double* dqdt) {
+  if (int rc = check_common(ctx, n, h, x, v, m)) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(t != nullptr, "t is required");
+  NBG_ARG(!grad || (jac_step && jac_error && dqdt), "grad integration needs jac_step, jac_error and dqdt");
+  NBG_ARG(!grad || mass_rows_ok(n, jac_step, jac_error), "jac_step mass rows must be unit rows, jac_error's zero");
+  double hs = h;
+  if (N < 0) { hs = -h; N = -N; }
+  nbg::SysDesc d{};
+  d.n = n; d.mode = nbg::MODE_STEPS; d.grad = grad ? 1 : 0; d.ti = 0; d.ntbv = 1; d.any_kick = any_kick(n, pair);
+  d.nsteps = N; d.ntt = 0; d.h = hs; d.hstep = h; d.hlast = 0.0; d.t0 = *t;
+  if (N > 0) {
+    int rc = run_single(ctx, d, x, v, xerror, verror, m, pair, jac_step, jac_error, dqdt, nullptr, nullptr, nullptr,
+                        nullptr, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  // s.t[1], s2 = comp_sum(s.t[1], s2, intr.h) per step (Integrator.jl:228)
+  double tv = *t, s2 = 0.0;
+  for (int64_t q = 0; q < N; ++q) {
+    s2 += hs;
+    const double tmp = tv + s2;
+    s2 = (tv - tmp) + s2;
+    tv = tmp;
+  }
+  *t = tv;
+  return 0;
+}
+
+int nbg_integrate_to(nbg_ctx* ctx, int32_t n, double h, double time, int32_t grad, double* t, double* x, double* v,
+                     double* xerror, double* verror, const double* m, const uint8_t* pair, double* jac_step,
+                     double* jac_error, double* dqdt) {
+  if (int rc = check_common(ctx, n, h, x, v, m)) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(t != nullptr, "t is required");
+  NBG_ARG(!grad || (jac_step && jac_error && dqdt), "grad integration needs jac_step, jac_error and dqdt");
+  NBG_ARG(!grad || mass_rows_ok(n, jac_step, jac_error), "jac_step mass rows must be unit rows, jac_error's zero");
+  const double t0 = *t;
+  const long long nsteps = std::llabs(jround((time - t0) / h));
+  const double hs = h * check_step(t0, time);
+  const double tmax = t0 + (hs * (double)nsteps);
+  nbg::SysDesc d{};
+  d.n = n; d.mode = nbg::MODE_STEPS; d.grad = grad ? 1 : 0; d.ti = 0; d.ntbv = 1; d.any_kick = any_kick(n, pair);
+  d.nsteps = nsteps; d.ntt = 0; d.h = hs; d.hstep = h; d.hlast = (tmax != time) ? time - tmax : 0.0; d.t0 = t0;
+  if (nsteps > 0 || d.hlast != 0.0) {
+    int rc = run_single(ctx, d, x, v, xerror, verror, m, pair, jac_step, jac_error, dqdt, nullptr, nullptr, nullptr,
+                        nullptr, nullptr, nullptr);
+    if (rc) return rc;
+  }
+  *t = time;
+  return 0;
+}
+
+int nbg_integrate_transits(nbg_ctx* ctx, int32_t n, double h, double tmax, int32_t ti, int32_t grad, int32_t ntbv,
+                           double* t, double* x, double* v, double* xerror, double* verror, const double* m,
+                           const uint8_t* pair, double* jac_step, double* jac_error, double* dqdt,
+                           const double* jac_init, int64_t ntt, double* tt, double* dtdq0, double* dtdelements,
+                           int64_t* count) {
+  if (int rc = check_common(ctx, n, h, x, v, m)) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(t && tt && count, "t, tt and count are required");
+  NBG_ARG(ti >= 0 && ti < n, "ti out of range");
+  NBG_ARG(ntbv == 1 || ntbv == 3, "ntbv must be 1 (TransitTiming) or 3 (TransitParameters)");
+  NBG_ARG(ntt >= 0, "ntt must be >= 0");
+  NBG_ARG(!grad || (jac_step && jac_error && dqdt && jac_init && dtdq0 && dtdelements),
+          "grad run needs jac_step, jac_error, dqdt, jac_init, dtdq0 and dtdelements");
+  NBG_ARG(!grad || mass_rows_ok(n, jac_step, jac_error), "jac_step mass rows must be unit rows, jac_error's zero");
+  const double t0 = *t;
+  nbg::SysDesc d{};
+  d.n = n; d.mode = nbg::MODE_TRANSITS; d.grad = grad ? 1 : 0; d.ti = ti; d.ntbv = ntbv;
+  d.any_kick = any_kick(n, pair);
+  d.nsteps = std::llabs(jround(tmax / h));
+  d.ntt = ntt; d.h = h * check_step(t0, tmax + t0); d.hstep = h; d.hlast = 0.0; d.t0 = t0;
+  for (int q = 0; q < n; ++q) count[q] = 0;
+  return run_single(ctx, d, x, v, xerror, verror, m, pair, jac_step, jac_error, dqdt, jac_init, t, tt, dtdq0,
+                    dtdelements, count);
+}
+
+// ---------------------------------------------------------------- batches
+int nbg_batch_create(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* tmax,
+                     const double* t0, const int32_t* ti, const int64_t* ntt, int32_t grad, int32_t ntbv,
+                     const double* x, const double* v, const double* m, const uint8_t* pair,
+                     const double* jac_init, nbg_batch** out) {
+  NBG_ARG(ctx != nullptr && out != nullptr, "null argument");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(nsys > 0, "nsys must be positive");
+  NBG_ARG(n && h && tmax && t0 && ntt && x && v && m, "missing per-system arrays");
+  NBG_ARG(ntbv == 1 || ntbv == 3, "ntbv must be 1 or 3");
+  NBG_ARG(!grad || jac_init, "grad batch needs jac_init");
+  auto* b = new nbg_batch();
+  b->ctx = ctx;
+  b->L.desc.resize(nsys);
+  for (int s = 0; s < nsys; ++s) {
+    if (n[s] < NB_MIN || n[s] > NB_MAX || !(std::isfinite(h[s]) && h[s] != 0.0) || ntt[s] < 0) {
+      delete b;
+      ctx->err = "system " + std::to_string(s) + ": n must be in [2,12], h finite non-zero, ntt >= 0";
+      return -1;
+    }
+  }
+  // per-system offsets into the packed host inputs
+  std::vector<int64_t> ob(nsys + 1, 0), oj(nsys + 1, 0), op(nsys + 1, 0);
+  for (int s = 0; s < nsys; ++s) {
+    ob[s + 1] = ob[s] + n[s];
+    oj[s + 1] = oj[s] + 49LL * n[s] * n[s];
+    op[s + 1] = op[s] + (int64_t)n[s] * n[s];
+  }
+  for (int s = 0; s < nsys; ++s) {
+    nbg::SysDesc& d = b->L.desc[s];
+    const int tis = ti ? ti[s] : 0;
+    if (tis < 0 || tis >= n[s]) {
+      delete b;
+      ctx->err = "system " + std::to_string(s) + ": ti out of range";
+      return -1;
+    }
+    d.n = n[s]; d.mode = nbg::MODE_TRANSITS; d.grad = grad ? 1 : 0; d.ti = tis; d.ntbv = ntbv;
+    d.any_kick = pair ? any_kick(n[s], pair + op[s]) : 0;
+    d.nsteps = std::llabs(jround(tmax[s] / h[s]));
+    d.ntt = ntt[s]; d.h = h[s] * check_step(t0[s], tmax[s] + t0[s]); d.hstep = h[s]; d.hlast = 0.0; d.t0 = t0[s];
+    b->steps_main += d.nsteps;
+  }
+  finalize_layout(b->L);
+  if (int rc = store_alloc(ctx, b->S, b->L, true)) {
+    b->S.release();
+    delete b;
+    return rc;
+  }
+  std::vector<double> hs((size_t)b->L.n_state), hj((size_t)b->L.n_jac);
+  std::vector<int64_t> hi((size_t)b->L.n_int, 0);
+  std::vector<uint8_t> hp((size_t)b->L.n_pair, 0);
+  HostRec R;
+  for (int s = 0; s < nsys; ++s) {
+    const nbg::SysDesc& d = b->L.desc[s];
+    pack_system(d, x + 3 * ob[s], v + 3 * ob[s], nullptr, nullptr, m + ob[s], pair ? pair + op[s] : nullptr, nullptr,
+                nullptr, nullptr, jac_init ? jac_init + oj[s] : nullptr, t0[s], nullptr, nullptr, R);
+    std::copy(R.state.begin(), R.state.end(), hs.begin() + d.off_state);
+    std::copy(R.jac.begin(), R.jac.end(), hj.begin() + d.off_jac);
+    std::copy(R.ints.begin(), R.ints.end(), hi.begin() + d.off_int);
+    std::copy(R.pair.begin(), R.pair.end(), hp.begin() + d.off_pair);
+  }
+  NBG_HIP(hipMemcpy(b->S.state0.p, hs.data(), hs.size() * sizeof(double), hipMemcpyHostToDevice));
+  NBG_HIP(hipMemcpy(b->S.jac0.p, hj.data(), hj.size() * sizeof(double), hipMemcpyHostToDevice));
+  NBG_HIP(hipMemcpy(b->S.pair.p, hp.data(), hp.size(), hipMemcpyHostToDevice));
+  *out = b;
+  // ints start at zero; the reset below also copies the initial state
+  return nbg_batch_reset(b);
+}
+
+int nbg_batch_reset(nbg_batch* b) {
+  if (!b) return -1;
+  nbg_ctx* ctx = b->ctx;
+  NBG_HIP(hipSetDevice(ctx->device));
+  NBG_HIP(hipMemcpy(b->S.state.p, b->S.state0.p, b->L.n_state * sizeof(double), hipMemcpyDeviceToDevice));
+  NBG_HIP(hipMemcpy(b->S.jac.p, b->S.jac0.p, b->L.n_jac * sizeof(double), hipMemcpyDeviceToDevice));
+  NBG_HIP(hipMemset(b->S.ints.p, 0, b->L.n_int * sizeof(int64_t)));
+  NBG_HIP(hipMemset(b->S.tt.p, 0, std::max<int64_t>(b->L.n_tt, 1) * sizeof(double)));
+  NBG_HIP(hipMemset(b->S.dq.p, 0, std::max<int64_t>(b->L.n_dq, 1) * sizeof(double)));
+  NBG_HIP(hipMemset(b->S.del.p, 0, std::max<int64_t>(b->L.n_dq, 1) * sizeof(double)));
+  NBG_HIP(hipDeviceSynchronize());
+  return 0;
+}
+
+int nbg_batch_run(nbg_batch* b, nbg_batch_stats* stats) {
+  if (!b) return -1;
+  nbg_ctx* ctx = b->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  double ms = 0.0;
+  if (int rc = run_groups(ctx, b->S, b->L, 0, &ms)) return rc;
+  if (stats) {
+    std::vector<int64_t> hi((size_t)b->L.n_int);
+    NBG_HIP(hipMemcpy(hi.data(), b->S.ints.p, hi.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    int64_t tr = 0, it = 0;
+    for (const auto& d : b->L.desc) {
+      tr += hi[d.off_int + d.n + 1];
+      it += hi[d.off_int + d.n + 2];
+    }
+    stats->transits = tr;
+    stats->newton_iters = it;
+    stats->system_steps = b->steps_main + (b->L.desc.empty() || !b->L.desc[0].grad ? 0 : tr);
+    stats->kernel_ms = ms;
+  }
+  return 0;
+}
+
+int nbg_batch_download(nbg_batch* b, double* tt, double* dtdq0, double* dtdelements, int64_t* count, double* x,
+                       double* v) {
+  if (!b) return -1;
+  nbg_ctx* ctx = b->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  const Layout& L = b->L;
+  // device layout == packed host layout in system order for tt / dq / del
+  if (tt) NBG_HIP(hipMemcpy(tt, b->S.tt.p, L.n_tt * sizeof(double), hipMemcpyDeviceToHost));
+  if (dtdq0) NBG_HIP(hipMemcpy(dtdq0, b->S.dq.p, L.n_dq * sizeof(double), hipMemcpyDeviceToHost));
+  if (dtdelements) NBG_HIP(hipMemcpy(dtdelements, b->S.del.p, L.n_dq * sizeof(double), hipMemcpyDeviceToHost));
+  if (count || x || v) {
+    std::vector<int64_t> hi((size_t)L.n_int);
+    std::vector<double> hs((size_t)L.n_state);
+    NBG_HIP(hipMemcpy(hi.data(), b->S.ints.p, hi.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
+    NBG_HIP(hipMemcpy(hs.data(), b->S.state.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
+    int64_t ob = 0;
+    for (const auto& d : L.desc) {
+      for (int q = 0; q < d.n; ++q) {
+        if (count) count[ob + q] = hi[d.off_int + q];
+      }
+      for (int q = 0; q < 3 * d.n; ++q) {
+        if (x) x[3 * ob + q] = hs[d.off_state + nbg::off_x(d.n) + q];
+        if (v) v[3 * ob + q] = hs[d.off_state + nbg::off_v(d.n) + q];
+      }
+      ob += d.n;
+    }
+  }
+  return 0;
+}
+
+void nbg_batch_destroy(nbg_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->ctx->device);
+  (void)hipDeviceSynchronize();
+  b->S.release();
+  delete b;
+}
+
+int nbg_batch_transits(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* tmax,
+                       const double* t0, const int32_t* ti, const int64_t* ntt, int32_t grad, int32_t ntbv,
+                       const double* x, const double* v, const double* m, const uint8_t* pair,
+                       const double* jac_init, double* tt, double* dtdq0, double* dtdelements, int64_t* count,
+                       nbg_batch_stats* stats) {
+  nbg_batch* b = nullptr;
+  int rc = nbg_batch_create(ctx, nsys, n, h, tmax, t0, ti, ntt, grad, ntbv, x, v, m, pair, jac_init, &b);
+  if (rc) return rc;
+  rc = nbg_batch_run(b, stats);
+  if (!rc) rc = nbg_batch_download(b, tt, dtdq0, dtdelements, count, nullptr, nullptr);
+  nbg_batch_destroy(b);
+  return rc;
+}
+
+int nbg_selftest_pair_host(const double* x0, const double* v0, double k, double h, int32_t drift_first,
+                           double* out) {
+  if (!x0 || !v0 || !out) return -1;
+  double delxv[6];
+  const nbg::KParams P = nbg::jac_delxv_gamma(x0, v0, delxv, k, h, drift_first != 0);
+  double kep[48], jm[6];
+  for (int q = 0; q < 48; ++q) kep[q] = 0.0;
+  for (int q = 0; q < 6; ++q) jm[q] = 0.0;
+  nbg::compute_jacobian_gamma(P, x0, v0, kep, jm, drift_first != 0);
+  for (int q = 0; q < 6; ++q) out[q] = delxv[q];
+  for (int q = 0; q < 48; ++q) out[6 + q] = kep[q];
+  for (int q = 0; q < 6; ++q) out[54 + q] = jm[q];
+  out[60] = (double)P.iters;
+  return 0;
+}
+
+}  // extern "C"

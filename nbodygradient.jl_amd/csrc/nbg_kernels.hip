@@ -1,0 +1,1054 @@
+// nbg_kernels.hip -- AHL21 step with Jacobian propagation and transit search
+// on gfx950.
+//
+// One wavefront (64 lanes, one workgroup) integrates one system; a persistent
+// grid pulls systems from a work queue.  Inside a wavefront:
+//  * the state (x, v, Kahan errors, masses, dq/dt) lives in LDS; the scalar
+//    pair work (universal-variable Kepler solve and its analytic Jacobian) is
+//    executed identically by every lane and stored by lane 0;
+//  * the 7n x 7n Jacobian lives in LDS without its mass rows, which the AHL21
+//    map never changes (they stay unit rows): J and its compensation error are
+//    6n x 7n, row-major, and lane c owns column c for every update (pair maps,
+//    drifts, the kick and force-gradient products, Kahan sums);
+//  * the kick matrix K, the force-gradient matrix Phi and d(acceleration)/dq
+//    have non-zeros only in velocity rows x (position, mass) columns; they are
+//    stored as 3n x 4n blocks and built with one lane per column.
+// Operation order follows NbodyGradient.jl src/integrator/ahl21/ahl21.jl:5-95
+// (mul! as a k-ascending fused multiply-add accumulation), so results match
+// the CPU restatement bit for bit.  Build with -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include "nbg_pair.h"
+#include "nbg_types.h"
+
+namespace nbg {
+
+constexpr int WAVE = 64;
+
+template <int NB> struct Shape {
+  static constexpr int N3 = 3 * NB, N4 = 4 * NB, N6 = 6 * NB, N7 = 7 * NB;
+};
+
+template <int NB> struct Smem {
+  static constexpr int N3 = 3 * NB, N4 = 4 * NB, N6 = 6 * NB, N7 = 7 * NB;
+  static constexpr int JOFF = N6 * N7;
+  // jac_step without mass rows (row 6b+k <-> Julia row 7b+k), then jac_error
+  // in the same layout at JOFF.  2*N6*N7 >= N7*N7, so jac_init fits here too.
+  double JJ[2 * N6 * N7];
+  double phi[N3 * N4];   // jac_phi block: row 3b+k (velocity k of body b), col 3d+p (x_p of d) or N3+d (m_d)
+  double dadq[N3 * N4];  // d a / d q: row 3b+k, same columns; also holds jac_kick (disjoint lifetimes)
+  double jij[14 * 14];   // jac_ij, column-major as in the reference
+  double dqij[14];       // dqdt_ij
+  double x[N3], v[N3], xe[N3], ve[N3], m[NB];
+  double px[N3], pv[N3], pxe[N3], pve[N3];  // s_prior
+  double a[N3];
+  double dqdt[N7], pdqdt[N7], dqdt_phi[N7], dqdt_kick[N7];
+  double gsave[NB];
+  double t, pt;
+  int64_t count[NB];
+  uint8_t pair[NB * NB];
+};
+
+__device__ inline void wsync() { __syncthreads(); }
+
+__device__ inline void kahan(double& s, double& e, double add) {
+  // comp_sum: utils.jl:16-23
+  e += add;
+  const double t = s + e;
+  e = (s - t) + e;
+  s = t;
+}
+
+// ---------------------------------------------------------------------------
+// Lane <-> column helpers for the 3n x 4n blocks: column cc is (d, p) with
+// p in 0..2 for positions and p = 3 for the mass.
+// ---------------------------------------------------------------------------
+template <int NB> __device__ inline void col_dp(int cc, int& d, int& p) {
+  constexpr int N3 = 3 * NB;
+  if (cc < N3) { d = cc / 3; p = cc - 3 * d; }
+  else { d = cc - N3; p = 3; }
+}
+
+// ---------------------------------------------------------------------------
+// kickfast! with Jacobian (ahl21.jl:337-386): impulses for kicked pairs.
+// v (Kahan) and dqdt_kick are uniform (lane 0 stores); jac_kick is built with
+// one lane per column into S.dadq.
+// ---------------------------------------------------------------------------
+template <int NB> __device__ void kick_build(Smem<NB>& S, double h, bool needQ, int lane) {
+  constexpr int N3 = 3 * NB, N4 = 4 * NB;
+  double* K = S.dadq;
+  if (lane < N4)
+    for (int r = 0; r < N3; ++r) K[r * N4 + lane] = 0.0;
+  int myd = -1, myp = -1;
+  if (lane < N4) col_dp<NB>(lane, myd, myp);
+  for (int i = 0; i < NB - 1; ++i) {
+    for (int j = i + 1; j < NB; ++j) {
+      if (!S.pair[i + NB * j]) continue;
+      double rij[3];
+      for (int k = 0; k < 3; ++k) rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
+      const double r2inv = 1.0 / dot3(rij);
+      const double r3inv = r2inv * __builtin_sqrt(r2inv);
+      const double fac2 = (h * NBG_GNEWT) * r3inv;
+      const double mi = S.m[i], mj = S.m[j];
+      for (int k = 0; k < 3; ++k) {
+        const double fac = fac2 * rij[k];
+        if (lane == 0) {
+          kahan(S.v[3 * i + k], S.ve[3 * i + k], (-mj) * fac);
+          kahan(S.v[3 * j + k], S.ve[3 * j + k], mi * fac);
+          if (needQ) {
+            S.dqdt_kick[7 * i + 3 + k] -= (mj * fac) / h;
+            S.dqdt_kick[7 * j + 3 + k] += (mi * fac) / h;
+          }
+        }
+        if (lane < N4) {
+          const int ri = 3 * i + k, rj = 3 * j + k;
+          if (myp == 3 && myd == j) K[ri * N4 + lane] -= fac;
+          if (myp == 3 && myd == i) K[rj * N4 + lane] += fac;
+          const double facx = fac * (3.0 * r2inv);
+          if (myp < 3 && myd == i) {
+            K[ri * N4 + lane] += (facx * mj) * rij[myp];
+            K[rj * N4 + lane] -= (facx * mi) * rij[myp];
+          }
+          if (myp < 3 && myd == j) {
+            K[ri * N4 + lane] -= (facx * mj) * rij[myp];
+            K[rj * N4 + lane] += (facx * mi) * rij[myp];
+          }
+          if (myp == k && myd == i) {
+            K[ri * N4 + lane] -= fac2 * mj;
+            K[rj * N4 + lane] += fac2 * mi;
+          }
+          if (myp == k && myd == j) {
+            K[ri * N4 + lane] += fac2 * mj;
+            K[rj * N4 + lane] -= fac2 * mi;
+          }
+        }
+      }
+      wsync();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// phic! (ahl21.jl:392-552, kicked pairs) and phisalpha! (ahl21.jl:558-700,
+// Kepler pairs, alpha = 2) with Jacobian.  Phi and dadq columns are owned by
+// lanes; a, v and dqdt_phi are uniform (lane 0 stores).
+// ---------------------------------------------------------------------------
+template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick, bool needQ, int lane) {
+  constexpr int N3 = 3 * NB, N4 = 4 * NB;
+  const bool own = lane < N4;
+  int myd = -1, myp = -1;
+  if (own) col_dp<NB>(lane, myd, myp);
+  double* PHI = S.phi;
+  double* D = S.dadq;
+  // phic!: a, dadq, jac_phi set to zero (ahl21.jl:393-399)
+  if (own)
+    for (int r = 0; r < N3; ++r) { PHI[r * N4 + lane] = 0.0; D[r * N4 + lane] = 0.0; }
+  if (lane == 0)
+    for (int r = 0; r < N3; ++r) S.a[r] = 0.0;
+  wsync();
+  if (anyKick) {
+    const double coeff = ((h * h * h) / 36.0) * NBG_GNEWT;
+    // loop 1: kicks 2h/3, accelerations, dadq (ahl21.jl:403-466)
+    for (int i = 0; i < NB - 1; ++i)
+      for (int j = i + 1; j < NB; ++j) {
+        if (!S.pair[i + NB * j]) continue;
+        double rij[3];
+        for (int k = 0; k < 3; ++k) rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
+        const double r2inv = 1.0 / dot3(rij);
+        const double r3inv = r2inv * __builtin_sqrt(r2inv);
+        const double mi = S.m[i], mj = S.m[j];
+        for (int k = 0; k < 3; ++k) {
+          double fac = (NBG_GNEWT * rij[k]) * r3inv;
+          double facv = ((fac * 2.0) * h) / 3.0;
+          const int ri = 3 * i + k, rj = 3 * j + k;
+          if (lane == 0) {
+            kahan(S.v[ri], S.ve[ri], (-mj) * facv);
+            kahan(S.v[rj], S.ve[rj], mi * facv);
+            if (needQ) {
+              S.dqdt_phi[7 * i + 3 + k] -= ((1.0 / h) * mj) * facv;
+              S.dqdt_phi[7 * j + 3 + k] += ((1.0 / h) * mi) * facv;
+            }
+            S.a[ri] -= mj * fac;
+            S.a[rj] += mi * fac;
+          }
+          if (own) {
+            if (myp == 3 && myd == j) PHI[ri * N4 + lane] -= facv;
+            if (myp == 3 && myd == i) PHI[rj * N4 + lane] += facv;
+            const double fv3 = facv * (3.0 * r2inv);
+            if (myp < 3 && myd == i) {
+              PHI[ri * N4 + lane] += (fv3 * mj) * rij[myp];
+              PHI[rj * N4 + lane] -= (fv3 * mi) * rij[myp];
+            }
+            if (myp < 3 && myd == j) {
+              PHI[ri * N4 + lane] -= (fv3 * mj) * rij[myp];
+              PHI[rj * N4 + lane] += (fv3 * mi) * rij[myp];
+            }
+            const double facd = (((2.0 * h) / 3.0) * NBG_GNEWT) * r3inv;
+            if (myp == k && myd == i) {
+              PHI[ri * N4 + lane] -= facd * mj;
+              PHI[rj * N4 + lane] += facd * mi;
+            }
+            if (myp == k && myd == j) {
+              PHI[ri * N4 + lane] += facd * mj;
+              PHI[rj * N4 + lane] -= facd * mi;
+            }
+            if (myp == 3 && myd == j) D[ri * N4 + lane] -= fac;
+            if (myp == 3 && myd == i) D[rj * N4 + lane] += fac;
+            const double f3 = fac * (3.0 * r2inv);
+            if (myp < 3 && myd == i) {
+              D[ri * N4 + lane] += (f3 * mj) * rij[myp];
+              D[rj * N4 + lane] -= (f3 * mi) * rij[myp];
+            }
+            if (myp < 3 && myd == j) {
+              D[ri * N4 + lane] -= (f3 * mj) * rij[myp];
+              D[rj * N4 + lane] += (f3 * mi) * rij[myp];
+            }
+            const double fg = NBG_GNEWT * r3inv;
+            if (myp == k && myd == i) {
+              D[ri * N4 + lane] -= fg * mj;
+              D[rj * N4 + lane] += fg * mi;
+            }
+            if (myp == k && myd == j) {
+              D[ri * N4 + lane] += fg * mj;
+              D[rj * N4 + lane] -= fg * mi;
+            }
+          }
+        }
+        wsync();
+      }
+    // loop 2: fourth-order correction for kicked pairs (ahl21.jl:471-550)
+    for (int i = 0; i < NB - 1; ++i)
+      for (int j = i + 1; j < NB; ++j) {
+        if (!S.pair[i + NB * j]) continue;
+        double rij[3], aij[3];
+        for (int k = 0; k < 3; ++k) {
+          aij[k] = S.a[3 * i + k] - S.a[3 * j + k];
+          rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
+        }
+        double dot = 0.0;
+        if (own)
+          for (int k = 0; k < 3; ++k) dot += rij[k] * (D[(3 * i + k) * N4 + lane] - D[(3 * j + k) * N4 + lane]);
+        const double r2 = dot3(rij);
+        const double r1 = __builtin_sqrt(r2);
+        const double ardot = dot3(aij, rij);
+        const double fac1 = coeff / ((r2 * r2) * r1);
+        const double fac2 = 3.0 * ardot;
+        const double mi = S.m[i], mj = S.m[j];
+        for (int k = 0; k < 3; ++k) {
+          const double fac = fac1 * (rij[k] * fac2 - r2 * aij[k]);
+          const int ri = 3 * i + k, rj = 3 * j + k;
+          if (lane == 0) {
+            kahan(S.v[ri], S.ve[ri], mj * fac);
+            kahan(S.v[rj], S.ve[rj], (-mi) * fac);
+            if (needQ) {
+              S.dqdt_phi[7 * i + 3 + k] += ((3.0 / h) * mj) * fac;
+              S.dqdt_phi[7 * j + 3 + k] -= ((3.0 / h) * mi) * fac;
+            }
+          }
+          if (own) {
+            double* pi_ = &PHI[ri * N4 + lane];
+            double* pj_ = &PHI[rj * N4 + lane];
+            if (myp == 3 && myd == j) *pi_ += fac;
+            if (myp == 3 && myd == i) *pj_ -= fac;
+            const double f5 = fac * (5.0 / r2);
+            if (myp < 3 && myd == i) { *pi_ -= (f5 * mj) * rij[myp]; *pj_ += (f5 * mi) * rij[myp]; }
+            if (myp < 3 && myd == j) { *pi_ += (f5 * mj) * rij[myp]; *pj_ -= (f5 * mi) * rij[myp]; }
+            const double fd = fac1 * fac2;
+            if (myp == k && myd == i) { *pi_ += fd * mj; *pj_ -= fd * mi; }
+            if (myp == k && myd == j) { *pi_ -= fd * mj; *pj_ += fd * mi; }
+            const double fr = ((-2.0) * fac1) * aij[k];
+            if (myp < 3 && (myd == i || myd == j)) {
+              const double fac3 = fr * rij[myp] + (((fac1 * 3.0) * rij[k]) * aij[myp]);
+              if (myd == i) { *pi_ += mj * fac3; *pj_ -= mi * fac3; }
+              else { *pi_ -= mj * fac3; *pj_ += mi * fac3; }
+            }
+            const double fa = (-fac1) * r2;
+            const double diff = D[ri * N4 + lane] - D[rj * N4 + lane];
+            *pi_ += (fa * mj) * diff;
+            *pj_ -= (fa * mi) * diff;
+            const double ff = (3.0 * fac1) * rij[k];
+            *pi_ += (ff * mj) * dot;
+            *pj_ -= (ff * mi) * dot;
+          }
+        }
+        wsync();
+      }
+  }
+  // phisalpha!: a, dadq zeroed (ahl21.jl:559-563)
+  if (own)
+    for (int r = 0; r < N3; ++r) D[r * N4 + lane] = 0.0;
+  if (lane == 0)
+    for (int r = 0; r < N3; ++r) S.a[r] = 0.0;
+  wsync();
+  const double coeff = (((2.0 * (h * h * h)) / 96.0) * 2.0) * NBG_GNEWT;
+  // loop 1: accelerations and dadq of Kepler pairs (ahl21.jl:567-602)
+  for (int i = 0; i < NB - 1; ++i)
+    for (int j = i + 1; j < NB; ++j) {
+      if (S.pair[i + NB * j]) continue;
+      double rij[3];
+      for (int k = 0; k < 3; ++k) rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
+      const double r2 = dot3(rij);
+      const double r3 = r2 * __builtin_sqrt(r2);
+      const double fac2 = NBG_GNEWT / r3;
+      const double mi = S.m[i], mj = S.m[j];
+      for (int k = 0; k < 3; ++k) {
+        const double fac = fac2 * rij[k];
+        const int ri = 3 * i + k, rj = 3 * j + k;
+        if (lane == 0) {
+          S.a[ri] -= mj * fac;
+          S.a[rj] += mi * fac;
+        }
+        if (own) {
+          if (myp == 3 && myd == j) D[ri * N4 + lane] -= fac;
+          if (myp == 3 && myd == i) D[rj * N4 + lane] += fac;
+          const double f3 = fac * (3.0 / r2);
+          if (myp < 3 && myd == i) {
+            D[ri * N4 + lane] += (f3 * mj) * rij[myp];
+            D[rj * N4 + lane] -= (f3 * mi) * rij[myp];
+          }
+          if (myp < 3 && myd == j) {
+            D[ri * N4 + lane] -= (f3 * mj) * rij[myp];
+            D[rj * N4 + lane] += (f3 * mi) * rij[myp];
+          }
+          if (myp == k && myd == i) {
+            D[ri * N4 + lane] -= fac2 * mj;
+            D[rj * N4 + lane] += fac2 * mi;
+          }
+          if (myp == k && myd == j) {
+            D[ri * N4 + lane] += fac2 * mj;
+            D[rj * N4 + lane] -= fac2 * mi;
+          }
+        }
+      }
+    }
+  wsync();
+  // loop 2: velocity corrections and jac_phi (ahl21.jl:607-698)
+  for (int i = 0; i < NB - 1; ++i)
+    for (int j = i + 1; j < NB; ++j) {
+      if (S.pair[i + NB * j]) continue;
+      double rij[3], aij[3];
+      for (int k = 0; k < 3; ++k) {
+        aij[k] = S.a[3 * i + k] - S.a[3 * j + k];
+        rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
+      }
+      double dot = 0.0;
+      if (own)
+        for (int k = 0; k < 3; ++k) dot += rij[k] * (D[(3 * i + k) * N4 + lane] - D[(3 * j + k) * N4 + lane]);
+      const double r2 = dot3(rij);
+      const double r1 = __builtin_sqrt(r2);
+      const double ardot = dot3(aij, rij);
+      const double fac1 = coeff / ((r2 * r2) * r1);
+      const double mi = S.m[i], mj = S.m[j];
+      const double fac2 = ((2.0 * NBG_GNEWT) * (mi + mj)) / r1 + 3.0 * ardot;
+      for (int k = 0; k < 3; ++k) {
+        const double fac = fac1 * (rij[k] * fac2 - r2 * aij[k]);
+        const int ri = 3 * i + k, rj = 3 * j + k;
+        if (lane == 0) {
+          kahan(S.v[ri], S.ve[ri], mj * fac);
+          kahan(S.v[rj], S.ve[rj], (-mi) * fac);
+          if (needQ) {
+            S.dqdt_phi[7 * i + 3 + k] += ((3.0 / h) * mj) * fac;
+            S.dqdt_phi[7 * j + 3 + k] -= ((3.0 / h) * mi) * fac;
+          }
+        }
+        if (own) {
+          double* pi_ = &PHI[ri * N4 + lane];
+          double* pj_ = &PHI[rj * N4 + lane];
+          if (myp == 3 && myd == j) *pi_ += fac;
+          if (myp == 3 && myd == i) *pj_ -= fac;
+          const double f5 = fac * (5.0 / r2);
+          if (myp < 3 && myd == i) { *pi_ -= (f5 * mj) * rij[myp]; *pj_ += (f5 * mi) * rij[myp]; }
+          if (myp < 3 && myd == j) { *pi_ += (f5 * mj) * rij[myp]; *pj_ -= (f5 * mi) * rij[myp]; }
+          const double fm = (((2.0 * NBG_GNEWT) * fac1) * rij[k]) / r1;
+          if (myp == 3 && myd == i) { *pi_ += fm * mj; *pj_ -= fm * mi; }
+          if (myp == 3 && myd == j) { *pi_ += fm * mj; *pj_ -= fm * mi; }
+          const double fd = fac1 * fac2;
+          if (myp == k && myd == i) { *pi_ += fd * mj; *pj_ -= fd * mi; }
+          if (myp == k && myd == j) { *pi_ -= fd * mj; *pj_ += fd * mi; }
+          const double fr = ((-2.0) * fac1) * ((((rij[k] * NBG_GNEWT) * (mi + mj)) / (r2 * r1)) + aij[k]);
+          if (myp < 3 && (myd == i || myd == j)) {
+            const double fac3 = fr * rij[myp] + (((fac1 * 3.0) * rij[k]) * aij[myp]);
+            if (myd == i) { *pi_ += mj * fac3; *pj_ -= mi * fac3; }
+            else { *pi_ -= mj * fac3; *pj_ += mi * fac3; }
+          }
+          const double fa = (-fac1) * r2;
+          const double diff = D[ri * N4 + lane] - D[rj * N4 + lane];
+          *pi_ += (fa * mj) * diff;
+          *pj_ -= (fa * mi) * diff;
+          const double ff = (3.0 * fac1) * rij[k];
+          *pi_ += (ff * mj) * dot;
+          *pj_ -= (ff * mi) * dot;
+        }
+      }
+      wsync();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Jacobian column kernels (lane owns column c).
+// ---------------------------------------------------------------------------
+// Position rows of column c (pre-update), the operand of K*J and Phi*J.
+template <int NB> __device__ inline void load_pos(const Smem<NB>& S, int c, double* P) {
+  constexpr int N7 = 7 * NB;
+#pragma unroll
+  for (int d = 0; d < NB; ++d)
+#pragma unroll
+    for (int p = 0; p < 3; ++p) P[3 * d + p] = S.JJ[(6 * d + p) * N7 + c];
+}
+
+// J[velocity rows, c] += (B * J)[velocity rows, c] with Kahan compensation,
+// B = K or Phi (3n x 4n block); then the zero-addend Kahan pass over the
+// position rows that comp_sum_matrix! applies to the rest of the matrix.
+template <int NB> __device__ inline void vel_product_add(Smem<NB>& S, const double* B, int c, const double* P) {
+  constexpr int N3 = 3 * NB, N4 = 4 * NB, N7 = 7 * NB;
+#pragma unroll 1
+  for (int rv = 0; rv < N3; ++rv) {
+    double s = 0.0;
+    const double* Br = B + rv * N4;
+#pragma unroll
+    for (int d = 0; d < NB; ++d) {
+      s = fmad(Br[3 * d + 0], P[3 * d + 0], s);
+      s = fmad(Br[3 * d + 1], P[3 * d + 1], s);
+      s = fmad(Br[3 * d + 2], P[3 * d + 2], s);
+      s = fmad(Br[N3 + d], (c == 7 * d + 6) ? 1.0 : 0.0, s);
+    }
+    const int b = rv / 3, k = rv - 3 * b;
+    const int r = 6 * b + 3 + k;
+    kahan(S.JJ[r * N7 + c], S.JJ[Smem<NB>::JOFF + r * N7 + c], s);
+  }
+}
+
+template <int NB> __device__ inline void zero_add_rows(Smem<NB>& S, int c, bool pos, bool vel) {
+  constexpr int N7 = 7 * NB;
+#pragma unroll 1
+  for (int b = 0; b < NB; ++b)
+    for (int k = 0; k < 6; ++k) {
+      if ((k < 3 && !pos) || (k >= 3 && !vel)) continue;
+      const int r = 6 * b + k;
+      double& J = S.JJ[r * N7 + c];
+      double& E = S.JJ[Smem<NB>::JOFF + r * N7 + c];
+      E += 0.0;
+      const double t = J + E;
+      E += J - t;
+      J = t;
+    }
+}
+
+// drift_grad! Jacobian part (ahl21.jl:326-328)
+template <int NB> __device__ inline void drift_cols(Smem<NB>& S, int c, double hh) {
+  constexpr int N7 = 7 * NB;
+#pragma unroll 1
+  for (int b = 0; b < NB; ++b)
+    for (int k = 0; k < 3; ++k) {
+      const int r = 6 * b + k;
+      kahan(S.JJ[r * N7 + c], S.JJ[Smem<NB>::JOFF + r * N7 + c], hh * S.JJ[(r + 3) * N7 + c]);
+    }
+}
+
+// Pair map on rows (i, j): copy_submatrix!, mul!, comp_sum_matrix!,
+// ypoc_submatrix! (ahl21.jl:31-42, utils.jl:51-100).
+template <int NB> __device__ inline void pair_cols(Smem<NB>& S, int i, int j, int c) {
+  constexpr int N7 = 7 * NB;
+  const int ri = 6 * i, rj = 6 * j;
+  double a[14], e[12];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    a[k] = S.JJ[(ri + k) * N7 + c];
+    a[7 + k] = S.JJ[(rj + k) * N7 + c];
+    e[k] = S.JJ[Smem<NB>::JOFF + (ri + k) * N7 + c];
+    e[6 + k] = S.JJ[Smem<NB>::JOFF + (rj + k) * N7 + c];
+  }
+  a[6] = (c == 7 * i + 6) ? 1.0 : 0.0;
+  a[13] = (c == 7 * j + 6) ? 1.0 : 0.0;
+  double o[12];
+#pragma unroll
+  for (int q = 0; q < 12; ++q) {
+    const int r = q < 6 ? q : q + 1;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < 14; ++k) s = fmad(S.jij[r + 14 * k], a[k], s);
+    o[q] = s;
+  }
+#pragma unroll
+  for (int q = 0; q < 12; ++q) {
+    const int src = q < 6 ? q : q + 1;
+    double sv = a[src], se = e[q];
+    se += o[q];
+    const double t = sv + se;
+    se += sv - t;
+    sv = t;
+    const int r = q < 6 ? ri + q : rj + (q - 6);
+    S.JJ[r * N7 + c] = sv;
+    S.JJ[Smem<NB>::JOFF + r * N7 + c] = se;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// kepler_driftij_gamma! with Jacobian (ahl21.jl:706-760) + the caller's
+// Jacobian and dq/dt updates (ahl21.jl:29-43 / 60-76).
+// ---------------------------------------------------------------------------
+template <int NB>
+__device__ void pair_step(Smem<NB>& S, int i, int j, double hh, bool drift_first, bool needJ, bool needQ, int lane) {
+  constexpr int N7 = 7 * NB;
+  double x0[3], v0[3];
+  for (int k = 0; k < 3; ++k) {
+    x0[k] = S.x[3 * i + k] - S.x[3 * j + k];
+    v0[k] = S.v[3 * i + k] - S.v[3 * j + k];
+  }
+  const double m_i = S.m[i], m_j = S.m[j];
+  const double gm = NBG_GNEWT * (m_i + m_j);
+  if (gm != 0.0) {
+    double delxv[6];
+    const KParams P = jac_delxv_gamma(x0, v0, delxv, gm, hh, drift_first);
+    const double mijinv = 1.0 / (m_i + m_j);
+    const double mi = m_i * mijinv;
+    const double mj = m_j * mijinv;
+    if (lane == 0) {
+      for (int k = 0; k < 3; ++k) {
+        kahan(S.x[3 * i + k], S.xe[3 * i + k], mj * delxv[k]);
+        kahan(S.x[3 * j + k], S.xe[3 * j + k], (-mi) * delxv[k]);
+      }
+      for (int k = 0; k < 3; ++k) {
+        kahan(S.v[3 * i + k], S.ve[3 * i + k], mj * delxv[3 + k]);
+        kahan(S.v[3 * j + k], S.ve[3 * j + k], (-mi) * delxv[3 + k]);
+      }
+    }
+    if (needJ || needQ) {
+      double kep[48], jm[6];
+      for (int q = 0; q < 48; ++q) kep[q] = 0.0;
+      for (int q = 0; q < 6; ++q) jm[q] = 0.0;
+      compute_jacobian_gamma(P, x0, v0, kep, jm, drift_first);
+      if (lane == 0) {
+        for (int q = 0; q < 196; ++q) S.jij[q] = 0.0;
+        for (int l = 0; l < 6; ++l)
+          for (int k = 0; k < 6; ++k) {
+            const double kv = kep[k + 6 * l];
+            S.jij[k + 14 * l] += mj * kv;
+            S.jij[k + 14 * (7 + l)] -= mj * kv;
+            S.jij[7 + k + 14 * l] -= mi * kv;
+            S.jij[7 + k + 14 * (7 + l)] += mi * kv;
+          }
+        for (int k = 0; k < 6; ++k) {
+          S.jij[k + 14 * 6] = jm[k] * m_j;
+          S.jij[k + 14 * 13] = ((mi * delxv[k]) * mijinv) + ((NBG_GNEWT * mj) * kep[k + 36]);
+          S.jij[7 + k + 14 * 6] = (((-mj) * delxv[k]) * mijinv) - ((NBG_GNEWT * mi) * kep[k + 36]);
+          S.jij[7 + k + 14 * 13] = (-jm[k]) * m_i;
+        }
+        for (int k = 0; k < 6; ++k) {
+          S.dqij[k] = mj * kep[k + 42];
+          S.dqij[7 + k] = (-mi) * kep[k + 42];
+        }
+      }
+    }
+  }
+  // (gm == 0: the reference returns before touching jac_ij / dqdt_ij, and the
+  // caller reuses the previous values; S.jij / S.dqij keep them likewise.)
+  wsync();
+  if (needJ)
+    for (int c = lane; c < N7; c += WAVE) pair_cols<NB>(S, i, j, c);
+  if (needQ) {
+    double q1[14];
+    for (int k = 0; k < 7; ++k) {
+      q1[k] = S.dqdt[7 * i + k];
+      q1[7 + k] = S.dqdt[7 * j + k];
+    }
+    if (lane < 14) {
+      double s = 0.0;
+      for (int k = 0; k < 14; ++k) s = fmad(S.jij[lane + 14 * k], q1[k], s);
+      const double nd = (S.dqij[lane] * 0.5) + (q1[lane] + s);
+      S.dqij[lane] = nd;
+      S.dqdt[lane < 7 ? 7 * i + lane : 7 * j + lane - 7] = nd;
+    }
+  }
+  wsync();
+}
+
+// ---------------------------------------------------------------------------
+// One AHL21 step: ahl21!(s, d, h) (ahl21.jl:5-95).  needJ: propagate the
+// Jacobian; needQ: compute dq/dt.  The state arithmetic is identical for
+// every flag combination (so grad and no-grad runs agree bit for bit, as the
+// reference's no-grad map ahl21_no_grad.jl:6-18 does).
+// ---------------------------------------------------------------------------
+template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, bool needQ, bool anyKick, int lane) {
+  constexpr int N3 = 3 * NB, N4 = 4 * NB, N7 = 7 * NB;
+  const double h2 = 0.5 * h;
+  const double h6 = h / 6.0;
+  // zero_out!(d); fill!(s.dqdt, 0)
+  if (lane < 14) S.dqij[lane] = 0.0;
+  for (int q = lane; q < 196; q += WAVE) S.jij[q] = 0.0;
+  if (needQ)
+    for (int r = lane; r < N7; r += WAVE) {
+      S.dqdt[r] = 0.0;
+      S.dqdt_phi[r] = 0.0;
+      S.dqdt_kick[r] = 0.0;
+    }
+  wsync();
+  // ---- kickfast!(s, d, h/6)
+  if (anyKick) kick_build<NB>(S, h6, needQ, lane);
+  wsync();
+  if (needQ) {
+    // dqdt_kick ./= 6; tmp7n = K*dqdt (dqdt = 0 here); dqdt .+= dqdt_kick .+ tmp7n
+    for (int r = lane; r < N7; r += WAVE) {
+      double tmp = 0.0;
+      if (anyKick) {
+        const int b = r / 7, k = r - 7 * b;
+        if (k >= 3 && k < 6) {
+          const double* Kr = S.dadq + (3 * b + k - 3) * N4;
+          for (int d = 0; d < NB; ++d) {
+            tmp = fmad(Kr[3 * d + 0], S.dqdt[7 * d + 0], tmp);
+            tmp = fmad(Kr[3 * d + 1], S.dqdt[7 * d + 1], tmp);
+            tmp = fmad(Kr[3 * d + 2], S.dqdt[7 * d + 2], tmp);
+            tmp = fmad(Kr[N3 + d], S.dqdt[7 * d + 6], tmp);
+          }
+        }
+      }
+      const double dk = S.dqdt_kick[r] / 6.0;
+      S.dqdt[r] = S.dqdt[r] + (dk + tmp);
+    }
+  }
+  // ---- Jacobian: jac_copy = K*J (pre-drift), drift_grad!, J += jac_copy
+  if (needJ)
+    for (int c = lane; c < N7; c += WAVE) {
+      double P[N3];
+      load_pos<NB>(S, c, P);
+      drift_cols<NB>(S, c, h2);
+      zero_add_rows<NB>(S, c, true, !anyKick);
+      if (anyKick) vel_product_add<NB>(S, S.dadq, c, P);
+    }
+  // ---- drift of the state (drift_grad! / drift!)
+  if (lane == 0)
+    for (int q = 0; q < N3; ++q) kahan(S.x[q], S.xe[q], h2 * S.v[q]);
+  wsync();
+  if (needQ) {
+    for (int q = lane; q < N3; q += WAVE) {
+      const int b = q / 3, k = q - 3 * b;
+      S.dqdt[7 * b + k] = 0.5 * S.v[q] + h2 * S.dqdt[7 * b + 3 + k];
+    }
+  }
+  wsync();
+  // ---- forward pair sweep (drift first)
+#pragma unroll 1
+  for (int i = 0; i < NB - 1; ++i)
+#pragma unroll 1
+    for (int j = i + 1; j < NB; ++j)
+      if (!S.pair[i + NB * j]) pair_step<NB>(S, i, j, h2, true, needJ, needQ, lane);
+  // ---- phic!, phisalpha!
+  phi_build<NB>(S, h, anyKick, needQ, lane);
+  wsync();
+  if (needJ)
+    for (int c = lane; c < N7; c += WAVE) {
+      double P[N3];
+      load_pos<NB>(S, c, P);
+      vel_product_add<NB>(S, S.phi, c, P);
+      zero_add_rows<NB>(S, c, true, false);
+    }
+  if (needQ) {
+    // tmp7n = jac_phi*dqdt; dqdt .+= dqdt_phi .+ tmp7n
+    double nv[2];
+    int nr = 0;
+    for (int r = lane; r < N7; r += WAVE, ++nr) {
+      double tmp = 0.0;
+      const int b = r / 7, k = r - 7 * b;
+      if (k >= 3 && k < 6) {
+        const double* Pr = S.phi + (3 * b + k - 3) * N4;
+        for (int d = 0; d < NB; ++d) {
+          tmp = fmad(Pr[3 * d + 0], S.dqdt[7 * d + 0], tmp);
+          tmp = fmad(Pr[3 * d + 1], S.dqdt[7 * d + 1], tmp);
+          tmp = fmad(Pr[3 * d + 2], S.dqdt[7 * d + 2], tmp);
+          tmp = fmad(Pr[N3 + d], S.dqdt[7 * d + 6], tmp);
+        }
+      }
+      nv[nr] = S.dqdt[r] + (S.dqdt_phi[r] + tmp);
+    }
+    wsync();
+    nr = 0;
+    for (int r = lane; r < N7; r += WAVE, ++nr) S.dqdt[r] = nv[nr];
+  }
+  wsync();
+  // ---- backward pair sweep (drift after)
+#pragma unroll 1
+  for (int i = NB - 2; i >= 0; --i)
+#pragma unroll 1
+    for (int j = NB - 1; j >= i + 1; --j)
+      if (!S.pair[i + NB * j]) pair_step<NB>(S, i, j, h2, false, needJ, needQ, lane);
+  // ---- drift
+  if (needJ)
+    for (int c = lane; c < N7; c += WAVE) drift_cols<NB>(S, c, h2);
+  if (lane == 0)
+    for (int q = 0; q < N3; ++q) kahan(S.x[q], S.xe[q], h2 * S.v[q]);
+  wsync();
+  if (needQ) {
+    for (int q = lane; q < N3; q += WAVE) {
+      const int b = q / 3, k = q - 3 * b;
+      S.dqdt[7 * b + k] += 0.5 * S.v[q] + h2 * S.dqdt[7 * b + 3 + k];
+    }
+    for (int r = lane; r < N7; r += WAVE) S.dqdt_kick[r] = 0.0;
+  }
+  wsync();
+  // ---- kickfast!(s, d, h/6)
+  if (anyKick) kick_build<NB>(S, h6, needQ, lane);
+  wsync();
+  if (needQ) {
+    double nv[2];
+    int nr = 0;
+    for (int r = lane; r < N7; r += WAVE, ++nr) {
+      double tmp = 0.0;
+      if (anyKick) {
+        const int b = r / 7, k = r - 7 * b;
+        if (k >= 3 && k < 6) {
+          const double* Kr = S.dadq + (3 * b + k - 3) * N4;
+          for (int d = 0; d < NB; ++d) {
+            tmp = fmad(Kr[3 * d + 0], S.dqdt[7 * d + 0], tmp);
+            tmp = fmad(Kr[3 * d + 1], S.dqdt[7 * d + 1], tmp);
+            tmp = fmad(Kr[3 * d + 2], S.dqdt[7 * d + 2], tmp);
+            tmp = fmad(Kr[N3 + d], S.dqdt[7 * d + 6], tmp);
+          }
+        }
+      }
+      const double dk = S.dqdt_kick[r] / 6.0;
+      nv[nr] = S.dqdt[r] + (dk + tmp);
+    }
+    wsync();
+    nr = 0;
+    for (int r = lane; r < N7; r += WAVE, ++nr) S.dqdt[r] = nv[nr];
+  }
+  if (needJ)
+    for (int c = lane; c < N7; c += WAVE) {
+      if (anyKick) {
+        double P[N3];
+        load_pos<NB>(S, c, P);
+        vel_product_add<NB>(S, S.dadq, c, P);
+        zero_add_rows<NB>(S, c, true, false);
+      } else {
+        zero_add_rows<NB>(S, c, true, true);
+      }
+    }
+  wsync();
+}
+
+// ---------------------------------------------------------------------------
+// Transit search (timing.jl:3-110) and element gradients (timing.jl:112-138)
+// ---------------------------------------------------------------------------
+template <int NB> __device__ inline double gsky(const Smem<NB>& S, int i, int j) {
+  return (S.x[3 * j] - S.x[3 * i]) * (S.v[3 * j] - S.v[3 * i]) +
+         (S.x[3 * j + 1] - S.x[3 * i + 1]) * (S.v[3 * j + 1] - S.v[3 * i + 1]);
+}
+template <int NB> __device__ inline double gdot(const Smem<NB>& S, int i, int j) {
+  return (((S.x[3 * j] - S.x[3 * i]) * (S.dqdt[7 * j + 3] - S.dqdt[7 * i + 3]) +
+           (S.x[3 * j + 1] - S.x[3 * i + 1]) * (S.dqdt[7 * j + 4] - S.dqdt[7 * i + 4])) +
+          (S.v[3 * j] - S.v[3 * i]) * (S.dqdt[7 * j + 0] - S.dqdt[7 * i + 0])) +
+         (S.v[3 * j + 1] - S.v[3 * i + 1]) * (S.dqdt[7 * j + 1] - S.dqdt[7 * i + 1]);
+}
+
+template <int NB> __device__ inline void save_prior(Smem<NB>& S, int lane) {
+  constexpr int N3 = 3 * NB, N7 = 7 * NB;
+  for (int q = lane; q < N3; q += WAVE) {
+    S.px[q] = S.x[q]; S.pv[q] = S.v[q]; S.pxe[q] = S.xe[q]; S.pve[q] = S.ve[q];
+  }
+  for (int q = lane; q < N7; q += WAVE) S.pdqdt[q] = S.dqdt[q];
+  if (lane == 0) S.pt = S.t;
+  wsync();
+}
+template <int NB> __device__ inline void restore_prior(Smem<NB>& S, int lane) {
+  constexpr int N3 = 3 * NB, N7 = 7 * NB;
+  for (int q = lane; q < N3; q += WAVE) {
+    S.x[q] = S.px[q]; S.v[q] = S.pv[q]; S.xe[q] = S.pxe[q]; S.ve[q] = S.pve[q];
+  }
+  for (int q = lane; q < N7; q += WAVE) S.dqdt[q] = S.pdqdt[q];
+  if (lane == 0) S.t = S.pt;
+  wsync();
+}
+
+template <int NB>
+__device__ void findtransit(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int ti, int jb, double dt0, int lane,
+                            double* ws) {
+  constexpr int N6 = 6 * NB, N7 = 7 * NB;
+  const int n = NB;
+  double stmp = 0.0;
+  double tt1 = dt0 + 1.0;
+  double tt2 = dt0 + 2.0;
+  int iter = 0;
+  while (true) {
+    tt2 = tt1;
+    tt1 = dt0;
+    restore_prior<NB>(S, lane);
+    ahl21_step<NB>(S, dt0, false, true, D.any_kick != 0, lane);
+    const double gs = gsky<NB>(S, ti, jb);
+    const double gd = gdot<NB>(S, ti, jb);
+    const double dt = -gs / gd;
+    kahan(dt0, stmp, dt);
+    iter += 1;
+    if (iter >= 20 || dt0 == tt1 || dt0 == tt2) break;
+  }
+  const int64_t c = S.count[jb] - 1;
+  const int nb = D.ntbv;
+  double* tt = B.tt + D.off_tt;
+  if (D.grad) {
+    restore_prior<NB>(S, lane);
+    // keep the prior Jacobian: the step below updates J in place
+    for (int col = lane; col < N7; col += WAVE)
+      for (int r = 0; r < N6; ++r) {
+        ws[r * N7 + col] = S.JJ[r * N7 + col];
+        ws[N6 * N7 + r * N7 + col] = S.JJ[Smem<NB>::JOFF + r * N7 + col];
+      }
+    ahl21_step<NB>(S, dt0, true, true, D.any_kick != 0, lane);
+    const double gd = gdot<NB>(S, ti, jb);
+    const double dx = S.x[3 * jb] - S.x[3 * ti], dy = S.x[3 * jb + 1] - S.x[3 * ti + 1];
+    const double dvx = S.v[3 * jb] - S.v[3 * ti], dvy = S.v[3 * jb + 1] - S.v[3 * ti + 1];
+    double* dq = B.dq + D.off_dq;
+    const int64_t ntt = D.ntt;
+    for (int col = lane; col < N7; col += WAVE) {
+      const int rj = 6 * jb, ri = 6 * ti;
+      const double d0 = -((((S.JJ[rj * N7 + col] - S.JJ[ri * N7 + col]) * dvx +
+                            (S.JJ[(rj + 1) * N7 + col] - S.JJ[(ri + 1) * N7 + col]) * dvy) +
+                           (S.JJ[(rj + 3) * N7 + col] - S.JJ[(ri + 3) * N7 + col]) * dx) +
+                          (S.JJ[(rj + 4) * N7 + col] - S.JJ[(ri + 4) * N7 + col]) * dy) /
+                        gd;
+      const int64_t base = (int64_t)jb + n * (c + ntt * (int64_t)col);
+      dq[0 + nb * base] = d0;
+      if (nb == 3) {
+        const double vsky = __builtin_sqrt(dvx * dvx + dvy * dvy);
+        const double dvdt = (dvx * (S.dqdt[7 * jb + 3] - S.dqdt[7 * ti + 3]) +
+                             dvy * (S.dqdt[7 * jb + 4] - S.dqdt[7 * ti + 4])) / vsky;
+        dq[1 + nb * base] = ((S.JJ[(rj + 3) * N7 + col] - S.JJ[(ri + 3) * N7 + col]) * dvx +
+                             (S.JJ[(rj + 4) * N7 + col] - S.JJ[(ri + 4) * N7 + col]) * dvy) / vsky +
+                            dvdt * d0;
+        dq[2 + nb * base] = 2.0 * ((S.JJ[rj * N7 + col] - S.JJ[ri * N7 + col]) * dx +
+                                   (S.JJ[(rj + 1) * N7 + col] - S.JJ[(ri + 1) * N7 + col]) * dy);
+      }
+    }
+    if (lane == 0) {
+      tt[0 + nb * (jb + n * c)] = S.t + dt0;
+      if (nb == 3) {
+        tt[1 + nb * (jb + n * c)] = __builtin_sqrt(dvx * dvx + dvy * dvy);
+        tt[2 + nb * (jb + n * c)] = dx * dx + dy * dy;
+      }
+    }
+    for (int col = lane; col < N7; col += WAVE)
+      for (int r = 0; r < N6; ++r) {
+        S.JJ[r * N7 + col] = ws[r * N7 + col];
+        S.JJ[Smem<NB>::JOFF + r * N7 + col] = ws[N6 * N7 + r * N7 + col];
+      }
+  } else if (lane == 0) {
+    tt[0 + nb * (jb + n * c)] = S.t + dt0;
+    if (nb == 3) {
+      const double dx = S.x[3 * jb] - S.x[3 * ti], dy = S.x[3 * jb + 1] - S.x[3 * ti + 1];
+      const double dvx = S.v[3 * jb] - S.v[3 * ti], dvy = S.v[3 * jb + 1] - S.v[3 * ti + 1];
+      tt[1 + nb * (jb + n * c)] = __builtin_sqrt(dvx * dvx + dvy * dvy);
+      tt[2 + nb * (jb + n * c)] = dx * dx + dy * dy;
+    }
+  }
+  if (lane == 0) {
+    int64_t* ints = B.ints + D.off_int;
+    ints[NB + 1] += 1;
+    ints[NB + 2] += iter;
+  }
+  wsync();
+}
+
+template <int NB>
+__device__ void detect_transits(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int lane, double* ws) {
+  save_prior<NB>(S, lane);
+  const int ti = D.ti;
+  for (int i = 0; i < NB; ++i) {
+    if (i == ti) continue;
+    const double gi = gsky<NB>(S, i, ti);
+    const double ri = __builtin_sqrt((S.x[3 * i] * S.x[3 * i] + S.x[3 * i + 1] * S.x[3 * i + 1]) + S.x[3 * i + 2] * S.x[3 * i + 2]);
+    if (gi > 0.0 && S.gsave[i] < 0.0 && -S.x[3 * i + 2] > 0.25 * ri && ri < 1e12) {
+      const int64_t cnt = S.count[i] + 1;
+      wsync();
+      if (lane == 0) S.count[i] = cnt;
+      wsync();
+      if (cnt <= D.ntt) {
+        const double dt0 = ((-gi) * D.hstep) / (gi - S.gsave[i]);
+        findtransit<NB>(S, D, B, ti, i, dt0, lane, ws);
+      }
+    }
+    wsync();
+    if (lane == 0) S.gsave[i] = gi;
+    restore_prior<NB>(S, lane);
+  }
+}
+
+// calc_dtdelements! (timing.jl:112-138) for one system; jac_init is staged
+// transposed in the Jacobian's LDS space (the Jacobian is already stored).
+template <int NB> __device__ void dtdelements(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int lane) {
+  constexpr int N7 = 7 * NB;
+  const int n = NB;
+  double* jt = S.JJ;  // N7*N7 <= 2*N6*N7 doubles
+  const double* jinit = B.jac + D.off_jac + 2 * (int64_t)N7 * N7;
+  for (int col = lane; col < N7; col += WAVE)
+    for (int cp = 0; cp < N7; ++cp) jt[col + N7 * cp] = jinit[cp + (int64_t)N7 * col];
+  wsync();
+  const double* dq = B.dq + D.off_dq;
+  double* del = B.del + D.off_dq;
+  const int nb = D.ntbv;
+  const int64_t ntt = D.ntt;
+  for (int b = 0; b < nb; ++b)
+    for (int i = 0; i < n; ++i) {
+      const int64_t cnt = S.count[i] < ntt ? S.count[i] : ntt;
+      for (int64_t jj = 0; jj < cnt; ++jj) {
+        for (int col = lane; col < N7; col += WAVE) {
+          double acc = 0.0;
+          for (int cp = 0; cp < N7; ++cp)
+            acc = acc + dq[b + nb * ((int64_t)i + n * (jj + ntt * cp))] * jt[col + N7 * cp];
+          del[b + nb * ((int64_t)i + n * (jj + ntt * col))] = acc;
+        }
+      }
+    }
+  wsync();
+}
+
+// ---------------------------------------------------------------------------
+// Load / store of a system between its HBM record and LDS.
+// ---------------------------------------------------------------------------
+template <int NB> __device__ void load_system(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int lane) {
+  constexpr int N3 = 3 * NB, N6 = 6 * NB, N7 = 7 * NB;
+  const double* st = B.state + D.off_state;
+  for (int q = lane; q < N3; q += WAVE) {
+    S.x[q] = st[off_x(NB) + q];
+    S.v[q] = st[off_v(NB) + q];
+    S.xe[q] = st[off_xe(NB) + q];
+    S.ve[q] = st[off_ve(NB) + q];
+  }
+  for (int q = lane; q < NB; q += WAVE) {
+    S.m[q] = st[off_m(NB) + q];
+    S.gsave[q] = st[off_gsave(NB) + q];
+    S.count[q] = B.ints[D.off_int + q];
+  }
+  for (int q = lane; q < N7; q += WAVE) S.dqdt[q] = st[off_dqdt(NB) + q];
+  for (int q = lane; q < NB * NB; q += WAVE) S.pair[q] = B.pair[D.off_pair + q];
+  if (lane == 0) S.t = st[off_t(NB)];
+  if (D.grad) {
+    const double* J = B.jac + D.off_jac;
+    const double* JE = J + (int64_t)N7 * N7;
+    for (int c = lane; c < N7; c += WAVE)
+      for (int b = 0; b < NB; ++b)
+        for (int k = 0; k < 6; ++k) {
+          S.JJ[(6 * b + k) * N7 + c] = J[(7 * b + k) + (int64_t)N7 * c];
+          S.JJ[Smem<NB>::JOFF + (6 * b + k) * N7 + c] = JE[(7 * b + k) + (int64_t)N7 * c];
+        }
+  }
+  (void)N6;
+  wsync();
+}
+
+template <int NB> __device__ void store_system(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int lane) {
+  constexpr int N3 = 3 * NB, N7 = 7 * NB;
+  wsync();
+  double* st = B.state + D.off_state;
+  for (int q = lane; q < N3; q += WAVE) {
+    st[off_x(NB) + q] = S.x[q];
+    st[off_v(NB) + q] = S.v[q];
+    st[off_xe(NB) + q] = S.xe[q];
+    st[off_ve(NB) + q] = S.ve[q];
+  }
+  for (int q = lane; q < NB; q += WAVE) {
+    st[off_gsave(NB) + q] = S.gsave[q];
+    B.ints[D.off_int + q] = S.count[q];
+  }
+  for (int q = lane; q < N7; q += WAVE) st[off_dqdt(NB) + q] = S.dqdt[q];
+  if (lane == 0) st[off_t(NB)] = S.t;
+  if (D.grad) {
+    double* J = B.jac + D.off_jac;
+    double* JE = J + (int64_t)N7 * N7;
+    for (int c = lane; c < N7; c += WAVE)
+      for (int b = 0; b < NB; ++b)
+        for (int k = 0; k < 6; ++k) {
+          J[(7 * b + k) + (int64_t)N7 * c] = S.JJ[(6 * b + k) * N7 + c];
+          JE[(7 * b + k) + (int64_t)N7 * c] = S.JJ[Smem<NB>::JOFF + (6 * b + k) * N7 + c];
+        }
+  }
+  wsync();
+}
+
+// Run (or resume) one system for at most `chunk` steps.  Returns true when
+// the system has finished.
+template <int NB>
+__device__ void run_system(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int64_t chunk, int lane) {
+  load_system<NB>(S, D, B, lane);
+  int64_t* ints = B.ints + D.off_int;
+  const int64_t istep0 = ints[NB];
+  if (istep0 > D.nsteps) return;  // finished earlier
+  const int64_t iend = (chunk > 0 && istep0 + chunk < D.nsteps) ? istep0 + chunk : D.nsteps;
+  const bool anyKick = D.any_kick != 0;
+  const bool grad = D.grad != 0;
+  double* ws = B.ws + D.off_ws;
+  if (D.mode == MODE_TRANSITS && istep0 == 0) {
+    // gsave[i] = g!(i, ti, x, v) before the loop (Transits.jl:146-149)
+    if (lane == 0)
+      for (int i = 0; i < NB; ++i)
+        if (i != D.ti) S.gsave[i] = gsky<NB>(S, i, D.ti);
+    wsync();
+  }
+#pragma unroll 1
+  for (int64_t is = istep0; is < iend; ++is) {
+    const bool last = (is + 1 == D.nsteps);
+    const bool needQ = last && (D.mode == MODE_TRANSITS || D.hlast == 0.0);
+    ahl21_step<NB>(S, D.h, grad, needQ, anyKick, lane);
+    if (D.mode == MODE_TRANSITS) {
+      if (lane == 0) S.t = D.t0 + ((double)(is + 1) * D.h);
+      wsync();
+      detect_transits<NB>(S, D, B, lane, ws);
+    }
+  }
+  const bool done = (iend == D.nsteps);
+  if (done && D.mode == MODE_STEPS && D.hlast != 0.0) ahl21_step<NB>(S, D.hlast, grad, true, anyKick, lane);
+  store_system<NB>(S, D, B, lane);
+  if (lane == 0) ints[NB] = done ? D.nsteps + 1 : iend;
+  if (done && D.mode == MODE_TRANSITS && grad) dtdelements<NB>(S, D, B, lane);
+}
+
+template <int NB>
+__global__ __launch_bounds__(WAVE) void nbg_run(const SysDesc* __restrict__ descs, int nsys, int* __restrict__ queue,
+                                                SysBufs B, int64_t chunk) {
+  __shared__ Smem<NB> S;
+  const int lane = threadIdx.x;
+  while (true) {
+    int idx = 0;
+    if (lane == 0) idx = atomicAdd(queue, 1);
+    idx = __shfl(idx, 0);
+    if (idx >= nsys) break;
+    const SysDesc D = descs[idx];
+    run_system<NB>(S, D, B, chunk, lane);
+    wsync();
+  }
+}
+
+}  // namespace nbg
+
+// ---------------------------------------------------------------------------
+// Host launcher (one launch per body count; the grid is persistent).
+// ---------------------------------------------------------------------------
+extern "C" hipError_t nbg_launch_group(int nb, const nbg::SysDesc* descs, int nsys, int* queue, nbg::SysBufs bufs,
+                                       int64_t chunk, int grid, hipStream_t stream) {
+  switch (nb) {
+#define NBG_CASE(N)                                                                                      \
+  case N:                                                                                                \
+    hipLaunchKernelGGL(nbg::nbg_run<N>, dim3(grid), dim3(nbg::WAVE), 0, stream, descs, nsys, queue, bufs, chunk); \
+    return hipGetLastError();
+    NBG_CASE(2)
+    NBG_CASE(3)
+    NBG_CASE(4)
+    NBG_CASE(5)
+    NBG_CASE(6)
+    NBG_CASE(7)
+    NBG_CASE(8)
+    NBG_CASE(9)
+    NBG_CASE(10)
+    NBG_CASE(11)
+    NBG_CASE(12)
+#undef NBG_CASE
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+extern "C" size_t nbg_lds_bytes(int nb) {
+  switch (nb) {
+#define NBG_SZ(N) case N: return sizeof(nbg::Smem<N>);
+    NBG_SZ(2) NBG_SZ(3) NBG_SZ(4) NBG_SZ(5) NBG_SZ(6) NBG_SZ(7) NBG_SZ(8) NBG_SZ(9) NBG_SZ(10) NBG_SZ(11) NBG_SZ(12)
+#undef NBG_SZ
+    default: return 0;
+  }
+}

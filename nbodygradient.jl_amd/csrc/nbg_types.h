@@ -1,0 +1,75 @@
+// nbg_types.h -- per-system work descriptors shared by host and device code.
+//
+// A batch is a set of independent systems.  Every system owns slices of a few
+// flat device buffers (offsets below, in elements); layouts follow the
+// reference's Julia (column-major) arrays so host copies need no transposes.
+//
+//  state  (double): x[3n] v[3n] xerror[3n] verror[3n] m[n] dqdt[7n] gsave[n]
+//                   t  (scalars at the end, STATE_T0_SLOT..)
+//  jac    (double): jac_step[7n*7n] jac_error[7n*7n] jac_init[7n*7n]
+//  ws     (double): two compressed 6n x 7n blocks (Jacobian copy during the
+//                   final gradient step of a transit)
+//  tt     (double): ntbv x n x ntt   (TransitTiming.tt / TransitParameters.ttbv)
+//  dq     (double): ntbv x n x ntt x 7 x n (dtdq0); del (double) same (dtdelements)
+//  ints   (int64) : count[n] istep ntransits_done newton_iters_total
+//  pair   (uint8) : n x n
+#pragma once
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define NBG_TD __host__ __device__ inline
+#else
+#define NBG_TD inline
+#endif
+
+namespace nbg {
+
+enum SysMode : int32_t {
+  MODE_STEPS = 0,     // (intr)(s, N) / (intr)(s, time): nsteps steps of h, then an optional step of hlast
+  MODE_TRANSITS = 1,  // (intr)(s, tt): step loop with transit search
+};
+
+struct SysDesc {
+  int32_t n;
+  int32_t mode;
+  int32_t grad;
+  int32_t ti;        // transited body, 0-based
+  int32_t ntbv;      // 1 TransitTiming, 3 TransitParameters
+  int32_t any_kick;  // some pair[i,j] (i<j) is true
+  int64_t nsteps;    // full steps in the run
+  int64_t ntt;
+  double h;          // signed step used by the loop
+  double hstep;      // intr.h (unsigned) for the first transit-time guess
+  double hlast;      // MODE_STEPS: extra final step (0 = none)
+  double t0;         // time at step 0 of the run (t = t0 + istep*h)
+  int64_t off_state, off_jac, off_ws, off_tt, off_dq, off_int, off_pair;
+};
+
+struct SysBufs {
+  double* state;
+  double* jac;
+  double* ws;
+  double* tt;
+  double* dq;
+  double* del;
+  int64_t* ints;
+  uint8_t* pair;
+};
+
+// state slots
+NBG_TD int64_t state_len(int n) { return 13 * (int64_t)n + 7 * n + n + 4; }
+NBG_TD int64_t off_x(int) { return 0; }
+NBG_TD int64_t off_v(int n) { return 3 * (int64_t)n; }
+NBG_TD int64_t off_xe(int n) { return 6 * (int64_t)n; }
+NBG_TD int64_t off_ve(int n) { return 9 * (int64_t)n; }
+NBG_TD int64_t off_m(int n) { return 12 * (int64_t)n; }
+NBG_TD int64_t off_dqdt(int n) { return 13 * (int64_t)n; }
+NBG_TD int64_t off_gsave(int n) { return 20 * (int64_t)n; }
+NBG_TD int64_t off_t(int n) { return 21 * (int64_t)n; }
+NBG_TD int64_t jac_len(int n) { return 3 * 49 * (int64_t)n * n; }
+NBG_TD int64_t ws_len(int n) { return 2 * 42 * (int64_t)n * n; }
+NBG_TD int64_t int_len(int n) { return (int64_t)n + 3; }
+
+}  // namespace nbg

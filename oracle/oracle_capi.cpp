@@ -298,6 +298,21 @@ void orc_fd_operator(int which, int n, const double* x, const double* v, const d
   for (int r = 0; r < sevn; ++r) dqdt_num[r] = (double)((op[r] - om[r]) / (2 * dq));
 }
 
+
+// Scalar pair Kepler formulas (jac_delxv_gamma! + compute_jacobian_gamma!):
+// out = delxv[6], jac_kepler[48], jac_mass[6], iterations.
+void orc_pair_kepler(const double* x0, const double* v0, double k, double h, int drift_first, double* out) {
+  State<double> s(2);
+  for (int q = 0; q < 3; ++q) { s.x0[q] = x0[q]; s.v0[q] = v0[q]; }
+  KParams<double> P = jac_delxv_gamma(s, k, h, drift_first != 0);
+  double kep[48] = {0}, jm[6] = {0};
+  compute_jacobian_gamma(P, s.x0, s.v0, kep, jm, drift_first != 0);
+  for (int q = 0; q < 6; ++q) out[q] = s.delxv[q];
+  for (int q = 0; q < 48; ++q) out[6 + q] = kep[q];
+  for (int q = 0; q < 6; ++q) out[54 + q] = jm[q];
+  out[60] = (double)P.iters;
+}
+
 // ---------------------------------------------------------------- stepping
 int orc_ahl21_step(int n, double h, int grad, double* x, double* v, double* xe, double* ve, const double* m,
                    const uint8_t* pair, double* jac, double* jace, double* dqdt) {

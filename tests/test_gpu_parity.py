@@ -1,0 +1,180 @@
+"""GPU parity: libnbg_hip.so (through the C ABI) against the CPU oracle on
+identical inputs.
+
+The kernels follow the reference's operation order with the same portable
+libm, so every comparison here is bit for bit (np.array_equal; signed zeros
+compare equal).  Sizes are small enough for the oracle to finish in seconds.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import nbodygradient_amd as nb
+from refcases import elements, integrator_case, tilt, transit_case
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _same(a, b):
+    return np.array_equal(a, b)
+
+
+def _pstate(x, v, m, t, pair=None, jac_init=None):
+    """A product State built from raw arrays."""
+    n = x.shape[1]
+    el = np.zeros((n, 7))
+    el[:, 0] = m
+    s = nb.State.__new__(nb.State)
+    s.n = n
+    s.x = np.asfortranarray(x.copy())
+    s.v = np.asfortranarray(v.copy())
+    s.t = np.array([t])
+    s.m = np.asarray(m, dtype=np.float64).copy()
+    s.jac_step = np.asfortranarray(np.eye(7 * n))
+    s.dqdt = np.zeros(7 * n)
+    s.jac_init = np.asfortranarray(np.eye(7 * n)) if jac_init is None else np.asfortranarray(jac_init.copy())
+    s.pair = np.zeros((n, n), dtype=bool, order="F") if pair is None else np.asfortranarray(pair.astype(bool))
+    s.xerror = np.zeros((3, n), order="F")
+    s.verror = np.zeros((3, n), order="F")
+    s.dqdt_error = np.zeros(7 * n)
+    s.jac_error = np.zeros((7 * n, 7 * n), order="F")
+    return s
+
+
+def _compare_state(ps, os_, jac=True):
+    assert _same(ps.x, os_.x) and _same(ps.v, os_.v)
+    assert _same(ps.xerror, os_.xerror) and _same(ps.verror, os_.verror)
+    if jac:
+        assert _same(ps.jac_step, os_.jac_step)
+        assert _same(ps.jac_error, os_.jac_error)
+        assert _same(ps.dqdt, os_.dqdt)
+
+
+@pytest.mark.parametrize("grad", [True, False])
+def test_single_step(grad):
+    say("gpu: single step grad=%s" % grad)
+    el, t0, x, v, _ = integrator_case()
+    m = el[:, 0]
+    ps = _pstate(x, v, m, t0)
+    if grad:
+        nb.ahl21(ps, nb.Derivatives(float, 3), 0.05)
+    else:
+        nb.ahl21(ps, 0.05)
+    os_ = O.OState(x, v, m, t=t0)
+    O.step(os_, 0.05, grad=grad)
+    _compare_state(ps, os_, jac=grad)
+
+
+def test_steps_and_time_target():
+    say("gpu: 100 steps + integrate_to")
+    el, t0, x, v, _ = integrator_case()
+    m = el[:, 0]
+    ps = _pstate(x, v, m, t0)
+    nb.Integrator(0.05, 0.0, 5.0)(ps, 100)
+    os_ = O.OState(x, v, m, t=t0)
+    O.integrate_steps(os_, 0.05, 100, grad=True)
+    _compare_state(ps, os_)
+    assert ps.t[0] == os_.t
+    nb.Integrator(0.05, 0.0, 5.0)(ps, t0 + 7.013)
+    O.integrate_to(os_, 0.05, t0 + 7.013, grad=True)
+    _compare_state(ps, os_)
+    assert ps.t[0] == os_.t
+    # backward (N < 0)
+    nb.Integrator(0.05, 0.0, 5.0)(ps, -37)
+    O.integrate_steps(os_, 0.05, -37, grad=True)
+    _compare_state(ps, os_)
+
+
+def test_kicked_pairs_steps():
+    # test_kickfast.jl / test_phic.jl set-up: planet-planet pairs kicked,
+    # so kickfast!, phic! and their Jacobians are active.
+    say("gpu: kicked pairs")
+    el = elements()[:4].copy()
+    el[1, 0] = el[2, 0] = el[3, 0] = 1e-3
+    n = 4
+    pair = np.ones((n, n), dtype=np.uint8)
+    for i in range(1, n):
+        pair[0, i] = pair[i, 0] = 0
+    x, v, _ = O.init_nbody(7257.93115525, el)
+    x, v = tilt(x, v, 3)
+    m = el[:, 0]
+    ps = _pstate(x, v, m, 0.0, pair=pair)
+    nb.Integrator(0.05, 0.0, 1.0)(ps, 40)
+    os_ = O.OState(x, v, m, t=0.0, pair=pair)
+    O.integrate_steps(os_, 0.05, 40, grad=True)
+    _compare_state(ps, os_)
+
+
+@pytest.mark.parametrize("n", [2, 3, 5, 8, 12])
+def test_steps_many_sizes(n):
+    say("gpu: steps n=%d" % n)
+    rng = np.random.default_rng(100 + n)
+    el = np.zeros((n, 7))
+    el[0, 0] = 1.0
+    P = 3.0
+    for i in range(1, n):
+        el[i] = [10 ** rng.uniform(-6, -4.5), P, rng.uniform(0, P), rng.uniform(-0.03, 0.03),
+                 rng.uniform(-0.03, 0.03), np.pi / 2 + rng.uniform(-0.01, 0.01), rng.uniform(-0.01, 0.01)]
+        P *= rng.uniform(1.35, 1.6)
+    x, v, jinit = O.init_nbody(0.0, el)
+    m = el[:, 0]
+    ps = _pstate(x, v, m, 0.0)
+    nb.Integrator(0.1, 0.0, 1.0)(ps, 30)
+    os_ = O.OState(x, v, m, t=0.0)
+    O.integrate_steps(os_, 0.1, 30, grad=True)
+    _compare_state(ps, os_)
+
+
+def _transit_run(el, t0, h, tmax, ti=0, grad=True, ntbv=1):
+    ic = nb.ElementsIC(t0, el.shape[0], el)
+    s = nb.State(ic)
+    tt = nb.TransitTiming(tmax, ic, ti) if ntbv == 1 else nb.TransitParameters(tmax, ic, ti)
+    nb.Integrator(h, tmax)(s, tt, grad=grad)
+    x, v, jinit = O.init_nbody(t0, el)
+    os_ = O.OState(x, v, el[:, 0], t=t0, jac_init=jinit)
+    ott, odq, odel, ocount = O.integrate_transits(os_, h, tmax, tt.ntt, ti=ti, grad=grad, ntbv=ntbv)
+    return s, tt, os_, ott, odq, odel, ocount
+
+
+@pytest.mark.parametrize("ti", [0, 1])
+@pytest.mark.parametrize("grad", [True, False])
+def test_transit_timing(ti, grad):
+    say("gpu: transit timing ti=%d grad=%s" % (ti, grad))
+    el, t0 = transit_case()
+    s, tt, os_, ott, odq, odel, ocount = _transit_run(el, t0, 0.04, 120.0, ti=ti, grad=grad)
+    assert _same(tt.count, ocount) and tt.count.sum() > 0
+    assert _same(tt.tt, ott)
+    if grad:
+        assert _same(tt.dtdq0, odq)
+        assert _same(tt.dtdelements, odel)
+        _compare_state(s, os_)
+    else:
+        assert _same(s.x, os_.x) and _same(s.v, os_.v)
+
+
+def test_transit_parameters():
+    say("gpu: transit parameters")
+    el, t0 = transit_case()
+    s, tt, os_, ott, odq, odel, ocount = _transit_run(el, t0, 0.04, 60.0, ntbv=3)
+    assert _same(tt.count, ocount)
+    assert _same(tt.ttbv, ott) and _same(tt.dtbvdq0, odq) and _same(tt.dtbvdelements, odel)
+
+
+def test_trappist_transits_recover_t0():
+    # test_findtransit.jl on the GPU path: n = 7, h = 0.01, 10 days
+    say("gpu: trappist findtransit")
+    n = 7
+    el = elements()[:n]
+    ic = nb.ElementsIC(7257.0, n, el)
+    s = nb.State(ic)
+    tt = nb.TransitTiming(10.0, ic)
+    nb.Integrator(0.01, 7257.0, 10.0)(s, tt, grad=False)
+    for i in range(1, n):
+        assert abs((el[i, 2] - tt.tt[i, 0]) / el[i, 2]) < 1e-6
