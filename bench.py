@@ -1,0 +1,213 @@
+"""Benchmark: AHL21 + Jacobian + transit timing, batched on one MI355X.
+
+Workload (default): 4096 independent systems, n ~ U{3..9} bodies, periods
+log-uniform in [2, 200] d, h = P_in/30, tmax ~ U[500, 4000] d, FP64 state and
+Jacobian, every transit time with its full d t / d elements gradient.  A
+"step" of this benchmark is one complete pass of the hot path over the whole
+batch: (intr)(s, tt; grad=true) for all 4096 systems in one persistent launch
+per body count, inputs already resident in HBM.
+
+Reported beside it:
+  plain_gpu    the one-system-per-launch form on a stratified sample, per-step
+               latency measured per n and summed over the batch's step counts;
+  cpu_baseline the CPU restatement (oracle) on a seeded sample of the same
+               systems, one system per thread on all host cores;
+  roofline     algorithmic FP64 flops W(n) = 2016 n^2 (n-1) + 126 n^3 per
+               system-step (the structure-aware Jacobian-chain work), over the
+               batch's device time, against the FP64 peak (78.6 TFLOP/s).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--systems N]
+                       [--tmax-scale S] [--no-plain] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "nbodygradient.jl_amd"))
+
+import numpy as np  # noqa: E402
+
+FP64_PEAK_TFLOPS = 78.6
+METRIC = "system-steps/s w/ Jacobian, batched vs plain GPU; p50/p99 request latency"
+
+
+def W(n):
+    return 2016.0 * n * n * (n - 1) + 126.0 * n ** 3
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def plain_latency(specs, states, per_n=2, steps_cap=1500):
+    """Per-step latency (s) of the one-system-per-launch form for each n."""
+    import nbodygradient_amd as nb
+    lat = {}
+    for n in sorted(set(s.n for s in specs)):
+        idx = [i for i, s in enumerate(specs) if s.n == n][:per_n]
+        tot_t, tot_steps = 0.0, 0
+        for i in idx:
+            sp = specs[i]
+            tmax = min(sp.tmax, steps_cap * sp.h)
+            ic = nb.ElementsIC(sp.t0, n, sp.elements)
+            s = nb.State(ic)
+            tt = nb.TransitTiming(tmax, ic)
+            t = time.perf_counter()
+            nb.Integrator(sp.h, tmax)(s, tt, grad=True)
+            tot_t += time.perf_counter() - t
+            tot_steps += int(round(tmax / sp.h)) + int(tt.count.sum())
+        lat[n] = tot_t / max(tot_steps, 1)
+        log("plain n=%d: %.1f us per system-step" % (n, 1e6 * lat[n]))
+    return lat
+
+
+def cpu_baseline(specs, states, sample, threads, cap_steps=2000):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import ctypes as C
+    import oracle_lib as O
+    from nbodygradient_amd.workloads import SystemSpec
+    idx = list(range(min(sample, len(specs))))
+    # cap each sampled system at `cap_steps` steps: the rate per system-step
+    # does not depend on the run length, and the cap keeps threads balanced
+    sp = [SystemSpec(specs[i].elements, specs[i].t0, specs[i].h, min(specs[i].tmax, cap_steps * specs[i].h))
+          for i in idx]
+    st = [states[i] for i in idx]
+    n = np.array([s.n for s in sp], dtype=np.int32)
+    off_b = np.concatenate([[0], np.cumsum(n)]).astype(np.int64)
+    off_state = (3 * off_b[:-1]).astype(np.int64)
+    off_jac = np.concatenate([[0], np.cumsum(49 * n.astype(np.int64) ** 2)])[:-1].astype(np.int64)
+    ntt = np.array([s.ntt() for s in sp], dtype=np.int64)
+    off_tt = np.concatenate([[0], np.cumsum(n * ntt)])[:-1].astype(np.int64)
+    off_dq = np.concatenate([[0], np.cumsum(7 * n.astype(np.int64) ** 2 * ntt)])[:-1].astype(np.int64)
+    h = np.array([s.h for s in sp])
+    tmax = np.array([s.tmax for s in sp])
+    t0 = np.array([s.t0 for s in sp])
+    x = np.concatenate([np.asarray(a[0]).ravel(order="F") for a in st])
+    v = np.concatenate([np.asarray(a[1]).ravel(order="F") for a in st])
+    m = np.concatenate([s.elements[:, 0] for s in sp])
+    jinit = np.concatenate([np.asarray(a[2]).ravel(order="F") for a in st])
+    tt = np.zeros(int((n * ntt).sum()))
+    dq = np.zeros(int((7 * n.astype(np.int64) ** 2 * ntt).sum()))
+    de = np.zeros_like(dq)
+    count = np.zeros(int(n.sum()), dtype=np.int64)
+    done = C.c_longlong(0)
+    I32 = C.POINTER(C.c_int)
+    secs = O.lib().orc_batch_transits(len(sp), n.ctypes.data_as(I32), O.ip(off_state), O.ip(off_jac), O.ip(off_tt),
+                                      O.ip(off_dq), O.dp(h), O.dp(tmax), O.ip(ntt), O.dp(t0), O.dp(x), O.dp(v),
+                                      O.dp(m), O.dp(jinit), O.dp(tt), O.dp(dq), O.dp(de), O.ip(count), threads,
+                                      C.byref(done))
+    return done.value, secs, len(sp)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--systems", type=int, default=4096)
+    ap.add_argument("--tmax-scale", type=float, default=1.0)
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=64)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus %d is not supported (systems are independent: a "
+              "multi-GPU run would partition the batch, see DESIGN.md)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import nbodygradient_amd as nb
+    from nbodygradient_amd import workloads as Wk
+    from nbodygradient_amd.batch import Batch
+
+    t_setup = time.perf_counter()
+    specs = Wk.c2(args.systems)
+    if args.tmax_scale != 1.0:
+        for s in specs:
+            s.tmax *= args.tmax_scale
+    states = Wk.initial_states(specs)
+    ctx = nb.default_context()
+    name, cus = ctx.info()
+    log("device %s, %d CUs; %d systems prepared in %.1fs" % (name, cus, len(specs), time.perf_counter() - t_setup))
+
+    # prime code loading with a tiny batch (not part of the workload)
+    pb = Batch(specs[:8], states[:8])
+    pb.run()
+    pb.close()
+
+    batch = Batch(specs, states, grad=True)
+    for w in range(args.warmup):
+        batch.reset()
+        batch.run()
+    ms = []
+    stats = None
+    flops = 0.0
+    for k in range(args.steps):
+        batch.reset()
+        t = time.perf_counter()
+        stats = batch.run()
+        wall = time.perf_counter() - t
+        ms.append(wall * 1e3)
+        log("step %d: %.1f ms wall, %.1f ms device, %d system-steps, %d transits" %
+            (k, wall * 1e3, stats["kernel_ms"], stats["system_steps"], stats["transits"]))
+    ms_per_step = float(np.mean(ms))
+    sys_steps = stats["system_steps"]
+    value = sys_steps / (ms_per_step / 1e3)
+    # algorithmic FP64 flops of the batch: W(n) per main step and per
+    # recorded transit (its final Jacobian step)
+    res = batch.download(derivatives=False)
+    cmax = max(int(r["count"].max()) for r in res)
+    log("download: %d systems, max count %d, transits %d" % (len(res), cmax,
+        sum(int(np.minimum(r["count"], sp.ntt()).sum()) for sp, r in zip(specs, res))))
+    for sp, r in zip(specs, res):
+        tr = int(np.minimum(r["count"], sp.ntt()).sum())
+        flops += W(sp.n) * (sp.nsteps + tr)
+    achieved = flops / (stats["kernel_ms"] / 1e3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "system-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "C2: %d systems, n~U{3..9}, P log-U[2,200] d, h=P_in/30, tmax~U[500,4000] d%s, "
+                               "Jacobian + transit times + dt/delements" %
+                               (len(specs), "" if args.tmax_scale == 1.0 else " x%g" % args.tmax_scale),
+                   "systems": len(specs), "system_steps": sys_steps, "transits": stats["transits"],
+                   "newton_iters": stats["newton_iters"], "parallelism": "1 GPU, persistent launch per n"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "note": "FP64 compute bound (VALU); MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s"},
+    }
+    if not args.no_plain:
+        lat = plain_latency(specs, states)
+        t_plain = 0.0
+        for sp, r in zip(specs, res):
+            tr = int(np.minimum(r["count"], sp.ntt()).sum())
+            t_plain += lat[sp.n] * (sp.nsteps + tr)
+        plain_rate = sys_steps / t_plain
+        out["plain_gpu"] = {"value": plain_rate, "unit": "system-steps/s",
+                            "method": "one system per launch (nbg_integrate_transits); per-n step latency from 2 "
+                                      "systems x <=1500 steps, summed over the batch"}
+        out["batched_vs_plain"] = value / plain_rate
+    if not args.no_cpu:
+        threads = os.cpu_count() or 1
+        done, secs, ns = cpu_baseline(specs, states, args.cpu_sample, threads)
+        cpu_rate = done / secs
+        out["cpu_baseline"] = {"value": cpu_rate, "unit": "system-steps/s", "cores": threads, "kind": "port",
+                               "sample": "first %d of the %d systems (seeded), each capped at 2000 steps, %d system-steps "
+                                         "in %.1f s, OpenMP one system per thread" % (ns, len(specs), done, secs)}
+        out["batched_vs_cpu"] = value / cpu_rate
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -110,6 +110,8 @@ int nbg_batch_transits(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const doubl
 
 /* Tuning knobs for the persistent launches (0 = automatic). */
 int nbg_set_launch(nbg_ctx* ctx, int32_t max_groups_per_cu, int32_t streams);
+/* Upper bound on persistent workgroups per launch (0 = no bound). */
+int nbg_set_grid_cap(nbg_ctx* ctx, int32_t grid_cap);
 
 /* Diagnostic only: evaluates the device pair formulas (nbg_pair.h) on the
  * host CPU so the test suite can compare them bit for bit with the CPU

@@ -133,6 +133,7 @@ struct nbg_ctx {
   int cu_count = 0;
   std::string name;
   int max_groups_per_cu = 0;
+  int grid_cap = 0;
   int nstreams = 4;
   std::vector<hipStream_t> streams;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -170,7 +171,8 @@ int grid_for(nbg_ctx* ctx, int nb, int nsys) {
   int per_cu = (int)(163840 / std::max<size_t>(lds, 1));
   per_cu = std::max(1, std::min(per_cu, 8));
   if (ctx->max_groups_per_cu > 0) per_cu = std::min(per_cu, ctx->max_groups_per_cu);
-  const int g = ctx->cu_count * per_cu;
+  int g = ctx->cu_count * per_cu;
+  if (ctx->grid_cap > 0) g = std::min(g, ctx->grid_cap);
   return std::max(1, std::min(g, nsys));
 }
 
@@ -402,6 +404,13 @@ int nbg_set_launch(nbg_ctx* ctx, int32_t max_groups_per_cu, int32_t streams) {
     ctx->streams.assign(streams, nullptr);
     for (auto& s : ctx->streams) NBG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   }
+  return 0;
+}
+
+int nbg_set_grid_cap(nbg_ctx* ctx, int32_t grid_cap) {
+  if (!ctx) return -1;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  ctx->grid_cap = std::max(0, grid_cap);
   return 0;
 }
 

@@ -29,14 +29,38 @@ template <int NB> struct Shape {
   static constexpr int N3 = 3 * NB, N4 = 4 * NB, N6 = 6 * NB, N7 = 7 * NB;
 };
 
+// One pair of a sweep chunk: inputs and results of the universal-variable
+// Kepler drift (phase A) and its analytic 6x8 Jacobian (phase B).
+struct PairSlot {
+  double x0[3], v0[3];
+  KParams P;
+  double delxv[6];
+  double mi, mj, mijinv, m_i, m_j;
+  int valid, i, j, pad;
+  double kep[48], jm[6];
+};
+
 template <int NB> struct Smem {
   static constexpr int N3 = 3 * NB, N4 = 4 * NB, N6 = 6 * NB, N7 = 7 * NB;
   static constexpr int JOFF = N6 * N7;
   // jac_step without mass rows (row 6b+k <-> Julia row 7b+k), then jac_error
   // in the same layout at JOFF.  2*N6*N7 >= N7*N7, so jac_init fits here too.
   double JJ[2 * N6 * N7];
-  double phi[N3 * N4];   // jac_phi block: row 3b+k (velocity k of body b), col 3d+p (x_p of d) or N3+d (m_d)
-  double dadq[N3 * N4];  // d a / d q: row 3b+k, same columns; also holds jac_kick (disjoint lifetimes)
+  static constexpr int NP = NB * (NB - 1) / 2;  // pairs per sweep
+  static constexpr int CH_FIT = (int)((2 * N3 * N4 * sizeof(double)) / sizeof(PairSlot));
+  static constexpr int CH_MIN = CH_FIT < 4 ? 4 : CH_FIT;
+  static constexpr int CH = NP < CH_MIN ? NP : (CH_MIN > 64 ? 64 : CH_MIN);  // pairs per sweep chunk
+  // Lifetimes are disjoint: jac_phi/dadq live in the force-gradient stage
+  // (dadq also holds jac_kick around the kicks), the pair slots in the sweeps.
+  union {
+    struct {
+      double phi[N3 * N4];   // jac_phi block: row 3b+k (velocity k of body b), col 3d+p (x_p of d) or N3+d (m_d)
+      double dadq[N3 * N4];  // d a / d q: row 3b+k, same columns
+    } f;
+    PairSlot slot[CH];
+  } u;
+  uint8_t plist[2][2 * (NP > 0 ? NP : 1)];  // Kepler pairs (i, j) in forward / backward sweep order
+  int npl[2];
   double jij[14 * 14];   // jac_ij, column-major as in the reference
   double dqij[14];       // dqdt_ij
   double x[N3], v[N3], xe[N3], ve[N3], m[NB];
@@ -72,11 +96,11 @@ template <int NB> __device__ inline void col_dp(int cc, int& d, int& p) {
 // ---------------------------------------------------------------------------
 // kickfast! with Jacobian (ahl21.jl:337-386): impulses for kicked pairs.
 // v (Kahan) and dqdt_kick are uniform (lane 0 stores); jac_kick is built with
-// one lane per column into S.dadq.
+// one lane per column into S.u.f.dadq.
 // ---------------------------------------------------------------------------
 template <int NB> __device__ void kick_build(Smem<NB>& S, double h, bool needQ, int lane) {
   constexpr int N3 = 3 * NB, N4 = 4 * NB;
-  double* K = S.dadq;
+  double* K = S.u.f.dadq;
   if (lane < N4)
     for (int r = 0; r < N3; ++r) K[r * N4 + lane] = 0.0;
   int myd = -1, myp = -1;
@@ -138,8 +162,8 @@ template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick,
   const bool own = lane < N4;
   int myd = -1, myp = -1;
   if (own) col_dp<NB>(lane, myd, myp);
-  double* PHI = S.phi;
-  double* D = S.dadq;
+  double* PHI = S.u.f.phi;
+  double* D = S.u.f.dadq;
   // phic!: a, dadq, jac_phi set to zero (ahl21.jl:393-399)
   if (own)
     for (int r = 0; r < N3; ++r) { PHI[r * N4 + lane] = 0.0; D[r * N4 + lane] = 0.0; }
@@ -484,83 +508,134 @@ template <int NB> __device__ inline void pair_cols(Smem<NB>& S, int i, int j, in
 }
 
 // ---------------------------------------------------------------------------
-// kepler_driftij_gamma! with Jacobian (ahl21.jl:706-760) + the caller's
-// Jacobian and dq/dt updates (ahl21.jl:29-43 / 60-76).
+// One pair sweep of ahl21! (forward: ahl21.jl:25-45, drift_first = true;
+// backward: :56-78, drift_first = false) in chunks of pairs:
+//  A. sequentially per pair: the universal-variable Kepler drift
+//     (jac_delxv_gamma!, ahl21.jl:766-890) and the Kahan state update of
+//     kepler_driftij_gamma! (:706-733), all lanes alike, lane 0 stores;
+//  B. the analytic Jacobians (compute_jacobian_gamma!, :896-1139) of all the
+//     chunk's pairs at once, one lane per pair;
+//  C. sequentially per pair: jac_ij (:734-758), the Jacobian row update
+//     (copy_submatrix!, mul!, comp_sum_matrix!, ypoc_submatrix!) with one lane
+//     per column, and the dq/dt update.
+// The state never reads the Jacobian and the Jacobian updates never read the
+// state, so running all state updates of a chunk before its Jacobian updates
+// gives the reference's results exactly.
 // ---------------------------------------------------------------------------
 template <int NB>
-__device__ void pair_step(Smem<NB>& S, int i, int j, double hh, bool drift_first, bool needJ, bool needQ, int lane) {
+__device__ void sweep(Smem<NB>& S, int dir, double hh, bool needJ, bool needQ, int lane) {
   constexpr int N7 = 7 * NB;
-  double x0[3], v0[3];
-  for (int k = 0; k < 3; ++k) {
-    x0[k] = S.x[3 * i + k] - S.x[3 * j + k];
-    v0[k] = S.v[3 * i + k] - S.v[3 * j + k];
-  }
-  const double m_i = S.m[i], m_j = S.m[j];
-  const double gm = NBG_GNEWT * (m_i + m_j);
-  if (gm != 0.0) {
-    double delxv[6];
-    const KParams P = jac_delxv_gamma(x0, v0, delxv, gm, hh, drift_first);
-    const double mijinv = 1.0 / (m_i + m_j);
-    const double mi = m_i * mijinv;
-    const double mj = m_j * mijinv;
-    if (lane == 0) {
+  constexpr int CH = Smem<NB>::CH;
+  const bool drift_first = (dir == 0);
+  const int npairs = S.npl[dir];
+  const uint8_t* pl = S.plist[dir];
+  for (int c0 = 0; c0 < npairs; c0 += CH) {
+    const int cnt = (npairs - c0) < CH ? (npairs - c0) : CH;
+    // ---- A: Kepler drifts and state updates, in sweep order
+#pragma unroll 1
+    for (int q = 0; q < cnt; ++q) {
+      const int i = pl[2 * (c0 + q)], j = pl[2 * (c0 + q) + 1];
+      PairSlot& sl = S.u.slot[q];
+      double x0[3], v0[3];
       for (int k = 0; k < 3; ++k) {
-        kahan(S.x[3 * i + k], S.xe[3 * i + k], mj * delxv[k]);
-        kahan(S.x[3 * j + k], S.xe[3 * j + k], (-mi) * delxv[k]);
+        x0[k] = S.x[3 * i + k] - S.x[3 * j + k];
+        v0[k] = S.v[3 * i + k] - S.v[3 * j + k];
       }
-      for (int k = 0; k < 3; ++k) {
-        kahan(S.v[3 * i + k], S.ve[3 * i + k], mj * delxv[3 + k]);
-        kahan(S.v[3 * j + k], S.ve[3 * j + k], (-mi) * delxv[3 + k]);
+      const double m_i = S.m[i], m_j = S.m[j];
+      const double gm = NBG_GNEWT * (m_i + m_j);
+      if (gm != 0.0) {
+        double delxv[6];
+        const KParams P = jac_delxv_gamma(x0, v0, delxv, gm, hh, drift_first);
+        const double mijinv = 1.0 / (m_i + m_j);
+        const double mi = m_i * mijinv;
+        const double mj = m_j * mijinv;
+        if (lane == 0) {
+          for (int k = 0; k < 3; ++k) {
+            kahan(S.x[3 * i + k], S.xe[3 * i + k], mj * delxv[k]);
+            kahan(S.x[3 * j + k], S.xe[3 * j + k], (-mi) * delxv[k]);
+          }
+          for (int k = 0; k < 3; ++k) {
+            kahan(S.v[3 * i + k], S.ve[3 * i + k], mj * delxv[3 + k]);
+            kahan(S.v[3 * j + k], S.ve[3 * j + k], (-mi) * delxv[3 + k]);
+          }
+          if (needJ || needQ) {
+            for (int k = 0; k < 3; ++k) { sl.x0[k] = x0[k]; sl.v0[k] = v0[k]; }
+            sl.P = P;
+            for (int k = 0; k < 6; ++k) sl.delxv[k] = delxv[k];
+            sl.mi = mi; sl.mj = mj; sl.mijinv = mijinv; sl.m_i = m_i; sl.m_j = m_j;
+          }
+        }
       }
+      if (lane == 0) { sl.valid = (gm != 0.0); sl.i = i; sl.j = j; }
+      wsync();
     }
-    if (needJ || needQ) {
-      double kep[48], jm[6];
+    if (!(needJ || needQ)) continue;
+    // ---- B: analytic Kepler-drift Jacobians, one lane per pair
+    if (lane < cnt && S.u.slot[lane].valid) {
+      PairSlot& sl = S.u.slot[lane];
+      double x0[3], v0[3], kep[48], jm[6];
+      for (int k = 0; k < 3; ++k) { x0[k] = sl.x0[k]; v0[k] = sl.v0[k]; }
       for (int q = 0; q < 48; ++q) kep[q] = 0.0;
       for (int q = 0; q < 6; ++q) jm[q] = 0.0;
+      const KParams P = sl.P;
       compute_jacobian_gamma(P, x0, v0, kep, jm, drift_first);
-      if (lane == 0) {
-        for (int q = 0; q < 196; ++q) S.jij[q] = 0.0;
-        for (int l = 0; l < 6; ++l)
-          for (int k = 0; k < 6; ++k) {
-            const double kv = kep[k + 6 * l];
-            S.jij[k + 14 * l] += mj * kv;
-            S.jij[k + 14 * (7 + l)] -= mj * kv;
-            S.jij[7 + k + 14 * l] -= mi * kv;
-            S.jij[7 + k + 14 * (7 + l)] += mi * kv;
+      for (int q = 0; q < 48; ++q) sl.kep[q] = kep[q];
+      for (int q = 0; q < 6; ++q) sl.jm[q] = jm[q];
+    }
+    wsync();
+    // ---- C: jac_ij, Jacobian rows and dq/dt, in sweep order
+#pragma unroll 1
+    for (int q = 0; q < cnt; ++q) {
+      const PairSlot& sl = S.u.slot[q];
+      const int i = sl.i, j = sl.j;
+      if (sl.valid) {
+        // (gm == 0: the reference returns before touching jac_ij / dqdt_ij and
+        // the caller reuses the previous values; S.jij / S.dqij keep them.)
+        const double mi = sl.mi, mj = sl.mj, mijinv = sl.mijinv;
+        for (int e = lane; e < 196; e += WAVE) {
+          const int r = e % 14, c = e / 14;
+          double val = 0.0;
+          if (r != 6 && r != 13) {
+            const int rk = r < 7 ? r : r - 7;
+            if (c == 6) {
+              val = r < 7 ? sl.jm[rk] * sl.m_j : (((-mj) * sl.delxv[rk]) * mijinv) - ((NBG_GNEWT * mi) * sl.kep[rk + 36]);
+            } else if (c == 13) {
+              val = r < 7 ? ((mi * sl.delxv[rk]) * mijinv) + ((NBG_GNEWT * mj) * sl.kep[rk + 36]) : (-sl.jm[rk]) * sl.m_i;
+            } else {
+              const int ck = c < 7 ? c : c - 7;
+              const double kv = sl.kep[rk + 6 * ck];
+              // jac_ij starts at zero and accumulates +-m*kep (ahl21.jl:735-742)
+              if ((r < 7) == (c < 7)) val = 0.0 + (r < 7 ? mj : mi) * kv;
+              else val = 0.0 - (r < 7 ? mj : mi) * kv;
+            }
           }
-        for (int k = 0; k < 6; ++k) {
-          S.jij[k + 14 * 6] = jm[k] * m_j;
-          S.jij[k + 14 * 13] = ((mi * delxv[k]) * mijinv) + ((NBG_GNEWT * mj) * kep[k + 36]);
-          S.jij[7 + k + 14 * 6] = (((-mj) * delxv[k]) * mijinv) - ((NBG_GNEWT * mi) * kep[k + 36]);
-          S.jij[7 + k + 14 * 13] = (-jm[k]) * m_i;
+          S.jij[e] = val;
         }
-        for (int k = 0; k < 6; ++k) {
-          S.dqij[k] = mj * kep[k + 42];
-          S.dqij[7 + k] = (-mi) * kep[k + 42];
+        if (lane < 6) {
+          S.dqij[lane] = mj * sl.kep[lane + 42];
+          S.dqij[7 + lane] = (-mi) * sl.kep[lane + 42];
         }
       }
+      wsync();
+      if (needJ)
+        for (int c = lane; c < N7; c += WAVE) pair_cols<NB>(S, i, j, c);
+      if (needQ) {
+        double q1[14];
+        for (int k = 0; k < 7; ++k) {
+          q1[k] = S.dqdt[7 * i + k];
+          q1[7 + k] = S.dqdt[7 * j + k];
+        }
+        if (lane < 14) {
+          double s2 = 0.0;
+          for (int k = 0; k < 14; ++k) s2 = fmad(S.jij[lane + 14 * k], q1[k], s2);
+          const double nd = (S.dqij[lane] * 0.5) + (q1[lane] + s2);
+          S.dqij[lane] = nd;
+          S.dqdt[lane < 7 ? 7 * i + lane : 7 * j + lane - 7] = nd;
+        }
+      }
+      wsync();
     }
   }
-  // (gm == 0: the reference returns before touching jac_ij / dqdt_ij, and the
-  // caller reuses the previous values; S.jij / S.dqij keep them likewise.)
-  wsync();
-  if (needJ)
-    for (int c = lane; c < N7; c += WAVE) pair_cols<NB>(S, i, j, c);
-  if (needQ) {
-    double q1[14];
-    for (int k = 0; k < 7; ++k) {
-      q1[k] = S.dqdt[7 * i + k];
-      q1[7 + k] = S.dqdt[7 * j + k];
-    }
-    if (lane < 14) {
-      double s = 0.0;
-      for (int k = 0; k < 14; ++k) s = fmad(S.jij[lane + 14 * k], q1[k], s);
-      const double nd = (S.dqij[lane] * 0.5) + (q1[lane] + s);
-      S.dqij[lane] = nd;
-      S.dqdt[lane < 7 ? 7 * i + lane : 7 * j + lane - 7] = nd;
-    }
-  }
-  wsync();
 }
 
 // ---------------------------------------------------------------------------
@@ -593,7 +668,7 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
       if (anyKick) {
         const int b = r / 7, k = r - 7 * b;
         if (k >= 3 && k < 6) {
-          const double* Kr = S.dadq + (3 * b + k - 3) * N4;
+          const double* Kr = S.u.f.dadq + (3 * b + k - 3) * N4;
           for (int d = 0; d < NB; ++d) {
             tmp = fmad(Kr[3 * d + 0], S.dqdt[7 * d + 0], tmp);
             tmp = fmad(Kr[3 * d + 1], S.dqdt[7 * d + 1], tmp);
@@ -613,7 +688,7 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
       load_pos<NB>(S, c, P);
       drift_cols<NB>(S, c, h2);
       zero_add_rows<NB>(S, c, true, !anyKick);
-      if (anyKick) vel_product_add<NB>(S, S.dadq, c, P);
+      if (anyKick) vel_product_add<NB>(S, S.u.f.dadq, c, P);
     }
   // ---- drift of the state (drift_grad! / drift!)
   if (lane == 0)
@@ -627,11 +702,7 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
   }
   wsync();
   // ---- forward pair sweep (drift first)
-#pragma unroll 1
-  for (int i = 0; i < NB - 1; ++i)
-#pragma unroll 1
-    for (int j = i + 1; j < NB; ++j)
-      if (!S.pair[i + NB * j]) pair_step<NB>(S, i, j, h2, true, needJ, needQ, lane);
+  sweep<NB>(S, 0, h2, needJ, needQ, lane);
   // ---- phic!, phisalpha!
   phi_build<NB>(S, h, anyKick, needQ, lane);
   wsync();
@@ -639,7 +710,7 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
     for (int c = lane; c < N7; c += WAVE) {
       double P[N3];
       load_pos<NB>(S, c, P);
-      vel_product_add<NB>(S, S.phi, c, P);
+      vel_product_add<NB>(S, S.u.f.phi, c, P);
       zero_add_rows<NB>(S, c, true, false);
     }
   if (needQ) {
@@ -650,7 +721,7 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
       double tmp = 0.0;
       const int b = r / 7, k = r - 7 * b;
       if (k >= 3 && k < 6) {
-        const double* Pr = S.phi + (3 * b + k - 3) * N4;
+        const double* Pr = S.u.f.phi + (3 * b + k - 3) * N4;
         for (int d = 0; d < NB; ++d) {
           tmp = fmad(Pr[3 * d + 0], S.dqdt[7 * d + 0], tmp);
           tmp = fmad(Pr[3 * d + 1], S.dqdt[7 * d + 1], tmp);
@@ -666,11 +737,7 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
   }
   wsync();
   // ---- backward pair sweep (drift after)
-#pragma unroll 1
-  for (int i = NB - 2; i >= 0; --i)
-#pragma unroll 1
-    for (int j = NB - 1; j >= i + 1; --j)
-      if (!S.pair[i + NB * j]) pair_step<NB>(S, i, j, h2, false, needJ, needQ, lane);
+  sweep<NB>(S, 1, h2, needJ, needQ, lane);
   // ---- drift
   if (needJ)
     for (int c = lane; c < N7; c += WAVE) drift_cols<NB>(S, c, h2);
@@ -696,7 +763,7 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
       if (anyKick) {
         const int b = r / 7, k = r - 7 * b;
         if (k >= 3 && k < 6) {
-          const double* Kr = S.dadq + (3 * b + k - 3) * N4;
+          const double* Kr = S.u.f.dadq + (3 * b + k - 3) * N4;
           for (int d = 0; d < NB; ++d) {
             tmp = fmad(Kr[3 * d + 0], S.dqdt[7 * d + 0], tmp);
             tmp = fmad(Kr[3 * d + 1], S.dqdt[7 * d + 1], tmp);
@@ -717,7 +784,7 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
       if (anyKick) {
         double P[N3];
         load_pos<NB>(S, c, P);
-        vel_product_add<NB>(S, S.dadq, c, P);
+        vel_product_add<NB>(S, S.u.f.dadq, c, P);
         zero_add_rows<NB>(S, c, true, false);
       } else {
         zero_add_rows<NB>(S, c, true, true);
@@ -859,7 +926,7 @@ __device__ void detect_transits(Smem<NB>& S, const SysDesc& D, const SysBufs& B,
       wsync();
       if (lane == 0) S.count[i] = cnt;
       wsync();
-      if (cnt <= D.ntt) {
+      if (cnt >= 1 && cnt <= D.ntt) {  // count starts at 0; the lower bound keeps tt/dq indexing in range
         const double dt0 = ((-gi) * D.hstep) / (gi - S.gsave[i]);
         findtransit<NB>(S, D, B, ti, i, dt0, lane, ws);
       }
@@ -919,6 +986,18 @@ template <int NB> __device__ void load_system(Smem<NB>& S, const SysDesc& D, con
   for (int q = lane; q < N7; q += WAVE) S.dqdt[q] = st[off_dqdt(NB) + q];
   for (int q = lane; q < NB * NB; q += WAVE) S.pair[q] = B.pair[D.off_pair + q];
   if (lane == 0) S.t = st[off_t(NB)];
+  wsync();
+  if (lane == 0) {
+    int nf = 0, nbw = 0;
+    for (int i = 0; i < NB - 1; ++i)
+      for (int j = i + 1; j < NB; ++j)
+        if (!S.pair[i + NB * j]) { S.plist[0][2 * nf] = (uint8_t)i; S.plist[0][2 * nf + 1] = (uint8_t)j; ++nf; }
+    for (int i = NB - 2; i >= 0; --i)
+      for (int j = NB - 1; j >= i + 1; --j)
+        if (!S.pair[i + NB * j]) { S.plist[1][2 * nbw] = (uint8_t)i; S.plist[1][2 * nbw + 1] = (uint8_t)j; ++nbw; }
+    S.npl[0] = nf;
+    S.npl[1] = nbw;
+  }
   if (D.grad) {
     const double* J = B.jac + D.off_jac;
     const double* JE = J + (int64_t)N7 * N7;
@@ -1007,7 +1086,13 @@ __global__ __launch_bounds__(WAVE) void nbg_run(const SysDesc* __restrict__ desc
   while (true) {
     int idx = 0;
     if (lane == 0) idx = atomicAdd(queue, 1);
-    idx = __shfl(idx, 0);
+    // readfirstlane (lane 0: every lane is active here) makes the system
+    // index, its descriptor and every branch on it wave-uniform (scalar
+    // registers, scalar branches).  With a lane shuffle instead, the compiler
+    // treats them as divergent and builds exec-masked regions around the
+    // per-system code; one such region's exit once dropped a register copy
+    // carried to the next system (it ran with an empty exec mask).
+    idx = __builtin_amdgcn_readfirstlane(idx);
     if (idx >= nsys) break;
     const SysDesc D = descs[idx];
     run_system<NB>(S, D, B, chunk, lane);

@@ -59,6 +59,7 @@ def lib():
     L.nbg_batch_download.argtypes = [VP, D, D, D, I64, D, D]
     L.nbg_batch_destroy.argtypes = [VP]
     L.nbg_set_launch.argtypes = [VP, C.c_int32, C.c_int32]
+    L.nbg_set_grid_cap.argtypes = [VP, C.c_int32]
     L.nbg_selftest_pair_host.argtypes = [D, D, C.c_double, C.c_double, C.c_int32, D]
     _lib = L
     return L
@@ -121,6 +122,9 @@ class Context:
 
     def set_launch(self, max_groups_per_cu=0, streams=0):
         self.check(lib().nbg_set_launch(self._h, int(max_groups_per_cu), int(streams)), "nbg_set_launch")
+
+    def set_grid_cap(self, grid_cap=0):
+        self.check(lib().nbg_set_grid_cap(self._h, int(grid_cap)), "nbg_set_grid_cap")
 
     def close(self):
         if getattr(self, "_h", None):

@@ -1,0 +1,101 @@
+"""Batched form: many independent systems integrated in one persistent GPU
+launch per body count (C ABI: nbg_batch_create / run / download)."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import BatchStats, dptr, i32ptr, i64ptr, u8ptr
+from .workloads import initial_states
+
+
+class Batch:
+    """A set of systems resident on the GPU.
+
+    specs: list of SystemSpec (elements, t0, h, tmax); states: optional list
+    of (x, v, jac_init) as returned by init_nbody (computed when omitted).
+    Each system runs (intr)(s, tt; grad) with TransitTiming (ntbv = 1) or
+    TransitParameters (ntbv = 3), transits measured against body ti."""
+
+    def __init__(self, specs, states=None, grad=True, ntbv=1, ti=0, pair=None, ctx=None):
+        self.ctx = ctx if ctx is not None else _lib.default_context()
+        self.specs = list(specs)
+        if states is None:
+            states = initial_states(self.specs)
+        ns = len(self.specs)
+        self.n = np.array([s.n for s in self.specs], dtype=np.int32)
+        self.h = np.array([s.h for s in self.specs], dtype=np.float64)
+        self.tmax = np.array([s.tmax for s in self.specs], dtype=np.float64)
+        self.t0 = np.array([s.t0 for s in self.specs], dtype=np.float64)
+        self.ti = np.full(ns, ti, dtype=np.int32)
+        self.ntt = np.array([s.ntt() for s in self.specs], dtype=np.int64)
+        self.grad = bool(grad)
+        self.ntbv = int(ntbv)
+        x = np.concatenate([np.asarray(st[0], order="F").ravel(order="F") for st in states])
+        v = np.concatenate([np.asarray(st[1], order="F").ravel(order="F") for st in states])
+        m = np.concatenate([s.elements[:, 0] for s in self.specs])
+        jinit = np.concatenate([np.asarray(st[2], order="F").ravel(order="F") for st in states])
+        pr = None
+        if pair is not None:
+            pr = np.concatenate([np.asarray(p, dtype=np.uint8).ravel(order="F") for p in pair])
+        self._x, self._v, self._m, self._jinit, self._pair = x, v, m, jinit, pr
+        h = C.c_void_p()
+        rc = _lib.lib().nbg_batch_create(self.ctx.handle, ns, i32ptr(self.n), dptr(self.h), dptr(self.tmax),
+                                         dptr(self.t0), i32ptr(self.ti), i64ptr(self.ntt), int(grad), self.ntbv,
+                                         dptr(x), dptr(v), dptr(m), u8ptr(pr), dptr(jinit), C.byref(h))
+        self.ctx.check(rc, "nbg_batch_create")
+        self._h = h
+        nb = self.ntbv
+        self._off_tt = np.concatenate([[0], np.cumsum(nb * self.n.astype(np.int64) * self.ntt)])
+        self._off_dq = np.concatenate([[0], np.cumsum(nb * self.n.astype(np.int64) ** 2 * 7 * self.ntt)])
+        self._off_b = np.concatenate([[0], np.cumsum(self.n.astype(np.int64))])
+
+    @property
+    def main_steps(self):
+        return int(sum(s.nsteps for s in self.specs))
+
+    def run(self):
+        st = BatchStats()
+        self.ctx.check(_lib.lib().nbg_batch_run(self._h, C.byref(st)), "nbg_batch_run")
+        return {"system_steps": st.system_steps, "transits": st.transits, "newton_iters": st.newton_iters,
+                "kernel_ms": st.kernel_ms}
+
+    def reset(self):
+        self.ctx.check(_lib.lib().nbg_batch_reset(self._h), "nbg_batch_reset")
+
+    def download(self, derivatives=True):
+        """Per-system results: list of dicts with tt, dtdq0, dtdelements, count, x, v."""
+        tt = np.zeros(int(self._off_tt[-1]))
+        dq = np.zeros(int(self._off_dq[-1])) if derivatives else None
+        de = np.zeros(int(self._off_dq[-1])) if derivatives else None
+        count = np.zeros(int(self._off_b[-1]), dtype=np.int64)
+        x = np.zeros(3 * int(self._off_b[-1]))
+        v = np.zeros(3 * int(self._off_b[-1]))
+        rc = _lib.lib().nbg_batch_download(self._h, dptr(tt), dptr(dq), dptr(de), i64ptr(count), dptr(x), dptr(v))
+        self.ctx.check(rc, "nbg_batch_download")
+        out = []
+        nb = self.ntbv
+        for s in range(len(self.specs)):
+            n, ntt = int(self.n[s]), int(self.ntt[s])
+            shp_tt = (n, ntt) if nb == 1 else (3, n, ntt)
+            shp_dq = (n, ntt, 7, n) if nb == 1 else (3, n, ntt, 7, n)
+            r = {"tt": tt[self._off_tt[s]:self._off_tt[s + 1]].reshape(shp_tt, order="F"),
+                 "count": count[self._off_b[s]:self._off_b[s + 1]].copy(),
+                 "x": x[3 * self._off_b[s]:3 * self._off_b[s + 1]].reshape((3, n), order="F"),
+                 "v": v[3 * self._off_b[s]:3 * self._off_b[s + 1]].reshape((3, n), order="F")}
+            if derivatives:
+                r["dtdq0"] = dq[self._off_dq[s]:self._off_dq[s + 1]].reshape(shp_dq, order="F")
+                r["dtdelements"] = de[self._off_dq[s]:self._off_dq[s + 1]].reshape(shp_dq, order="F")
+            out.append(r)
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.lib().nbg_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -423,11 +423,9 @@ def init_nbody(ic):
                 dAinvdm[r, c, k] = s
     for ii in range(n):
         for k in range(n):
+            # row-wise: the same multiply and add per element, k ascending
             aik = Ainv[ii, k]
-            for l in range(sevn):
-                for j in range(3):
-                    jac_init[7 * ii + j, l] += aik * jk[6 * k + j, l]
-                    jac_init[7 * ii + 3 + j, l] += aik * jk[6 * k + 3 + j, l]
+            jac_init[7 * ii:7 * ii + 6, :] += aik * jk[6 * k:6 * k + 6, :]
         for k in range(n):
             for j in range(3):
                 sx = 0.0
