@@ -111,7 +111,7 @@ def main():
     ap.add_argument("--tmax-scale", type=float, default=1.0)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample", type=int, default=64)
+    ap.add_argument("--cpu-sample", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (systems are independent: a "
@@ -199,7 +199,8 @@ def main():
                                       "systems x <=1500 steps, summed over the batch"}
         out["batched_vs_plain"] = value / plain_rate
     if not args.no_cpu:
-        threads = os.cpu_count() or 1
+        # one thread per host core of this process's CPU share (at most 16)
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
         done, secs, ns = cpu_baseline(specs, states, args.cpu_sample, threads)
         cpu_rate = done / secs
         out["cpu_baseline"] = {"value": cpu_rate, "unit": "system-steps/s", "cores": threads, "kind": "port",

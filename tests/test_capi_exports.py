@@ -1,0 +1,50 @@
+"""The C-ABI library loads without a GPU and exports every entry point that
+include/nbg.h declares; without a usable device the product path raises
+instead of falling back to the CPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+import nbodygradient_amd as nb
+from nbodygradient_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "nbg.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+    names = re.findall(r"\b(nbg_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+def test_header_declares_the_entry_points():
+    names = declared_functions()
+    for required in ("nbg_ctx_create", "nbg_ahl21_step", "nbg_integrate_steps", "nbg_integrate_to",
+                     "nbg_integrate_transits", "nbg_batch_create", "nbg_batch_run", "nbg_batch_download"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libnbg_hip.so not built")
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(L, n)]
+    assert not missing, "declared in include/nbg.h but not exported: %s" % missing
+
+
+def test_no_device_raises():
+    """On a host without a usable GPU the context cannot be created: the
+    integration entry points have no CPU fallback."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libnbg_hip.so not built")
+    try:
+        ctx = nb.Context(0)
+    except nb.NbgError as e:
+        assert "no usable HIP device" in str(e)
+        return
+    ctx.close()
+    pytest.skip("a GPU is present; the no-device path is not reachable here")
