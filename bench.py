@@ -143,7 +143,6 @@ def main():
         batch.run()
     ms = []
     stats = None
-    flops = 0.0
     for k in range(args.steps):
         batch.reset()
         t = time.perf_counter()
@@ -155,16 +154,22 @@ def main():
     ms_per_step = float(np.mean(ms))
     sys_steps = stats["system_steps"]
     value = sys_steps / (ms_per_step / 1e3)
-    # algorithmic FP64 flops of the batch: W(n) per main step and per
-    # recorded transit (its final Jacobian step)
     res = batch.download(derivatives=False)
-    cmax = max(int(r["count"].max()) for r in res)
-    log("download: %d systems, max count %d, transits %d" % (len(res), cmax,
-        sum(int(np.minimum(r["count"], sp.ntt()).sum()) for sp, r in zip(specs, res))))
-    for sp, r in zip(specs, res):
-        tr = int(np.minimum(r["count"], sp.ntt()).sum())
-        flops += W(sp.n) * (sp.nsteps + tr)
-    achieved = flops / (stats["kernel_ms"] / 1e3) / 1e12
+    # per-launch roofline: algorithmic FP64 flops W(n) x units of the launch for
+    # n (main steps + one per recorded transit) over that launch's duration,
+    # timed with HIP events on the stream it ran on
+    launches = {}
+    for n, lms in stats["launch_ms"].items():
+        units = stats["launch_units"][n]
+        launches[str(n)] = {"ms": lms, "units": units, "tflops": W(n) * units / (lms / 1e3) / 1e12,
+                            "kernel": "nbg_run<%d>" % n}
+    dom = max(launches, key=lambda k: launches[k]["ms"])
+    achieved = launches[dom]["tflops"]
+    total_flops = sum(W(int(n)) * L["units"] for n, L in launches.items())
+    log("download: %d systems, max count %d" % (len(res), max(int(r["count"].max()) for r in res)))
+    for n in sorted(launches, key=int):
+        L = launches[n]
+        log("launch n=%s: %.1f ms, %d units, %.3f TFLOP/s" % (n, L["ms"], L["units"], L["tflops"]))
     out = {
         "metric": METRIC,
         "value": value,
@@ -185,7 +190,12 @@ def main():
                    "newton_iters": stats["newton_iters"], "parallelism": "1 GPU, persistent launch per n"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "note": "FP64 compute bound (VALU); MI355X FP64 vector and matrix peaks are both 78.6 TFLOP/s"},
+                     "kernel": launches[dom]["kernel"], "launch_ms": launches[dom]["ms"],
+                     "note": "FP64 compute bound (VALU); MI355X FP64 vector and matrix peaks are both 78.6 "
+                             "TFLOP/s; W(n) = 2016 n^2 (n-1) + 126 n^3 flops per system-step"},
+        "launches": launches,
+        "batch_tflops": total_flops / (stats["kernel_ms"] / 1e3) / 1e12,
+        "transits_per_s": stats["transits"] / (ms_per_step / 1e3),
     }
     if not args.no_plain:
         lat = plain_latency(specs, states)

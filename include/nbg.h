@@ -84,7 +84,12 @@ typedef struct {
   int64_t system_steps;    /* main steps + one per recorded transit (final gradient step) */
   int64_t transits;        /* transits recorded (count <= ntt) */
   int64_t newton_iters;    /* findtransit! Newton iterations */
-  double kernel_ms;        /* device time of the last run (HIP events, all groups) */
+  double kernel_ms;        /* device time of the last run (HIP events, all launches) */
+  /* Per body count n (index n = 2..12; other entries 0): device time of the
+   * launch for that n, from HIP events on the stream it ran on, and its work
+   * units (main steps + one per recorded transit when grad != 0). */
+  double launch_ms[13];
+  int64_t launch_units[13];
 } nbg_batch_stats;
 
 int nbg_batch_create(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* tmax,

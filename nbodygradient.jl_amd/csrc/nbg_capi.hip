@@ -137,6 +137,7 @@ struct nbg_ctx {
   int nstreams = 4;
   std::vector<hipStream_t> streams;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  hipEvent_t gev[2 * 16] = {};  // start/end of each body-count launch (timed on its own stream)
   Store single;  // reused by the single-system entry points
 };
 
@@ -206,7 +207,8 @@ nbg::SysBufs bufs_of(Store& S) {
 }
 
 // Launch every group (one persistent kernel per body count) and wait.
-int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* kernel_ms) {
+int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* kernel_ms,
+               std::vector<float>* group_ms = nullptr) {
   NBG_HIP(hipSetDevice(ctx->device));
   NBG_HIP(hipMemset(S.queue.p, 0, sizeof(int) * (L.gn.size() + 1)));
   nbg::SysBufs B = bufs_of(S);
@@ -216,7 +218,10 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
   for (size_t g = 0; g < L.gn.size(); ++g) {
     hipStream_t st = ctx->streams[g % ns];
     const int grid = grid_for(ctx, L.gn[g], L.gcnt[g]);
+    const bool timed = group_ms && g < 16;
+    if (timed) NBG_HIP(hipEventRecord(ctx->gev[2 * g], st));
     NBG_HIP(nbg_launch_group(L.gn[g], S.desc.p + L.goff[g], L.gcnt[g], S.queue.p + g, B, chunk, grid, st));
+    if (timed) NBG_HIP(hipEventRecord(ctx->gev[2 * g + 1], st));
   }
   for (int s = 0; s < ns; ++s) {
     hipEvent_t e;
@@ -232,6 +237,11 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
     float ms = 0;
     NBG_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     *kernel_ms = ms;
+  }
+  if (group_ms) {
+    group_ms->assign(L.gn.size(), 0.0f);
+    for (size_t g = 0; g < L.gn.size() && g < 16; ++g)
+      NBG_HIP(hipEventElapsedTime(&(*group_ms)[g], ctx->gev[2 * g], ctx->gev[2 * g + 1]));
   }
   return 0;
 }
@@ -366,6 +376,8 @@ int nbg_ctx_create(nbg_ctx** out, int device) {
   for (auto& s : ctx->streams)
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) { delete ctx; return -3; }
   if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) { delete ctx; return -3; }
+  for (auto& e : ctx->gev)
+    if (hipEventCreate(&e) != hipSuccess) { delete ctx; return -3; }
   *out = ctx;
   return 0;
 }
@@ -378,6 +390,8 @@ void nbg_ctx_destroy(nbg_ctx* ctx) {
   for (auto& s : ctx->streams) (void)hipStreamDestroy(s);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+  for (auto& e : ctx->gev)
+    if (e) (void)hipEventDestroy(e);
   delete ctx;
 }
 
@@ -595,19 +609,24 @@ int nbg_batch_run(nbg_batch* b, nbg_batch_stats* stats) {
   nbg_ctx* ctx = b->ctx;
   std::lock_guard<std::mutex> lk(ctx->mu);
   double ms = 0.0;
-  if (int rc = run_groups(ctx, b->S, b->L, 0, &ms)) return rc;
+  std::vector<float> gms;
+  if (int rc = run_groups(ctx, b->S, b->L, 0, &ms, &gms)) return rc;
   if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
     std::vector<int64_t> hi((size_t)b->L.n_int);
     NBG_HIP(hipMemcpy(hi.data(), b->S.ints.p, hi.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
     int64_t tr = 0, it = 0;
     for (const auto& d : b->L.desc) {
-      tr += hi[d.off_int + d.n + 1];
+      const int64_t t_d = hi[d.off_int + d.n + 1];
+      tr += t_d;
       it += hi[d.off_int + d.n + 2];
+      stats->launch_units[d.n] += d.nsteps + (d.grad ? t_d : 0);
     }
     stats->transits = tr;
     stats->newton_iters = it;
     stats->system_steps = b->steps_main + (b->L.desc.empty() || !b->L.desc[0].grad ? 0 : tr);
     stats->kernel_ms = ms;
+    for (size_t g = 0; g < b->L.gn.size() && g < gms.size(); ++g) stats->launch_ms[b->L.gn[g]] = gms[g];
   }
   return 0;
 }

@@ -59,8 +59,12 @@ template <int NB> struct Smem {
     } f;
     PairSlot slot[CH];
   } u;
-  uint8_t plist[2][2 * (NP > 0 ? NP : 1)];  // Kepler pairs (i, j) in forward / backward sweep order
-  int npl[2];
+  // Kepler pairs (i, j) of the forward / backward sweep grouped by level:
+  // a pair's level is one more than the latest earlier pair sharing a body,
+  // so the pairs of a level are disjoint; lvw holds the level widths.
+  uint8_t plist[2][2 * (NP > 0 ? NP : 1)];
+  uint8_t lvw[2][NP > 0 ? NP : 1];
+  int nlv[2];
   double jij[14 * 14];   // jac_ij, column-major as in the reference
   double dqij[14];       // dqdt_ij
   double x[N3], v[N3], xe[N3], ve[N3], m[NB];
@@ -81,6 +85,39 @@ __device__ inline void kahan(double& s, double& e, double add) {
   const double t = s + e;
   e = (s - t) + e;
   s = t;
+}
+
+// ---------------------------------------------------------------------------
+// The pair Kepler drift (jac_delxv_gamma!, ahl21.jl:766-890) and its analytic
+// Jacobian (compute_jacobian_gamma!, ahl21.jl:896-1139) as single out-of-line
+// copies shared by every body count's kernel: inlined, each would be
+// duplicated per kernel and per call site, and the hot loop of a step would
+// no longer fit the instruction cache.  Inputs and outputs pass through the
+// pair's slot in LDS; the arithmetic runs in registers.
+// ---------------------------------------------------------------------------
+__device__ __attribute__((noinline)) void pair_drift(PairSlot* sl, double gm, double hh, bool drift_first) {
+  double x0[3], v0[3], delxv[6];
+  for (int k = 0; k < 3; ++k) {
+    x0[k] = sl->x0[k];
+    v0[k] = sl->v0[k];
+  }
+  const KParams P = jac_delxv_gamma(x0, v0, delxv, gm, hh, drift_first);
+  sl->P = P;
+  for (int k = 0; k < 6; ++k) sl->delxv[k] = delxv[k];
+}
+
+__device__ __attribute__((noinline)) void pair_jacobian(PairSlot* sl, bool drift_first) {
+  double x0[3], v0[3], kep[48], jm[6];
+  for (int k = 0; k < 3; ++k) {
+    x0[k] = sl->x0[k];
+    v0[k] = sl->v0[k];
+  }
+  for (int q = 0; q < 48; ++q) kep[q] = 0.0;
+  for (int q = 0; q < 6; ++q) jm[q] = 0.0;
+  const KParams P = sl->P;
+  compute_jacobian_gamma(P, x0, v0, kep, jm, drift_first);
+  for (int q = 0; q < 48; ++q) sl->kep[q] = kep[q];
+  for (int q = 0; q < 6; ++q) sl->jm[q] = jm[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -164,12 +201,14 @@ template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick,
   if (own) col_dp<NB>(lane, myd, myp);
   double* PHI = S.u.f.phi;
   double* D = S.u.f.dadq;
-  // phic!: a, dadq, jac_phi set to zero (ahl21.jl:393-399)
-  if (own)
-    for (int r = 0; r < N3; ++r) { PHI[r * N4 + lane] = 0.0; D[r * N4 + lane] = 0.0; }
-  if (lane == 0)
-    for (int r = 0; r < N3; ++r) S.a[r] = 0.0;
-  wsync();
+  if (anyKick) {
+    // phic!: a, dadq, jac_phi set to zero (ahl21.jl:393-399)
+    if (own)
+      for (int r = 0; r < N3; ++r) { PHI[r * N4 + lane] = 0.0; D[r * N4 + lane] = 0.0; }
+    if (lane == 0)
+      for (int r = 0; r < N3; ++r) S.a[r] = 0.0;
+    wsync();
+  }
   if (anyKick) {
     const double coeff = ((h * h * h) / 36.0) * NBG_GNEWT;
     // loop 1: kicks 2h/3, accelerations, dadq (ahl21.jl:403-466)
@@ -298,16 +337,24 @@ template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick,
         wsync();
       }
   }
-  // phisalpha!: a, dadq zeroed (ahl21.jl:559-563)
-  if (own)
-    for (int r = 0; r < N3; ++r) D[r * N4 + lane] = 0.0;
-  if (lane == 0)
-    for (int r = 0; r < N3; ++r) S.a[r] = 0.0;
-  wsync();
+  // phisalpha! (ahl21.jl:558-700, alpha = 2) for the Kepler pairs, one row
+  // block (the three rows of body b) at a time: a lane keeps the block's rows
+  // of its dadq and jac_phi columns in registers while it accumulates them.
+  // Body b's rows receive contributions only from the pairs that contain b,
+  // and in the reference's lexicographic pair order those come partner by
+  // partner in increasing order; looping over the partner c in increasing
+  // order therefore repeats every accumulation in the reference's order.
+  // a and dadq start at zero (ahl21.jl:559-563); jac_phi keeps phic!'s terms.
   const double coeff = (((2.0 * (h * h * h)) / 96.0) * 2.0) * NBG_GNEWT;
-  // loop 1: accelerations and dadq of Kepler pairs (ahl21.jl:567-602)
-  for (int i = 0; i < NB - 1; ++i)
-    for (int j = i + 1; j < NB; ++j) {
+  // loop 1: accelerations and dadq (ahl21.jl:567-602)
+#pragma unroll 1
+  for (int b = 0; b < NB; ++b) {
+    double ab[3] = {0.0, 0.0, 0.0};
+    double db[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int c = 0; c < NB; ++c) {
+      if (c == b) continue;
+      const int i = c < b ? c : b, j = c < b ? b : c;
       if (S.pair[i + NB * j]) continue;
       double rij[3];
       for (int k = 0; k < 3; ++k) rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
@@ -315,97 +362,130 @@ template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick,
       const double r3 = r2 * __builtin_sqrt(r2);
       const double fac2 = NBG_GNEWT / r3;
       const double mi = S.m[i], mj = S.m[j];
+      const double rmy = myp == 0 ? rij[0] : (myp == 1 ? rij[1] : rij[2]);
+      const bool bi = (b == i);
+#pragma unroll
       for (int k = 0; k < 3; ++k) {
         const double fac = fac2 * rij[k];
-        const int ri = 3 * i + k, rj = 3 * j + k;
-        if (lane == 0) {
-          S.a[ri] -= mj * fac;
-          S.a[rj] += mi * fac;
+        const double f3 = fac * (3.0 / r2);
+        double d = db[k];
+        if (bi) {
+          ab[k] -= mj * fac;
+          if (myp == 3 && myd == j) d -= fac;
+          if (myp < 3 && myd == i) d += (f3 * mj) * rmy;
+          if (myp < 3 && myd == j) d -= (f3 * mj) * rmy;
+          if (myp == k && myd == i) d -= fac2 * mj;
+          if (myp == k && myd == j) d += fac2 * mj;
+        } else {
+          ab[k] += mi * fac;
+          if (myp == 3 && myd == i) d += fac;
+          if (myp < 3 && myd == i) d -= (f3 * mi) * rmy;
+          if (myp < 3 && myd == j) d += (f3 * mi) * rmy;
+          if (myp == k && myd == i) d += fac2 * mi;
+          if (myp == k && myd == j) d -= fac2 * mi;
         }
-        if (own) {
-          if (myp == 3 && myd == j) D[ri * N4 + lane] -= fac;
-          if (myp == 3 && myd == i) D[rj * N4 + lane] += fac;
-          const double f3 = fac * (3.0 / r2);
-          if (myp < 3 && myd == i) {
-            D[ri * N4 + lane] += (f3 * mj) * rij[myp];
-            D[rj * N4 + lane] -= (f3 * mi) * rij[myp];
-          }
-          if (myp < 3 && myd == j) {
-            D[ri * N4 + lane] -= (f3 * mj) * rij[myp];
-            D[rj * N4 + lane] += (f3 * mi) * rij[myp];
-          }
-          if (myp == k && myd == i) {
-            D[ri * N4 + lane] -= fac2 * mj;
-            D[rj * N4 + lane] += fac2 * mi;
-          }
-          if (myp == k && myd == j) {
-            D[ri * N4 + lane] += fac2 * mj;
-            D[rj * N4 + lane] -= fac2 * mi;
-          }
-        }
+        db[k] = d;
       }
     }
+    if (own)
+      for (int k = 0; k < 3; ++k) D[(3 * b + k) * N4 + lane] = db[k];
+    if (lane == 0)
+      for (int k = 0; k < 3; ++k) S.a[3 * b + k] = ab[k];
+  }
   wsync();
-  // loop 2: velocity corrections and jac_phi (ahl21.jl:607-698)
-  for (int i = 0; i < NB - 1; ++i)
-    for (int j = i + 1; j < NB; ++j) {
+  // loop 2: velocity corrections, dqdt_phi and jac_phi (ahl21.jl:607-698)
+  const double q3h = 3.0 / h;
+#pragma unroll 1
+  for (int b = 0; b < NB; ++b) {
+    double vb[3], veb[3], qb[3], pb[3];
+    for (int k = 0; k < 3; ++k) {
+      vb[k] = S.v[3 * b + k];
+      veb[k] = S.ve[3 * b + k];
+      qb[k] = needQ ? S.dqdt_phi[7 * b + 3 + k] : 0.0;
+      pb[k] = (anyKick && own) ? PHI[(3 * b + k) * N4 + lane] : 0.0;
+    }
+#pragma unroll 1
+    for (int c = 0; c < NB; ++c) {
+      if (c == b) continue;
+      const int i = c < b ? c : b, j = c < b ? b : c;
       if (S.pair[i + NB * j]) continue;
-      double rij[3], aij[3];
+      double rij[3], aij[3], Di[3], Dj[3];
       for (int k = 0; k < 3; ++k) {
         aij[k] = S.a[3 * i + k] - S.a[3 * j + k];
         rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
+        Di[k] = own ? D[(3 * i + k) * N4 + lane] : 0.0;
+        Dj[k] = own ? D[(3 * j + k) * N4 + lane] : 0.0;
       }
       double dot = 0.0;
       if (own)
-        for (int k = 0; k < 3; ++k) dot += rij[k] * (D[(3 * i + k) * N4 + lane] - D[(3 * j + k) * N4 + lane]);
+        for (int k = 0; k < 3; ++k) dot += rij[k] * (Di[k] - Dj[k]);
       const double r2 = dot3(rij);
       const double r1 = __builtin_sqrt(r2);
       const double ardot = dot3(aij, rij);
       const double fac1 = coeff / ((r2 * r2) * r1);
       const double mi = S.m[i], mj = S.m[j];
       const double fac2 = ((2.0 * NBG_GNEWT) * (mi + mj)) / r1 + 3.0 * ardot;
+      const double rmy = myp == 0 ? rij[0] : (myp == 1 ? rij[1] : rij[2]);
+      const double amy = myp == 0 ? aij[0] : (myp == 1 ? aij[1] : aij[2]);
+      const bool ij = myp < 3 && (myd == i || myd == j);
+      const bool bi = (b == i);
+#pragma unroll
       for (int k = 0; k < 3; ++k) {
         const double fac = fac1 * (rij[k] * fac2 - r2 * aij[k]);
-        const int ri = 3 * i + k, rj = 3 * j + k;
-        if (lane == 0) {
-          kahan(S.v[ri], S.ve[ri], mj * fac);
-          kahan(S.v[rj], S.ve[rj], (-mi) * fac);
-          if (needQ) {
-            S.dqdt_phi[7 * i + 3 + k] += ((3.0 / h) * mj) * fac;
-            S.dqdt_phi[7 * j + 3 + k] -= ((3.0 / h) * mi) * fac;
+        const double f5 = fac * (5.0 / r2);
+        const double fm = (((2.0 * NBG_GNEWT) * fac1) * rij[k]) / r1;
+        const double fd = fac1 * fac2;
+        const double fr = ((-2.0) * fac1) * ((((rij[k] * NBG_GNEWT) * (mi + mj)) / (r2 * r1)) + aij[k]);
+        const double fac3 = ij ? fr * rmy + (((fac1 * 3.0) * rij[k]) * amy) : 0.0;
+        const double fa = (-fac1) * r2;
+        const double diff = Di[k] - Dj[k];
+        const double ff = (3.0 * fac1) * rij[k];
+        double pv = pb[k];
+        if (bi) {
+          kahan(vb[k], veb[k], mj * fac);
+          if (needQ) qb[k] += (q3h * mj) * fac;
+          if (myp == 3 && myd == j) pv += fac;
+          if (myp < 3 && myd == i) pv -= (f5 * mj) * rmy;
+          if (myp < 3 && myd == j) pv += (f5 * mj) * rmy;
+          if (myp == 3 && myd == i) pv += fm * mj;
+          if (myp == 3 && myd == j) pv += fm * mj;
+          if (myp == k && myd == i) pv += fd * mj;
+          if (myp == k && myd == j) pv -= fd * mj;
+          if (ij) {
+            if (myd == i) pv += mj * fac3;
+            else pv -= mj * fac3;
           }
-        }
-        if (own) {
-          double* pi_ = &PHI[ri * N4 + lane];
-          double* pj_ = &PHI[rj * N4 + lane];
-          if (myp == 3 && myd == j) *pi_ += fac;
-          if (myp == 3 && myd == i) *pj_ -= fac;
-          const double f5 = fac * (5.0 / r2);
-          if (myp < 3 && myd == i) { *pi_ -= (f5 * mj) * rij[myp]; *pj_ += (f5 * mi) * rij[myp]; }
-          if (myp < 3 && myd == j) { *pi_ += (f5 * mj) * rij[myp]; *pj_ -= (f5 * mi) * rij[myp]; }
-          const double fm = (((2.0 * NBG_GNEWT) * fac1) * rij[k]) / r1;
-          if (myp == 3 && myd == i) { *pi_ += fm * mj; *pj_ -= fm * mi; }
-          if (myp == 3 && myd == j) { *pi_ += fm * mj; *pj_ -= fm * mi; }
-          const double fd = fac1 * fac2;
-          if (myp == k && myd == i) { *pi_ += fd * mj; *pj_ -= fd * mi; }
-          if (myp == k && myd == j) { *pi_ -= fd * mj; *pj_ += fd * mi; }
-          const double fr = ((-2.0) * fac1) * ((((rij[k] * NBG_GNEWT) * (mi + mj)) / (r2 * r1)) + aij[k]);
-          if (myp < 3 && (myd == i || myd == j)) {
-            const double fac3 = fr * rij[myp] + (((fac1 * 3.0) * rij[k]) * aij[myp]);
-            if (myd == i) { *pi_ += mj * fac3; *pj_ -= mi * fac3; }
-            else { *pi_ -= mj * fac3; *pj_ += mi * fac3; }
+          pv += (fa * mj) * diff;
+          pv += (ff * mj) * dot;
+        } else {
+          kahan(vb[k], veb[k], (-mi) * fac);
+          if (needQ) qb[k] -= (q3h * mi) * fac;
+          if (myp == 3 && myd == i) pv -= fac;
+          if (myp < 3 && myd == i) pv += (f5 * mi) * rmy;
+          if (myp < 3 && myd == j) pv -= (f5 * mi) * rmy;
+          if (myp == 3 && myd == i) pv -= fm * mi;
+          if (myp == 3 && myd == j) pv -= fm * mi;
+          if (myp == k && myd == i) pv -= fd * mi;
+          if (myp == k && myd == j) pv += fd * mi;
+          if (ij) {
+            if (myd == i) pv -= mi * fac3;
+            else pv += mi * fac3;
           }
-          const double fa = (-fac1) * r2;
-          const double diff = D[ri * N4 + lane] - D[rj * N4 + lane];
-          *pi_ += (fa * mj) * diff;
-          *pj_ -= (fa * mi) * diff;
-          const double ff = (3.0 * fac1) * rij[k];
-          *pi_ += (ff * mj) * dot;
-          *pj_ -= (ff * mi) * dot;
+          pv -= (fa * mi) * diff;
+          pv -= (ff * mi) * dot;
         }
+        pb[k] = pv;
       }
-      wsync();
     }
+    if (own)
+      for (int k = 0; k < 3; ++k) PHI[(3 * b + k) * N4 + lane] = pb[k];
+    if (lane == 0)
+      for (int k = 0; k < 3; ++k) {
+        S.v[3 * b + k] = vb[k];
+        S.ve[3 * b + k] = veb[k];
+        if (needQ) S.dqdt_phi[7 * b + 3 + k] = qb[k];
+      }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -527,61 +607,56 @@ __device__ void sweep(Smem<NB>& S, int dir, double hh, bool needJ, bool needQ, i
   constexpr int N7 = 7 * NB;
   constexpr int CH = Smem<NB>::CH;
   const bool drift_first = (dir == 0);
-  const int npairs = S.npl[dir];
+  const int nlev = S.nlv[dir];
   const uint8_t* pl = S.plist[dir];
-  for (int c0 = 0; c0 < npairs; c0 += CH) {
-    const int cnt = (npairs - c0) < CH ? (npairs - c0) : CH;
-    // ---- A: Kepler drifts and state updates, in sweep order
+  const uint8_t* lw = S.lvw[dir];
+  int lev = 0, c0 = 0;
+  while (lev < nlev) {
+    // a chunk is a run of whole levels holding at most CH pairs
+    int cnt = 0, lev_end = lev;
+    while (lev_end < nlev && cnt + lw[lev_end] <= CH) cnt += lw[lev_end++];
+    // ---- A: Kepler drifts and state updates, one level at a time: the
+    // pairs of a level share no body, so lane l solves pair l of the level
+    int q0 = 0;
 #pragma unroll 1
-    for (int q = 0; q < cnt; ++q) {
-      const int i = pl[2 * (c0 + q)], j = pl[2 * (c0 + q) + 1];
-      PairSlot& sl = S.u.slot[q];
-      double x0[3], v0[3];
-      for (int k = 0; k < 3; ++k) {
-        x0[k] = S.x[3 * i + k] - S.x[3 * j + k];
-        v0[k] = S.v[3 * i + k] - S.v[3 * j + k];
-      }
-      const double m_i = S.m[i], m_j = S.m[j];
-      const double gm = NBG_GNEWT * (m_i + m_j);
-      if (gm != 0.0) {
-        double delxv[6];
-        const KParams P = jac_delxv_gamma(x0, v0, delxv, gm, hh, drift_first);
-        const double mijinv = 1.0 / (m_i + m_j);
-        const double mi = m_i * mijinv;
-        const double mj = m_j * mijinv;
-        if (lane == 0) {
-          for (int k = 0; k < 3; ++k) {
-            kahan(S.x[3 * i + k], S.xe[3 * i + k], mj * delxv[k]);
-            kahan(S.x[3 * j + k], S.xe[3 * j + k], (-mi) * delxv[k]);
-          }
-          for (int k = 0; k < 3; ++k) {
-            kahan(S.v[3 * i + k], S.ve[3 * i + k], mj * delxv[3 + k]);
-            kahan(S.v[3 * j + k], S.ve[3 * j + k], (-mi) * delxv[3 + k]);
-          }
-          if (needJ || needQ) {
-            for (int k = 0; k < 3; ++k) { sl.x0[k] = x0[k]; sl.v0[k] = v0[k]; }
-            sl.P = P;
-            for (int k = 0; k < 6; ++k) sl.delxv[k] = delxv[k];
-            sl.mi = mi; sl.mj = mj; sl.mijinv = mijinv; sl.m_i = m_i; sl.m_j = m_j;
-          }
+    for (int L = lev; L < lev_end; ++L) {
+      const int w = lw[L];
+      const bool mine = lane < w;
+      if (mine) {
+        const int q = q0 + lane;
+        const int i = pl[2 * (c0 + q)], j = pl[2 * (c0 + q) + 1];
+        PairSlot& sl = S.u.slot[q];
+        for (int k = 0; k < 3; ++k) {
+          sl.x0[k] = S.x[3 * i + k] - S.x[3 * j + k];
+          sl.v0[k] = S.v[3 * i + k] - S.v[3 * j + k];
         }
+        const double m_i = S.m[i], m_j = S.m[j];
+        const double gm = NBG_GNEWT * (m_i + m_j);
+        if (gm != 0.0) {
+          pair_drift(&sl, gm, hh, drift_first);
+          const double mijinv = 1.0 / (m_i + m_j);
+          const double mi = m_i * mijinv;
+          const double mj = m_j * mijinv;
+          for (int k = 0; k < 3; ++k) {
+            kahan(S.x[3 * i + k], S.xe[3 * i + k], mj * sl.delxv[k]);
+            kahan(S.x[3 * j + k], S.xe[3 * j + k], (-mi) * sl.delxv[k]);
+          }
+          for (int k = 0; k < 3; ++k) {
+            kahan(S.v[3 * i + k], S.ve[3 * i + k], mj * sl.delxv[3 + k]);
+            kahan(S.v[3 * j + k], S.ve[3 * j + k], (-mi) * sl.delxv[3 + k]);
+          }
+          sl.mi = mi; sl.mj = mj; sl.mijinv = mijinv; sl.m_i = m_i; sl.m_j = m_j;
+        }
+        sl.valid = (gm != 0.0); sl.i = i; sl.j = j;
       }
-      if (lane == 0) { sl.valid = (gm != 0.0); sl.i = i; sl.j = j; }
+      q0 += w;
       wsync();
     }
+    lev = lev_end;
+    c0 += cnt;
     if (!(needJ || needQ)) continue;
     // ---- B: analytic Kepler-drift Jacobians, one lane per pair
-    if (lane < cnt && S.u.slot[lane].valid) {
-      PairSlot& sl = S.u.slot[lane];
-      double x0[3], v0[3], kep[48], jm[6];
-      for (int k = 0; k < 3; ++k) { x0[k] = sl.x0[k]; v0[k] = sl.v0[k]; }
-      for (int q = 0; q < 48; ++q) kep[q] = 0.0;
-      for (int q = 0; q < 6; ++q) jm[q] = 0.0;
-      const KParams P = sl.P;
-      compute_jacobian_gamma(P, x0, v0, kep, jm, drift_first);
-      for (int q = 0; q < 48; ++q) sl.kep[q] = kep[q];
-      for (int q = 0; q < 6; ++q) sl.jm[q] = jm[q];
-    }
+    if (lane < cnt && S.u.slot[lane].valid) pair_jacobian(&S.u.slot[lane], drift_first);
     wsync();
     // ---- C: jac_ij, Jacobian rows and dq/dt, in sweep order
 #pragma unroll 1
@@ -915,7 +990,11 @@ __device__ void findtransit(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int
 
 template <int NB>
 __device__ void detect_transits(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int lane, double* ws) {
-  save_prior<NB>(S, lane);
+  // set_state!(s_prior, s) (timing.jl:6) is taken lazily, before the first
+  // search of this step; a search leaves the state modified and the prior
+  // state is restored right after it (timing.jl:19-20), so bodies without a
+  // transit see the unmodified state either way.
+  bool saved = false;
   const int ti = D.ti;
   for (int i = 0; i < NB; ++i) {
     if (i == ti) continue;
@@ -927,13 +1006,18 @@ __device__ void detect_transits(Smem<NB>& S, const SysDesc& D, const SysBufs& B,
       if (lane == 0) S.count[i] = cnt;
       wsync();
       if (cnt >= 1 && cnt <= D.ntt) {  // count starts at 0; the lower bound keeps tt/dq indexing in range
+        if (!saved) {
+          save_prior<NB>(S, lane);
+          saved = true;
+        }
         const double dt0 = ((-gi) * D.hstep) / (gi - S.gsave[i]);
         findtransit<NB>(S, D, B, ti, i, dt0, lane, ws);
+        restore_prior<NB>(S, lane);
       }
     }
     wsync();
     if (lane == 0) S.gsave[i] = gi;
-    restore_prior<NB>(S, lane);
+    wsync();
   }
 }
 
@@ -988,15 +1072,50 @@ template <int NB> __device__ void load_system(Smem<NB>& S, const SysDesc& D, con
   if (lane == 0) S.t = st[off_t(NB)];
   wsync();
   if (lane == 0) {
-    int nf = 0, nbw = 0;
+    constexpr int NPS = Smem<NB>::NP > 0 ? Smem<NB>::NP : 1;
+    // a massless pair (gm == 0) leaves jac_ij from the pair before it in
+    // place (ahl21.jl:712-713), so systems with one keep the sequential order
+    bool seq = false;
     for (int i = 0; i < NB - 1; ++i)
       for (int j = i + 1; j < NB; ++j)
-        if (!S.pair[i + NB * j]) { S.plist[0][2 * nf] = (uint8_t)i; S.plist[0][2 * nf + 1] = (uint8_t)j; ++nf; }
-    for (int i = NB - 2; i >= 0; --i)
-      for (int j = NB - 1; j >= i + 1; --j)
-        if (!S.pair[i + NB * j]) { S.plist[1][2 * nbw] = (uint8_t)i; S.plist[1][2 * nbw + 1] = (uint8_t)j; ++nbw; }
-    S.npl[0] = nf;
-    S.npl[1] = nbw;
+        if (!S.pair[i + NB * j] && NBG_GNEWT * (S.m[i] + S.m[j]) == 0.0) seq = true;
+    for (int dir = 0; dir < 2; ++dir) {
+      uint8_t pi_[NPS], pj_[NPS], lv[NPS];
+      int np = 0;
+      if (dir == 0) {
+        for (int i = 0; i < NB - 1; ++i)
+          for (int j = i + 1; j < NB; ++j)
+            if (!S.pair[i + NB * j]) { pi_[np] = (uint8_t)i; pj_[np] = (uint8_t)j; ++np; }
+      } else {
+        for (int i = NB - 2; i >= 0; --i)
+          for (int j = NB - 1; j >= i + 1; --j)
+            if (!S.pair[i + NB * j]) { pi_[np] = (uint8_t)i; pj_[np] = (uint8_t)j; ++np; }
+      }
+      int last[NB];
+      for (int b = 0; b < NB; ++b) last[b] = 0;
+      int nl = 0;
+      for (int p = 0; p < np; ++p) {
+        const int a = last[pi_[p]], b = last[pj_[p]];
+        const int l = seq ? p + 1 : (a > b ? a : b) + 1;
+        lv[p] = (uint8_t)l;
+        last[pi_[p]] = l;
+        last[pj_[p]] = l;
+        nl = l > nl ? l : nl;
+      }
+      int w = 0;
+      for (int l = 1; l <= nl; ++l) {
+        int cntl = 0;
+        for (int p = 0; p < np; ++p)
+          if (lv[p] == l) {
+            S.plist[dir][2 * w] = pi_[p];
+            S.plist[dir][2 * w + 1] = pj_[p];
+            ++w;
+            ++cntl;
+          }
+        S.lvw[dir][l - 1] = (uint8_t)cntl;
+      }
+      S.nlv[dir] = nl;
+    }
   }
   if (D.grad) {
     const double* J = B.jac + D.off_jac;
