@@ -112,6 +112,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB (read MIB, write MIB) through an elementwise "
+                         "kernel: the reference reading for the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (systems are independent: a "
@@ -159,17 +162,19 @@ def main():
     # n (main steps + one per recorded transit) over that launch's duration,
     # timed with HIP events on the stream it ran on
     launches = {}
-    for n, lms in stats["launch_ms"].items():
-        units = stats["launch_units"][n]
-        launches[str(n)] = {"ms": lms, "units": units, "tflops": W(n) * units / (lms / 1e3) / 1e12,
-                            "kernel": "nbg_run<%d>" % n}
+    for cls, L in stats["launches"].items():
+        flops = sum(W(n) * u for n, u in L["units"].items())
+        launches[str(cls)] = {"kernel": "nbg_run<%d>" % cls, "ms": L["ms"], "units": sum(L["units"].values()),
+                              "units_by_n": {str(n): u for n, u in sorted(L["units"].items())}, "flops": flops,
+                              "tflops": flops / (L["ms"] / 1e3) / 1e12 if L["ms"] > 0 else 0.0}
     dom = max(launches, key=lambda k: launches[k]["ms"])
     achieved = launches[dom]["tflops"]
-    total_flops = sum(W(int(n)) * L["units"] for n, L in launches.items())
+    total_flops = sum(L["flops"] for L in launches.values())
     log("download: %d systems, max count %d" % (len(res), max(int(r["count"].max()) for r in res)))
     for n in sorted(launches, key=int):
         L = launches[n]
-        log("launch n=%s: %.1f ms, %d units, %.3f TFLOP/s" % (n, L["ms"], L["units"], L["tflops"]))
+        log("launch %s: %.1f ms, %d units %s, %.3f TFLOP/s" % (L["kernel"], L["ms"], L["units"], L["units_by_n"],
+                                                               L["tflops"]))
     out = {
         "metric": METRIC,
         "value": value,
@@ -217,6 +222,16 @@ def main():
                                "sample": "first %d of the %d systems (seeded), each capped at 2000 steps, %d system-steps "
                                          "in %.1f s, OpenMP one system per thread" % (ns, len(specs), done, secs)}
         out["batched_vs_cpu"] = value / cpu_rate
+    if args.calibrate_copy > 0:
+        import torch
+        nel = args.calibrate_copy * (1 << 20) // 4
+        src = torch.ones(nel, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 x elementwise copy of %d MiB (read %d B, write %d B each)" %
+            (args.calibrate_copy, nel * 4, nel * 4))
     print(json.dumps(out), flush=True)
 
 

@@ -35,7 +35,7 @@ typedef struct nbg_ctx nbg_ctx;
 int nbg_ctx_create(nbg_ctx** ctx, int device);
 void nbg_ctx_destroy(nbg_ctx* ctx);
 const char* nbg_last_error(const nbg_ctx* ctx);
-/* Device name, compute-unit count, LDS bytes per system for n bodies. */
+/* Device name, compute-unit count, LDS bytes per workgroup serving n bodies. */
 int nbg_device_info(nbg_ctx* ctx, char* name, int name_len, int* cu_count);
 size_t nbg_lds_bytes_per_system(int n);
 
@@ -85,11 +85,14 @@ typedef struct {
   int64_t transits;        /* transits recorded (count <= ntt) */
   int64_t newton_iters;    /* findtransit! Newton iterations */
   double kernel_ms;        /* device time of the last run (HIP events, all launches) */
-  /* Per body count n (index n = 2..12; other entries 0): device time of the
-   * launch for that n, from HIP events on the stream it ran on, and its work
-   * units (main steps + one per recorded transit when grad != 0). */
+  /* Per launch, indexed by its class c (the largest body count it serves:
+   * n = 2..6 share class 6, every larger n has its own): device time (HIP
+   * events on the stream it ran on) and the work units it completed per body
+   * count n, launch_units[c][n] (main steps + one per recorded transit when
+   * grad != 0).  A launch runs its own class first, then takes systems of
+   * smaller classes.  Entries of absent classes are 0. */
   double launch_ms[13];
-  int64_t launch_units[13];
+  int64_t launch_units[13][13];
 } nbg_batch_stats;
 
 int nbg_batch_create(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* tmax,
@@ -98,8 +101,9 @@ int nbg_batch_create(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double*
                      const double* jac_init, nbg_batch** out);
 /* Restore every system to its initial state (device-side copy). */
 int nbg_batch_reset(nbg_batch* b);
-/* Integrate all systems to completion; systems longest first (LPT order),
- * one persistent launch per body count, launches overlapped on streams. */
+/* Integrate all systems to completion: one persistent launch per class of
+ * body counts (2..6, then 7, 8, ... alone), systems longest first within a
+ * launch, launches overlapped on streams. */
 int nbg_batch_run(nbg_batch* b, nbg_batch_stats* stats);
 /* Copy results to host (any pointer may be NULL). */
 int nbg_batch_download(nbg_batch* b, double* tt, double* dtdq0, double* dtdelements, int64_t* count, double* x,

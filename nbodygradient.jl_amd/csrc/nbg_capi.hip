@@ -17,9 +17,10 @@
 #include "nbg_pair.h"
 #include "nbg_types.h"
 
-extern "C" hipError_t nbg_launch_group(int nb, const nbg::SysDesc* descs, int nsys, int* queue, nbg::SysBufs bufs,
-                                       int64_t chunk, int grid, hipStream_t stream);
+extern "C" hipError_t nbg_launch_group(int cls, nbg::GroupList gl, nbg::SysBufs bufs, int64_t chunk, int grid,
+                                       unsigned long long* units, hipStream_t stream);
 extern "C" size_t nbg_lds_bytes(int nb);
+extern "C" int nbg_class_of(int n);
 
 namespace {
 
@@ -52,18 +53,22 @@ struct Store {
   DevVec<uint8_t> pair;
   DevVec<nbg::SysDesc> desc;
   DevVec<int> queue;
+  DevVec<unsigned long long> units;  // work units completed, [launch][n]
   void release() {
     state.release(); state0.release(); jac.release(); jac0.release(); ws.release();
     tt.release(); dq.release(); del.release(); ints.release(); pair.release(); desc.release(); queue.release();
+    units.release();
   }
 };
 
 struct Layout {
   std::vector<nbg::SysDesc> desc;  // system order
   int64_t n_state = 0, n_jac = 0, n_ws = 0, n_tt = 0, n_dq = 0, n_int = 0, n_pair = 0;
-  // launch groups: systems sorted by (n, cost desc)
+  // launch groups: systems sorted by (launch class, cost desc); gn holds the
+  // class (the largest body count it serves), gcost an estimate of its time
   std::vector<int> order;
   std::vector<int> gn, goff, gcnt;
+  std::vector<double> gcost;
 };
 
 double check_step(double t0, double tmax) {  // Integrator.jl:249-259
@@ -91,24 +96,30 @@ void finalize_layout(Layout& L) {
   const int ns = (int)L.desc.size();
   L.order.resize(ns);
   std::iota(L.order.begin(), L.order.end(), 0);
+  // device time per step (microseconds, transit search included) measured on
+  // MI355X for n = 3, 5, 7, 9 and fitted by a quadratic; it only orders work
+  // (longest first) and shares the GPU between launches
   auto cost = [&](int i) {
     const auto& d = L.desc[i];
     const double n = d.n;
-    return (double)(d.nsteps + 1) * (2016.0 * n * n * (n - 1) + 126.0 * n * n * n + 3000.0 * n * (n - 1));
+    return (double)(d.nsteps + 1) * std::max(10.0, 4.6 * n * n + 39.6 * n - 67.0);
   };
   std::stable_sort(L.order.begin(), L.order.end(), [&](int a, int b) {
-    if (L.desc[a].n != L.desc[b].n) return L.desc[a].n > L.desc[b].n;
+    const int ca = nbg_class_of(L.desc[a].n), cb = nbg_class_of(L.desc[b].n);
+    if (ca != cb) return ca > cb;
     return cost(a) > cost(b);
   });
-  L.gn.clear(); L.goff.clear(); L.gcnt.clear();
+  L.gn.clear(); L.goff.clear(); L.gcnt.clear(); L.gcost.clear();
   for (int k = 0; k < ns; ++k) {
-    const int n = L.desc[L.order[k]].n;
-    if (L.gn.empty() || L.gn.back() != n) {
-      L.gn.push_back(n);
+    const int c = nbg_class_of(L.desc[L.order[k]].n);
+    if (L.gn.empty() || L.gn.back() != c) {
+      L.gn.push_back(c);
       L.goff.push_back(k);
       L.gcnt.push_back(0);
+      L.gcost.push_back(0.0);
     }
     L.gcnt.back() += 1;
+    L.gcost.back() += cost(L.order[k]);
   }
 }
 
@@ -189,6 +200,7 @@ int store_alloc(nbg_ctx* ctx, Store& S, const Layout& L, bool keep_initial) {
   NBG_HIP(S.pair.ensure(L.n_pair));
   NBG_HIP(S.desc.ensure(L.desc.size()));
   NBG_HIP(S.queue.ensure(L.gn.size() + 1));
+  NBG_HIP(S.units.ensure(13 * (L.gn.size() + 1)));
   if (keep_initial) {
     NBG_HIP(S.state0.ensure(L.n_state));
     NBG_HIP(S.jac0.ensure(L.n_jac));
@@ -208,19 +220,72 @@ nbg::SysBufs bufs_of(Store& S) {
 
 // Launch every group (one persistent kernel per body count) and wait.
 int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* kernel_ms,
-               std::vector<float>* group_ms = nullptr) {
+               std::vector<float>* group_ms = nullptr, std::vector<unsigned long long>* units = nullptr) {
   NBG_HIP(hipSetDevice(ctx->device));
-  NBG_HIP(hipMemset(S.queue.p, 0, sizeof(int) * (L.gn.size() + 1)));
+  const size_t G = L.gn.size();
+  if (G > (size_t)nbg::MAX_GROUPS) {
+    ctx->err = "too many launch classes";
+    return -1;
+  }
+  NBG_HIP(hipMemset(S.queue.p, 0, sizeof(int) * (G + 1)));
+  NBG_HIP(hipMemset(S.units.p, 0, sizeof(unsigned long long) * 13 * (G + 1)));
   nbg::SysBufs B = bufs_of(S);
   const int ns = (int)ctx->streams.size();
   NBG_HIP(hipEventRecord(ctx->ev0, 0));
   for (int s = 0; s < ns; ++s) NBG_HIP(hipStreamWaitEvent(ctx->streams[s], ctx->ev0, 0));
-  for (size_t g = 0; g < L.gn.size(); ++g) {
-    hipStream_t st = ctx->streams[g % ns];
-    const int grid = grid_for(ctx, L.gn[g], L.gcnt[g]);
-    const bool timed = group_ms && g < 16;
+  // One launch per launch class (groups are in decreasing class order).  A
+  // launch first drains its own class, then takes systems of the smaller
+  // classes, so no launch idles while smaller work remains.  Grids: with one
+  // launch it takes the whole GPU; with several, the GPU is shared in
+  // proportion to each class's estimated work (CU-time: the per-step cost
+  // model times the CU fraction one workgroup holds, by LDS or by its wave
+  // slot, one wave per SIMD at this kernel's register use), every class but
+  // the smallest with a margin (only larger launches can help a class), so
+  // that every workgroup is resident from the start and the long systems of
+  // the large-n classes start first.
+  std::vector<int> grids(G);
+  std::vector<double> frac(G), demand(G);
+  double tot = 0.0;
+  for (size_t g = 0; g < G; ++g) {
+    frac[g] = std::max((double)nbg_lds_bytes(L.gn[g]) / 163840.0, 0.25);
+    demand[g] = L.gcost[g] * frac[g];
+    tot += demand[g];
+  }
+  double used = 0.0;
+  for (size_t g = 0; g < G; ++g) {
+    const int full = grid_for(ctx, L.gn[g], L.gcnt[g]);
+    if (G == 1 || ctx->grid_cap > 0 || tot <= 0.0) {
+      grids[g] = full;
+      continue;
+    }
+    double cus = ctx->cu_count * (demand[g] / tot);
+    if (g + 1 < G) cus *= 1.15;
+    else cus = std::max(ctx->cu_count - used, 0.25 * ctx->cu_count * (demand[g] / tot));
+    grids[g] = std::max(1, std::min(full, (int)(cus / frac[g])));
+    used += grids[g] * frac[g];
+  }
+  // launches longest first, each on the stream with the least queued work
+  // (launches on one stream run one after the other)
+  std::vector<size_t> gord(G);
+  std::iota(gord.begin(), gord.end(), 0);
+  std::stable_sort(gord.begin(), gord.end(),
+                   [&](size_t a, size_t b) { return L.gcost[a] / grids[a] > L.gcost[b] / grids[b]; });
+  std::vector<double> load(ns, 0.0);
+  for (size_t g : gord) {
+    nbg::GroupList gl{};
+    for (size_t h = g; h < G; ++h) {  // own class, then the smaller ones
+      gl.nsys[gl.ng] = L.gcnt[h];
+      gl.descs[gl.ng] = S.desc.p + L.goff[h];
+      gl.queue[gl.ng] = S.queue.p + h;
+      gl.ng += 1;
+    }
+    const int grid = grids[g];
+    const int si = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[si] += L.gcost[g] / grid;
+    hipStream_t st = ctx->streams[si];
+    const bool timed = group_ms != nullptr;
     if (timed) NBG_HIP(hipEventRecord(ctx->gev[2 * g], st));
-    NBG_HIP(nbg_launch_group(L.gn[g], S.desc.p + L.goff[g], L.gcnt[g], S.queue.p + g, B, chunk, grid, st));
+    NBG_HIP(nbg_launch_group(L.gn[g], gl, B, chunk, grid, S.units.p + 13 * g, st));
     if (timed) NBG_HIP(hipEventRecord(ctx->gev[2 * g + 1], st));
   }
   for (int s = 0; s < ns; ++s) {
@@ -239,9 +304,12 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
     *kernel_ms = ms;
   }
   if (group_ms) {
-    group_ms->assign(L.gn.size(), 0.0f);
-    for (size_t g = 0; g < L.gn.size() && g < 16; ++g)
-      NBG_HIP(hipEventElapsedTime(&(*group_ms)[g], ctx->gev[2 * g], ctx->gev[2 * g + 1]));
+    group_ms->assign(G, 0.0f);
+    for (size_t g = 0; g < G; ++g) NBG_HIP(hipEventElapsedTime(&(*group_ms)[g], ctx->gev[2 * g], ctx->gev[2 * g + 1]));
+  }
+  if (units) {
+    units->assign(13 * G, 0);
+    NBG_HIP(hipMemcpy(units->data(), S.units.p, sizeof(unsigned long long) * 13 * G, hipMemcpyDeviceToHost));
   }
   return 0;
 }
@@ -406,7 +474,7 @@ int nbg_device_info(nbg_ctx* ctx, char* name, int name_len, int* cu_count) {
   return 0;
 }
 
-size_t nbg_lds_bytes_per_system(int n) { return nbg_lds_bytes(n); }
+size_t nbg_lds_bytes_per_system(int n) { return (n >= 2 && n <= 12) ? nbg_lds_bytes(nbg_class_of(n)) : 0; }
 
 int nbg_set_launch(nbg_ctx* ctx, int32_t max_groups_per_cu, int32_t streams) {
   if (!ctx) return -1;
@@ -610,23 +678,26 @@ int nbg_batch_run(nbg_batch* b, nbg_batch_stats* stats) {
   std::lock_guard<std::mutex> lk(ctx->mu);
   double ms = 0.0;
   std::vector<float> gms;
-  if (int rc = run_groups(ctx, b->S, b->L, 0, &ms, &gms)) return rc;
+  std::vector<unsigned long long> gun;
+  if (int rc = run_groups(ctx, b->S, b->L, 0, &ms, &gms, &gun)) return rc;
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     std::vector<int64_t> hi((size_t)b->L.n_int);
     NBG_HIP(hipMemcpy(hi.data(), b->S.ints.p, hi.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
     int64_t tr = 0, it = 0;
     for (const auto& d : b->L.desc) {
-      const int64_t t_d = hi[d.off_int + d.n + 1];
-      tr += t_d;
+      tr += hi[d.off_int + d.n + 1];
       it += hi[d.off_int + d.n + 2];
-      stats->launch_units[d.n] += d.nsteps + (d.grad ? t_d : 0);
     }
     stats->transits = tr;
     stats->newton_iters = it;
     stats->system_steps = b->steps_main + (b->L.desc.empty() || !b->L.desc[0].grad ? 0 : tr);
     stats->kernel_ms = ms;
-    for (size_t g = 0; g < b->L.gn.size() && g < gms.size(); ++g) stats->launch_ms[b->L.gn[g]] = gms[g];
+    for (size_t g = 0; g < b->L.gn.size(); ++g) {
+      const int c = b->L.gn[g];
+      stats->launch_ms[c] = gms[g];
+      for (int n = 0; n < 13; ++n) stats->launch_units[c][n] = (int64_t)gun[13 * g + n];
+    }
   }
   return 0;
 }

@@ -1197,14 +1197,36 @@ __device__ void run_system(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int6
   if (done && D.mode == MODE_TRANSITS && grad) dtdelements<NB>(S, D, B, lane);
 }
 
-template <int NB>
-__global__ __launch_bounds__(WAVE) void nbg_run(const SysDesc* __restrict__ descs, int nsys, int* __restrict__ queue,
-                                                SysBufs B, int64_t chunk) {
-  __shared__ Smem<NB> S;
+// Run system D with the LDS layout of its body count N (a compile-time
+// chain of scalar branches on the uniform D.n).
+template <int N, int HI>
+__device__ __forceinline__ void dispatch_n(unsigned char* raw, const SysDesc& D, const SysBufs& B, int64_t chunk,
+                                           int lane) {
+  if constexpr (N <= HI) {
+    if (D.n == N) {
+      run_system<N>(*reinterpret_cast<Smem<N>*>(raw), D, B, chunk, lane);
+      return;
+    }
+    dispatch_n<N + 1, HI>(raw, D, B, chunk, lane);
+  }
+}
+
+// Persistent kernel of launch class HI: every workgroup (one wave) takes
+// systems from the queues of its group list in order -- its own class first,
+// then classes of fewer bodies, each ordered longest first -- and runs each
+// to completion (or for `chunk` steps).  LDS is sized for HI bodies; Smem<N>
+// grows with N, so every smaller layout fits in it.  `units` (13 entries,
+// indexed by body count) collects the work units this launch completed:
+// main steps plus one per recorded transit when gradients are on.
+template <int HI>
+__global__ __launch_bounds__(WAVE) void nbg_run(GroupList gl, SysBufs B, int64_t chunk,
+                                                unsigned long long* __restrict__ units) {
+  __shared__ __attribute__((aligned(16))) unsigned char raw[sizeof(Smem<HI>)];
   const int lane = threadIdx.x;
-  while (true) {
+  int g = 0;
+  while (g < gl.ng) {
     int idx = 0;
-    if (lane == 0) idx = atomicAdd(queue, 1);
+    if (lane == 0) idx = atomicAdd(gl.queue[g], 1);
     // readfirstlane (lane 0: every lane is active here) makes the system
     // index, its descriptor and every branch on it wave-uniform (scalar
     // registers, scalar branches).  With a lane shuffle instead, the compiler
@@ -1212,29 +1234,36 @@ __global__ __launch_bounds__(WAVE) void nbg_run(const SysDesc* __restrict__ desc
     // per-system code; one such region's exit once dropped a register copy
     // carried to the next system (it ran with an empty exec mask).
     idx = __builtin_amdgcn_readfirstlane(idx);
-    if (idx >= nsys) break;
-    const SysDesc D = descs[idx];
-    run_system<NB>(S, D, B, chunk, lane);
+    if (idx >= gl.nsys[g]) {
+      ++g;
+      continue;
+    }
+    const SysDesc D = gl.descs[g][idx];
+    const int64_t* ints = B.ints + D.off_int;
+    const int64_t before = ints[D.n];  // step cursor (lane 0 reads and writes it)
+    dispatch_n<2, HI>(raw, D, B, chunk, lane);
     wsync();
+    if (lane == 0 && before <= D.nsteps && ints[D.n] > D.nsteps)  // finished during this visit
+      atomicAdd(&units[D.n], (unsigned long long)(D.nsteps + (D.grad ? ints[D.n + 1] : 0)));
   }
 }
 
 }  // namespace nbg
 
 // ---------------------------------------------------------------------------
-// Host launcher (one launch per body count; the grid is persistent).
+// Host launcher.  Body counts are served by launch classes of similar LDS
+// footprint: n = 2..6 share one kernel (sized for n = 6: four workgroups per
+// CU, which their register use allows anyway), every larger n has its own.
 // ---------------------------------------------------------------------------
-extern "C" hipError_t nbg_launch_group(int nb, const nbg::SysDesc* descs, int nsys, int* queue, nbg::SysBufs bufs,
-                                       int64_t chunk, int grid, hipStream_t stream) {
-  switch (nb) {
-#define NBG_CASE(N)                                                                                      \
-  case N:                                                                                                \
-    hipLaunchKernelGGL(nbg::nbg_run<N>, dim3(grid), dim3(nbg::WAVE), 0, stream, descs, nsys, queue, bufs, chunk); \
+extern "C" int nbg_class_of(int n) { return n <= 6 ? 6 : n; }
+
+extern "C" hipError_t nbg_launch_group(int cls, nbg::GroupList gl, nbg::SysBufs bufs, int64_t chunk, int grid,
+                                       unsigned long long* units, hipStream_t stream) {
+  switch (cls) {
+#define NBG_CASE(HI)                                                                                          \
+  case HI:                                                                                                    \
+    hipLaunchKernelGGL(nbg::nbg_run<HI>, dim3(grid), dim3(nbg::WAVE), 0, stream, gl, bufs, chunk, units);    \
     return hipGetLastError();
-    NBG_CASE(2)
-    NBG_CASE(3)
-    NBG_CASE(4)
-    NBG_CASE(5)
     NBG_CASE(6)
     NBG_CASE(7)
     NBG_CASE(8)

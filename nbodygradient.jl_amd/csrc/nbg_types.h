@@ -47,6 +47,17 @@ struct SysDesc {
   int64_t off_state, off_jac, off_ws, off_tt, off_dq, off_int, off_pair;
 };
 
+// The systems one launch may take: its own launch class first, then the
+// classes of fewer bodies (their LDS layouts fit in its LDS), each a queue of
+// descriptors ordered longest first with an atomic cursor.
+constexpr int MAX_GROUPS = 8;
+struct GroupList {
+  int32_t ng;
+  int32_t nsys[MAX_GROUPS];
+  const SysDesc* descs[MAX_GROUPS];
+  int* queue[MAX_GROUPS];
+};
+
 struct SysBufs {
   double* state;
   double* jac;

@@ -59,8 +59,9 @@ class Batch:
         self.ctx.check(_lib.lib().nbg_batch_run(self._h, C.byref(st)), "nbg_batch_run")
         return {"system_steps": st.system_steps, "transits": st.transits, "newton_iters": st.newton_iters,
                 "kernel_ms": st.kernel_ms,
-                "launch_ms": {n: st.launch_ms[n] for n in range(13) if st.launch_units[n] > 0},
-                "launch_units": {n: st.launch_units[n] for n in range(13) if st.launch_units[n] > 0}}
+                "launches": {c: {"ms": st.launch_ms[c],
+                                 "units": {n: st.launch_units[c][n] for n in range(13) if st.launch_units[c][n] > 0}}
+                             for c in range(13) if st.launch_ms[c] > 0}}
 
     def reset(self):
         self.ctx.check(_lib.lib().nbg_batch_reset(self._h), "nbg_batch_reset")
