@@ -31,6 +31,11 @@ sys.path.insert(0, os.path.join(ROOT, "nbodygradient.jl_amd"))
 import numpy as np  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per system-step of nbg_run<9>, from the rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes scaled by the same counters' reading of a copy of known
+# size (profiles/nbg_run_c2_x0.05_counters.md: 22.22 GB for 799,221 n = 9
+# system-steps in one dispatch)
+HBM_BYTES_PER_UNIT = 22.22e9 / 799221
 METRIC = "system-steps/s w/ Jacobian, batched vs plain GPU; p50/p99 request latency"
 
 
@@ -194,7 +199,10 @@ def main():
                    "systems": len(specs), "system_steps": sys_steps, "transits": stats["transits"],
                    "newton_iters": stats["newton_iters"], "parallelism": "1 GPU, persistent launch per n"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS,
+                     "traffic": HBM_BYTES_PER_UNIT * launches[dom]["units"],
+                     "traffic_basis": "rocprofv3 FETCH_SIZE/WRITE_SIZE of nbg_run<9>, calibrated on nbg_copy_kernel: "
+                                      "%.0f B per system-step x this launch's units" % HBM_BYTES_PER_UNIT,
                      "kernel": launches[dom]["kernel"], "launch_ms": launches[dom]["ms"],
                      "note": "FP64 compute bound (VALU); MI355X FP64 vector and matrix peaks are both 78.6 "
                              "TFLOP/s; W(n) = 2016 n^2 (n-1) + 126 n^3 flops per system-step"},
@@ -223,15 +231,15 @@ def main():
                                          "in %.1f s, OpenMP one system per thread" % (ns, len(specs), done, secs)}
         out["batched_vs_cpu"] = value / cpu_rate
     if args.calibrate_copy > 0:
-        import torch
-        nel = args.calibrate_copy * (1 << 20) // 4
-        src = torch.ones(nel, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            torch.mul(src, 1.0, out=dst)
-        torch.cuda.synchronize()
-        log("copy calibration: 3 x elementwise copy of %d MiB (read %d B, write %d B each)" %
-            (args.calibrate_copy, nel * 4, nel * 4))
+        import ctypes as C
+        from nbodygradient_amd import _lib
+        nbytes = args.calibrate_copy * (1 << 20)
+        cms = C.c_double(0.0)
+        ctx.check(_lib.lib().nbg_copy_calibrate(ctx.handle, nbytes, 3, C.byref(cms)), "nbg_copy_calibrate")
+        out["copy_calibration"] = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": cms.value,
+                                   "GB_per_s": 2 * nbytes / (cms.value / 1e3) / 1e9, "kernel": "nbg_copy_kernel"}
+        log("copy calibration: nbg_copy_kernel, %d MiB read + %d MiB written, %.3f ms, %.0f GB/s" %
+            (args.calibrate_copy, args.calibrate_copy, cms.value, 2 * nbytes / (cms.value / 1e3) / 1e9))
     print(json.dumps(out), flush=True)
 
 

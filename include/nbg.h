@@ -122,6 +122,12 @@ int nbg_set_launch(nbg_ctx* ctx, int32_t max_groups_per_cu, int32_t streams);
 /* Upper bound on persistent workgroups per launch (0 = no bound). */
 int nbg_set_grid_cap(nbg_ctx* ctx, int32_t grid_cap);
 
+/* Diagnostic only (no counterpart in the reference): a plain device copy of
+ * `bytes` (a multiple of 16) through a grid-stride kernel, run `reps` times;
+ * *ms = average duration.  It reads and writes `bytes` per run: the reference
+ * reading for HBM byte counters and the box's achieved copy bandwidth. */
+int nbg_copy_calibrate(nbg_ctx* ctx, int64_t bytes, int32_t reps, double* ms);
+
 /* Diagnostic only: evaluates the device pair formulas (nbg_pair.h) on the
  * host CPU so the test suite can compare them bit for bit with the CPU
  * restatement without a GPU.  Not used by any integration path.

@@ -21,6 +21,7 @@ extern "C" hipError_t nbg_launch_group(int cls, nbg::GroupList gl, nbg::SysBufs 
                                        unsigned long long* units, hipStream_t stream);
 extern "C" size_t nbg_lds_bytes(int nb);
 extern "C" int nbg_class_of(int n);
+extern "C" hipError_t nbg_launch_copy(const void* src, void* dst, size_t bytes, int cu_count, hipStream_t stream);
 
 namespace {
 
@@ -752,6 +753,35 @@ int nbg_batch_transits(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const doubl
   if (!rc) rc = nbg_batch_download(b, tt, dtdq0, dtdelements, count, nullptr, nullptr);
   nbg_batch_destroy(b);
   return rc;
+}
+
+int nbg_copy_calibrate(nbg_ctx* ctx, int64_t bytes, int32_t reps, double* ms) {
+  NBG_ARG(ctx && bytes > 0 && bytes % 16 == 0 && reps > 0 && ms, "nbg_copy_calibrate: bad arguments");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_HIP(hipSetDevice(ctx->device));
+  void *src = nullptr, *dst = nullptr;
+  NBG_HIP(hipMalloc(&src, (size_t)bytes));
+  if (hipMalloc(&dst, (size_t)bytes) != hipSuccess) {
+    (void)hipFree(src);
+    ctx->err = "nbg_copy_calibrate: out of device memory";
+    return -2;
+  }
+  hipError_t e = hipMemset(src, 0, (size_t)bytes);
+  if (e == hipSuccess) e = nbg_launch_copy(src, dst, (size_t)bytes, ctx->cu_count, 0);  // warm-up
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev0, 0);
+  for (int r = 0; r < reps && e == hipSuccess; ++r) e = nbg_launch_copy(src, dst, (size_t)bytes, ctx->cu_count, 0);
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev1, 0);
+  if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+  float t = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  if (e != hipSuccess) {
+    ctx->err = std::string("nbg_copy_calibrate: ") + hipGetErrorString(e);
+    return -2;
+  }
+  *ms = t / reps;
+  return 0;
 }
 
 int nbg_selftest_pair_host(const double* x0, const double* v0, double k, double h, int32_t drift_first,

@@ -1277,6 +1277,20 @@ extern "C" hipError_t nbg_launch_group(int cls, nbg::GroupList gl, nbg::SysBufs 
   }
 }
 
+// Plain device copy (16-byte elements, grid-stride): the reference kernel
+// for calibrating HBM byte counters and for measuring achieved bandwidth.
+__global__ __launch_bounds__(256) void nbg_copy_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                       size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+extern "C" hipError_t nbg_launch_copy(const void* src, void* dst, size_t bytes, int cu_count, hipStream_t stream) {
+  const size_t n = bytes / sizeof(double2);
+  hipLaunchKernelGGL(nbg_copy_kernel, dim3(cu_count * 8), dim3(256), 0, stream, (const double2*)src, (double2*)dst, n);
+  return hipGetLastError();
+}
+
 extern "C" size_t nbg_lds_bytes(int nb) {
   switch (nb) {
 #define NBG_SZ(N) case N: return sizeof(nbg::Smem<N>);

@@ -60,6 +60,7 @@ def lib():
     L.nbg_batch_destroy.argtypes = [VP]
     L.nbg_set_launch.argtypes = [VP, C.c_int32, C.c_int32]
     L.nbg_set_grid_cap.argtypes = [VP, C.c_int32]
+    L.nbg_copy_calibrate.argtypes = [VP, C.c_int64, C.c_int32, D]
     L.nbg_selftest_pair_host.argtypes = [D, D, C.c_double, C.c_double, C.c_int32, D]
     _lib = L
     return L
