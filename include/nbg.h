@@ -117,6 +117,37 @@ int nbg_batch_transits(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const doubl
                        const double* jac_init, double* tt, double* dtdq0, double* dtdelements, int64_t* count,
                        nbg_batch_stats* stats);
 
+/* ---- Serving: continuous batching of integrate-plus-gradient requests ----
+ * No counterpart in the reference (its callers run one system per call); this
+ * is the request loop of a timing fit's likelihood evaluations.  Each request
+ * is one (intr)(s, tt; grad) run as in nbg_integrate_transits.  A server owns
+ * a device arena of max_bytes; requests are admitted first come first served
+ * while their HBM footprint (nbg_server_footprint) fits, and every call of
+ * nbg_server_step advances all admitted requests by up to chunk_steps steps
+ * in one set of persistent launches, so requests that arrive while others run
+ * join at the next round.  Results are bitwise those of the one-system call.
+ * submit, finished, fetch and status may be called from other threads while
+ * a step runs; steps are serialised with the context's other calls. */
+typedef struct nbg_server nbg_server;
+int nbg_server_create(nbg_ctx* ctx, int64_t max_bytes, int64_t chunk_steps, nbg_server** out);
+void nbg_server_destroy(nbg_server* s);
+/* Queue one request (0-based ti; x, v: 3 x n; m: n; pair: n x n or NULL;
+ * jac_init: 7n x 7n, needed when grad != 0); *id identifies it. */
+int nbg_server_submit(nbg_server* s, int32_t n, double h, double tmax, double t0, int32_t ti, int64_t ntt,
+                      int32_t grad, int32_t ntbv, const double* x, const double* v, const double* m,
+                      const uint8_t* pair, const double* jac_init, int64_t* id);
+/* One round: admit, advance every admitted request, retire finished ones. */
+int nbg_server_step(nbg_server* s, int32_t* n_finished, double* kernel_ms);
+/* Ids of finished, not yet fetched requests (at most max_ids). */
+int nbg_server_finished(nbg_server* s, int64_t* ids, int32_t max_ids, int32_t* n);
+/* Copy a finished request's outputs (layouts as nbg_integrate_transits; any
+ * pointer may be NULL) and forget it. */
+int nbg_server_fetch(nbg_server* s, int64_t id, double* tt, double* dtdq0, double* dtdelements, int64_t* count,
+                     double* x, double* v);
+int nbg_server_status(nbg_server* s, int64_t* pending, int64_t* active, int64_t* finished, int64_t* bytes_free);
+/* Device bytes one request of n bodies and ntt transit slots occupies. */
+int64_t nbg_server_footprint(int32_t n, int32_t ntbv, int64_t ntt);
+
 /* Tuning knobs for the persistent launches (0 = automatic). */
 int nbg_set_launch(nbg_ctx* ctx, int32_t max_groups_per_cu, int32_t streams);
 /* Upper bound on persistent workgroups per launch (0 = no bound). */

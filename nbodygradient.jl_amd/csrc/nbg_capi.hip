@@ -8,6 +8,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <mutex>
 #include <numeric>
 #include <string>
@@ -70,6 +72,7 @@ struct Layout {
   std::vector<int> order;
   std::vector<int> gn, goff, gcnt;
   std::vector<double> gcost;
+  std::vector<double> cost;  // optional per-system cost (default: from nsteps)
 };
 
 double check_step(double t0, double tmax) {  // Integrator.jl:249-259
@@ -81,6 +84,8 @@ double check_step(double t0, double tmax) {  // Integrator.jl:249-259
 
 long long jround(double x) { return (long long)std::nearbyint(x); }  // Julia round(Int64, x): half to even
 
+void group_layout(Layout& L);
+
 void finalize_layout(Layout& L) {
   int64_t s = 0, j = 0, w = 0, t = 0, q = 0, in = 0, pr = 0;
   for (auto& d : L.desc) {
@@ -89,11 +94,17 @@ void finalize_layout(Layout& L) {
     d.off_jac = j; j += nbg::jac_len(n);
     d.off_ws = w; w += nbg::ws_len(n);
     d.off_tt = t; t += (int64_t)d.ntbv * n * d.ntt;
-    d.off_dq = q; q += (int64_t)d.ntbv * n * d.ntt * 7 * n;
+    d.off_dq = q; d.off_del = q; q += (int64_t)d.ntbv * n * d.ntt * 7 * n;
     d.off_int = in; in += nbg::int_len(n);
     d.off_pair = pr; pr += (int64_t)n * n;
   }
   L.n_state = s; L.n_jac = j; L.n_ws = w; L.n_tt = t; L.n_dq = q; L.n_int = in; L.n_pair = pr;
+  for (size_t k = 0; k < L.desc.size(); ++k) L.desc[k].tag = (int32_t)k;
+  group_layout(L);
+}
+
+// Order systems by (launch class, cost desc) and form the launch groups.
+void group_layout(Layout& L) {
   const int ns = (int)L.desc.size();
   L.order.resize(ns);
   std::iota(L.order.begin(), L.order.end(), 0);
@@ -103,7 +114,8 @@ void finalize_layout(Layout& L) {
   auto cost = [&](int i) {
     const auto& d = L.desc[i];
     const double n = d.n;
-    return (double)(d.nsteps + 1) * std::max(10.0, 4.6 * n * n + 39.6 * n - 67.0);
+    const double steps = L.cost.empty() ? (double)(d.nsteps + 1) : L.cost[i];
+    return steps * std::max(10.0, 4.6 * n * n + 39.6 * n - 67.0);
   };
   std::stable_sort(L.order.begin(), L.order.end(), [&](int a, int b) {
     const int ca = nbg_class_of(L.desc[a].n), cb = nbg_class_of(L.desc[b].n);
@@ -216,12 +228,14 @@ nbg::SysBufs bufs_of(Store& S) {
   nbg::SysBufs B;
   B.state = S.state.p; B.jac = S.jac.p; B.ws = S.ws.p; B.tt = S.tt.p; B.dq = S.dq.p; B.del = S.del.p;
   B.ints = S.ints.p; B.pair = S.pair.p;
+  B.done_count = nullptr; B.done_list = nullptr;
   return B;
 }
 
 // Launch every group (one persistent kernel per body count) and wait.
 int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* kernel_ms,
-               std::vector<float>* group_ms = nullptr, std::vector<unsigned long long>* units = nullptr) {
+               std::vector<float>* group_ms = nullptr, std::vector<unsigned long long>* units = nullptr,
+               const nbg::SysBufs* bufs = nullptr) {
   NBG_HIP(hipSetDevice(ctx->device));
   const size_t G = L.gn.size();
   if (G > (size_t)nbg::MAX_GROUPS) {
@@ -230,7 +244,7 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
   }
   NBG_HIP(hipMemset(S.queue.p, 0, sizeof(int) * (G + 1)));
   NBG_HIP(hipMemset(S.units.p, 0, sizeof(unsigned long long) * 13 * (G + 1)));
-  nbg::SysBufs B = bufs_of(S);
+  const nbg::SysBufs B = bufs ? *bufs : bufs_of(S);
   const int ns = (int)ctx->streams.size();
   NBG_HIP(hipEventRecord(ctx->ev0, 0));
   for (int s = 0; s < ns; ++s) NBG_HIP(hipStreamWaitEvent(ctx->streams[s], ctx->ev0, 0));
@@ -754,6 +768,336 @@ int nbg_batch_transits(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const doubl
   nbg_batch_destroy(b);
   return rc;
 }
+
+// ---------------------------------------------------------------------------
+// Serving: requests (one system each) admitted into a device arena as their
+// HBM footprint allows, advanced `chunk` steps per round by the persistent
+// class kernels together with every other active request, retired when done.
+// ---------------------------------------------------------------------------
+namespace {
+
+// First-fit allocator of 256-byte-aligned ranges of a device buffer.
+struct Arena {
+  int64_t cap = 0;
+  std::map<int64_t, int64_t> free_;  // offset -> length
+  static int64_t round(int64_t len) { return (len + 255) & ~(int64_t)255; }
+  void init(int64_t c) {
+    cap = c;
+    free_.clear();
+    if (c > 0) free_[0] = c;
+  }
+  int64_t alloc(int64_t len) {
+    len = round(len);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second < len) continue;
+      const int64_t off = it->first, rem = it->second - len;
+      free_.erase(it);
+      if (rem > 0) free_[off + len] = rem;
+      return off;
+    }
+    return -1;
+  }
+  void release(int64_t off, int64_t len) {
+    len = round(len);
+    auto it = free_.emplace(off, len).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  int64_t free_bytes() const {
+    int64_t f = 0;
+    for (const auto& kv : free_) f += kv.second;
+    return f;
+  }
+};
+
+// Byte offsets of a request's record inside its arena block.
+struct ReqLayout {
+  int64_t state, jac, ws, tt, dq, del, ints, pair, bytes;
+};
+
+ReqLayout req_layout(int n, int ntbv, int64_t ntt) {
+  ReqLayout R;
+  int64_t o = 0;
+  R.state = o; o += 8 * nbg::state_len(n);
+  R.jac = o; o += 8 * nbg::jac_len(n);
+  R.ws = o; o += 8 * nbg::ws_len(n);
+  R.tt = o; o += 8 * (int64_t)ntbv * n * ntt;
+  R.dq = o; o += 8 * (int64_t)ntbv * n * ntt * 7 * n;
+  R.del = o; o += 8 * (int64_t)ntbv * n * ntt * 7 * n;
+  R.ints = o; o += 8 * nbg::int_len(n);
+  R.pair = o; o += ((int64_t)n * n + 7) / 8 * 8;
+  R.bytes = o;
+  return R;
+}
+
+struct ServeReq {
+  int64_t id = 0;
+  nbg::SysDesc d{};
+  ReqLayout lay{};
+  int64_t off = -1;       // arena block (bytes)
+  int64_t steps_run = 0;  // steps advanced so far (chunk per round)
+  HostRec init;           // initial record, until admitted
+  std::vector<double> tt, dq, del, x, v;
+  std::vector<int64_t> count;
+};
+
+}  // namespace
+
+struct nbg_server {
+  nbg_ctx* ctx = nullptr;
+  int64_t chunk = 0;
+  DevVec<unsigned char> arena_mem;
+  Arena arena;
+  DevVec<int> done;  // [0]: count, [1..]: tags
+  Store S;           // descriptors, queues and unit counters of a round
+  std::mutex mu;     // pending, finished, next_id
+  std::deque<ServeReq*> pending;
+  std::vector<ServeReq*> active;
+  std::map<int64_t, ServeReq*> finished;
+  int64_t next_id = 1;
+  int64_t rounds = 0;
+};
+
+int nbg_server_create(nbg_ctx* ctx, int64_t max_bytes, int64_t chunk_steps, nbg_server** out) {
+  NBG_ARG(ctx && out && max_bytes >= 4096 && chunk_steps > 0, "nbg_server_create: bad arguments");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_HIP(hipSetDevice(ctx->device));
+  auto* sv = new nbg_server();
+  sv->ctx = ctx;
+  sv->chunk = chunk_steps;
+  const int64_t cap = Arena::round(max_bytes);
+  if (sv->arena_mem.ensure((size_t)cap) != hipSuccess) {
+    delete sv;
+    ctx->err = "nbg_server_create: cannot allocate the device arena";
+    return -2;
+  }
+  sv->arena.init(cap);
+  *out = sv;
+  return 0;
+}
+
+void nbg_server_destroy(nbg_server* sv) {
+  if (!sv) return;
+  {
+    std::lock_guard<std::mutex> lk(sv->ctx->mu);
+    (void)hipSetDevice(sv->ctx->device);
+    sv->arena_mem.release();
+    sv->done.release();
+    sv->S.release();
+  }
+  for (auto* r : sv->pending) delete r;
+  for (auto* r : sv->active) delete r;
+  for (auto& kv : sv->finished) delete kv.second;
+  delete sv;
+}
+
+int nbg_server_submit(nbg_server* sv, int32_t n, double h, double tmax, double t0, int32_t ti, int64_t ntt,
+                      int32_t grad, int32_t ntbv, const double* x, const double* v, const double* m,
+                      const uint8_t* pair, const double* jac_init, int64_t* id) {
+  if (!sv) return -1;
+  nbg_ctx* ctx = sv->ctx;
+  NBG_ARG(n >= NB_MIN && n <= NB_MAX, "n must be in [2, 12]");
+  NBG_ARG(std::isfinite(h) && h != 0.0 && std::isfinite(tmax) && std::isfinite(t0) && ntt >= 0,
+          "h finite non-zero, tmax and t0 finite, ntt >= 0");
+  NBG_ARG(ti >= 0 && ti < n, "ti out of range");
+  NBG_ARG(ntbv == 1 || ntbv == 3, "ntbv must be 1 or 3");
+  NBG_ARG(x && v && m && id && (!grad || jac_init), "missing arrays");
+  auto* r = new ServeReq();
+  nbg::SysDesc& d = r->d;
+  d.n = n; d.mode = nbg::MODE_TRANSITS; d.grad = grad ? 1 : 0; d.ti = ti; d.ntbv = ntbv;
+  d.any_kick = pair ? any_kick(n, pair) : 0;
+  d.nsteps = std::llabs(jround(tmax / h));
+  d.ntt = ntt; d.h = h * check_step(t0, tmax + t0); d.hstep = h; d.hlast = 0.0; d.t0 = t0;
+  r->lay = req_layout(n, ntbv, ntt);
+  pack_system(d, x, v, nullptr, nullptr, m, pair, nullptr, nullptr, nullptr, jac_init, t0, nullptr, nullptr, r->init);
+  if (Arena::round(r->lay.bytes) > sv->arena.cap) {
+    delete r;
+    ctx->err = "request footprint exceeds the server's arena";
+    return -1;
+  }
+  std::lock_guard<std::mutex> lk(sv->mu);
+  r->id = sv->next_id++;
+  sv->pending.push_back(r);
+  *id = r->id;
+  return 0;
+}
+
+int nbg_server_step(nbg_server* sv, int32_t* n_finished, double* kernel_ms) {
+  if (!sv) return -1;
+  nbg_ctx* ctx = sv->ctx;
+  std::lock_guard<std::mutex> glk(ctx->mu);
+  NBG_HIP(hipSetDevice(ctx->device));
+  unsigned char* base = sv->arena_mem.p;
+  // admission, first come first served, while the arena has room
+  std::vector<ServeReq*> admitted;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    while (!sv->pending.empty()) {
+      ServeReq* r = sv->pending.front();
+      const int64_t off = sv->arena.alloc(r->lay.bytes);
+      if (off < 0) break;
+      sv->pending.pop_front();
+      r->off = off;
+      admitted.push_back(r);
+    }
+  }
+  for (ServeReq* r : admitted) {
+    nbg::SysDesc& d = r->d;
+    const ReqLayout& Ly = r->lay;
+    d.off_state = (r->off + Ly.state) / 8;
+    d.off_jac = (r->off + Ly.jac) / 8;
+    d.off_ws = (r->off + Ly.ws) / 8;
+    d.off_tt = (r->off + Ly.tt) / 8;
+    d.off_dq = (r->off + Ly.dq) / 8;
+    d.off_del = (r->off + Ly.del) / 8;
+    d.off_int = (r->off + Ly.ints) / 8;
+    d.off_pair = r->off + Ly.pair;
+    NBG_HIP(hipMemcpy(base + r->off + Ly.state, r->init.state.data(), 8 * r->init.state.size(), hipMemcpyHostToDevice));
+    NBG_HIP(hipMemcpy(base + r->off + Ly.jac, r->init.jac.data(), 8 * r->init.jac.size(), hipMemcpyHostToDevice));
+    NBG_HIP(hipMemset(base + r->off + Ly.tt, 0, (size_t)(Ly.pair - Ly.tt)));  // tt, dq, del, ints
+    NBG_HIP(hipMemcpy(base + r->off + Ly.pair, r->init.pair.data(), r->init.pair.size(), hipMemcpyHostToDevice));
+    r->init = HostRec();
+    sv->active.push_back(r);
+  }
+  if (n_finished) *n_finished = 0;
+  if (kernel_ms) *kernel_ms = 0.0;
+  if (sv->active.empty()) return 0;
+  // one round over every active request
+  Layout L;
+  L.desc.resize(sv->active.size());
+  L.cost.resize(sv->active.size());
+  for (size_t k = 0; k < sv->active.size(); ++k) {
+    ServeReq* r = sv->active[k];
+    L.desc[k] = r->d;
+    L.desc[k].tag = (int32_t)k;
+    L.cost[k] = (double)std::max<int64_t>(1, std::min<int64_t>(sv->chunk, r->d.nsteps + 1 - r->steps_run));
+  }
+  group_layout(L);
+  const size_t na = sv->active.size();
+  NBG_HIP(sv->S.desc.ensure(na));
+  NBG_HIP(sv->S.queue.ensure(L.gn.size() + 1));
+  NBG_HIP(sv->S.units.ensure(13 * (L.gn.size() + 1)));
+  NBG_HIP(sv->done.ensure(na + 1));
+  std::vector<nbg::SysDesc> sorted(na);
+  for (size_t k = 0; k < na; ++k) sorted[k] = L.desc[L.order[k]];
+  NBG_HIP(hipMemcpy(sv->S.desc.p, sorted.data(), na * sizeof(nbg::SysDesc), hipMemcpyHostToDevice));
+  NBG_HIP(hipMemset(sv->done.p, 0, sizeof(int)));
+  nbg::SysBufs B;
+  B.state = B.jac = B.ws = B.tt = B.dq = B.del = reinterpret_cast<double*>(base);
+  B.ints = reinterpret_cast<int64_t*>(base);
+  B.pair = base;
+  B.done_count = sv->done.p;
+  B.done_list = sv->done.p + 1;
+  double ms = 0.0;
+  if (int rc = run_groups(ctx, sv->S, L, sv->chunk, &ms, nullptr, nullptr, &B)) return rc;
+  sv->rounds += 1;
+  if (kernel_ms) *kernel_ms = ms;
+  for (ServeReq* r : sv->active) r->steps_run = std::min<int64_t>(r->d.nsteps + 1, r->steps_run + sv->chunk);
+  int nd = 0;
+  NBG_HIP(hipMemcpy(&nd, sv->done.p, sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int> tags((size_t)std::max(nd, 0));
+  if (nd > 0) NBG_HIP(hipMemcpy(tags.data(), sv->done.p + 1, sizeof(int) * nd, hipMemcpyDeviceToHost));
+  std::vector<char> gone(na, 0);
+  for (int tg : tags) {
+    if (tg < 0 || (size_t)tg >= na || gone[tg]) continue;
+    ServeReq* r = sv->active[tg];
+    const nbg::SysDesc& d = r->d;
+    const ReqLayout& Ly = r->lay;
+    const int n = d.n;
+    const size_t ntv = (size_t)d.ntbv * n * d.ntt, ndq = ntv * 7 * n;
+    r->tt.resize(ntv);
+    r->dq.resize(d.grad ? ndq : 0);
+    r->del.resize(d.grad ? ndq : 0);
+    r->count.resize(n);
+    std::vector<double> st((size_t)nbg::state_len(n));
+    if (ntv) NBG_HIP(hipMemcpy(r->tt.data(), base + r->off + Ly.tt, 8 * ntv, hipMemcpyDeviceToHost));
+    if (d.grad && ndq) {
+      NBG_HIP(hipMemcpy(r->dq.data(), base + r->off + Ly.dq, 8 * ndq, hipMemcpyDeviceToHost));
+      NBG_HIP(hipMemcpy(r->del.data(), base + r->off + Ly.del, 8 * ndq, hipMemcpyDeviceToHost));
+    }
+    NBG_HIP(hipMemcpy(r->count.data(), base + r->off + Ly.ints, 8 * (size_t)n, hipMemcpyDeviceToHost));
+    NBG_HIP(hipMemcpy(st.data(), base + r->off + Ly.state, 8 * st.size(), hipMemcpyDeviceToHost));
+    r->x.assign(st.begin() + nbg::off_x(n), st.begin() + nbg::off_x(n) + 3 * n);
+    r->v.assign(st.begin() + nbg::off_v(n), st.begin() + nbg::off_v(n) + 3 * n);
+    gone[tg] = 1;
+  }
+  std::vector<ServeReq*> keep;
+  int fin = 0;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (size_t k = 0; k < na; ++k) {
+      ServeReq* r = sv->active[k];
+      if (gone[k]) {
+        sv->arena.release(r->off, r->lay.bytes);
+        sv->finished[r->id] = r;
+        ++fin;
+      } else {
+        keep.push_back(r);
+      }
+    }
+  }
+  sv->active.swap(keep);
+  if (n_finished) *n_finished = fin;
+  return 0;
+}
+
+int nbg_server_finished(nbg_server* sv, int64_t* ids, int32_t max_ids, int32_t* n) {
+  if (!sv || !n || (max_ids > 0 && !ids)) return -1;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  int32_t k = 0;
+  for (const auto& kv : sv->finished) {
+    if (k >= max_ids) break;
+    ids[k++] = kv.first;
+  }
+  *n = k;
+  return 0;
+}
+
+int nbg_server_fetch(nbg_server* sv, int64_t id, double* tt, double* dtdq0, double* dtdelements, int64_t* count,
+                     double* x, double* v) {
+  if (!sv) return -1;
+  nbg_ctx* ctx = sv->ctx;
+  ServeReq* r = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->finished.find(id);
+    if (it != sv->finished.end()) {
+      r = it->second;
+      sv->finished.erase(it);
+    }
+  }
+  NBG_ARG(r != nullptr, "nbg_server_fetch: no finished request with this id");
+  if (tt) std::copy(r->tt.begin(), r->tt.end(), tt);
+  if (dtdq0) std::copy(r->dq.begin(), r->dq.end(), dtdq0);
+  if (dtdelements) std::copy(r->del.begin(), r->del.end(), dtdelements);
+  if (count) std::copy(r->count.begin(), r->count.end(), count);
+  if (x) std::copy(r->x.begin(), r->x.end(), x);
+  if (v) std::copy(r->v.begin(), r->v.end(), v);
+  delete r;
+  return 0;
+}
+
+int nbg_server_status(nbg_server* sv, int64_t* pending, int64_t* active, int64_t* finished, int64_t* bytes_free) {
+  if (!sv) return -1;
+  std::lock_guard<std::mutex> lk(sv->mu);
+  if (pending) *pending = (int64_t)sv->pending.size();
+  if (active) *active = (int64_t)sv->active.size();
+  if (finished) *finished = (int64_t)sv->finished.size();
+  if (bytes_free) *bytes_free = sv->arena.free_bytes();
+  return 0;
+}
+
+int64_t nbg_server_footprint(int32_t n, int32_t ntbv, int64_t ntt) { return Arena::round(req_layout(n, ntbv, ntt).bytes); }
 
 int nbg_copy_calibrate(nbg_ctx* ctx, int64_t bytes, int32_t reps, double* ms) {
   NBG_ARG(ctx && bytes > 0 && bytes % 16 == 0 && reps > 0 && ms, "nbg_copy_calibrate: bad arguments");

@@ -1032,7 +1032,7 @@ template <int NB> __device__ void dtdelements(Smem<NB>& S, const SysDesc& D, con
     for (int cp = 0; cp < N7; ++cp) jt[col + N7 * cp] = jinit[cp + (int64_t)N7 * col];
   wsync();
   const double* dq = B.dq + D.off_dq;
-  double* del = B.del + D.off_dq;
+  double* del = B.del + D.off_del;
   const int nb = D.ntbv;
   const int64_t ntt = D.ntt;
   for (int b = 0; b < nb; ++b)
@@ -1243,8 +1243,10 @@ __global__ __launch_bounds__(WAVE) void nbg_run(GroupList gl, SysBufs B, int64_t
     const int64_t before = ints[D.n];  // step cursor (lane 0 reads and writes it)
     dispatch_n<2, HI>(raw, D, B, chunk, lane);
     wsync();
-    if (lane == 0 && before <= D.nsteps && ints[D.n] > D.nsteps)  // finished during this visit
+    if (lane == 0 && before <= D.nsteps && ints[D.n] > D.nsteps) {  // finished during this visit
       atomicAdd(&units[D.n], (unsigned long long)(D.nsteps + (D.grad ? ints[D.n + 1] : 0)));
+      if (B.done_list) B.done_list[atomicAdd(B.done_count, 1)] = D.tag;
+    }
   }
 }
 

@@ -10,7 +10,8 @@
 //  ws     (double): two compressed 6n x 7n blocks (Jacobian copy during the
 //                   final gradient step of a transit)
 //  tt     (double): ntbv x n x ntt   (TransitTiming.tt / TransitParameters.ttbv)
-//  dq     (double): ntbv x n x ntt x 7 x n (dtdq0); del (double) same (dtdelements)
+//  dq     (double): ntbv x n x ntt x 7 x n (dtdq0); del (double) same (dtdelements,
+//                   at off_del)
 //  ints   (int64) : count[n] istep ntransits_done newton_iters_total
 //  pair   (uint8) : n x n
 #pragma once
@@ -38,13 +39,15 @@ struct SysDesc {
   int32_t ti;        // transited body, 0-based
   int32_t ntbv;      // 1 TransitTiming, 3 TransitParameters
   int32_t any_kick;  // some pair[i,j] (i<j) is true
+  int32_t tag;       // reported in SysBufs.done_list when the system finishes
+  int32_t pad;
   int64_t nsteps;    // full steps in the run
   int64_t ntt;
   double h;          // signed step used by the loop
   double hstep;      // intr.h (unsigned) for the first transit-time guess
   double hlast;      // MODE_STEPS: extra final step (0 = none)
   double t0;         // time at step 0 of the run (t = t0 + istep*h)
-  int64_t off_state, off_jac, off_ws, off_tt, off_dq, off_int, off_pair;
+  int64_t off_state, off_jac, off_ws, off_tt, off_dq, off_del, off_int, off_pair;
 };
 
 // The systems one launch may take: its own launch class first, then the
@@ -67,6 +70,8 @@ struct SysBufs {
   double* del;
   int64_t* ints;
   uint8_t* pair;
+  int* done_count;     // optional (null): tags of the systems that finished
+  int32_t* done_list;  // during a launch are appended here
 };
 
 // state slots

@@ -60,6 +60,16 @@ def lib():
     L.nbg_batch_destroy.argtypes = [VP]
     L.nbg_set_launch.argtypes = [VP, C.c_int32, C.c_int32]
     L.nbg_set_grid_cap.argtypes = [VP, C.c_int32]
+    L.nbg_server_create.argtypes = [VP, C.c_int64, C.c_int64, C.POINTER(VP)]
+    L.nbg_server_destroy.argtypes = [VP]
+    L.nbg_server_submit.argtypes = [VP, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int64, C.c_int32,
+                                    C.c_int32, D, D, D, U8, D, I64]
+    L.nbg_server_step.argtypes = [VP, I32, D]
+    L.nbg_server_finished.argtypes = [VP, I64, C.c_int32, I32]
+    L.nbg_server_fetch.argtypes = [VP, C.c_int64, D, D, D, I64, D, D]
+    L.nbg_server_status.argtypes = [VP, I64, I64, I64, I64]
+    L.nbg_server_footprint.argtypes = [C.c_int32, C.c_int32, C.c_int64]
+    L.nbg_server_footprint.restype = C.c_int64
     L.nbg_copy_calibrate.argtypes = [VP, C.c_int64, C.c_int32, D]
     L.nbg_selftest_pair_host.argtypes = [D, D, C.c_double, C.c_double, C.c_int32, D]
     _lib = L
