@@ -117,6 +117,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--serve", type=int, default=1000, metavar="REQUESTS",
+                    help="serving leg: the first REQUESTS requests of the C4 trace (0 = skip)")
+    ap.add_argument("--serve-tmax-scale", type=float, default=0.1)
+    ap.add_argument("--serve-chunk", type=int, default=128)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (read MIB, write MIB) through an elementwise "
                          "kernel: the reference reading for the FETCH_SIZE / WRITE_SIZE counters")
@@ -172,7 +176,10 @@ def main():
         launches[str(cls)] = {"kernel": "nbg_run<%d>" % cls, "ms": L["ms"], "units": sum(L["units"].values()),
                               "units_by_n": {str(n): u for n, u in sorted(L["units"].items())}, "flops": flops,
                               "tflops": flops / (L["ms"] / 1e3) / 1e12 if L["ms"] > 0 else 0.0}
-    dom = max(launches, key=lambda k: launches[k]["ms"])
+    # the launches of a batch run concurrently and end together (each takes
+    # its own class first, then smaller ones); the dominant one is the launch
+    # that carries the most algorithmic work
+    dom = max(launches, key=lambda k: launches[k]["flops"])
     achieved = launches[dom]["tflops"]
     total_flops = sum(L["flops"] for L in launches.values())
     log("download: %d systems, max count %d" % (len(res), max(int(r["count"].max()) for r in res)))
@@ -210,6 +217,30 @@ def main():
         "batch_tflops": total_flops / (stats["kernel_ms"] / 1e3) / 1e12,
         "transits_per_s": stats["transits"] / (ms_per_step / 1e3),
     }
+    if args.serve > 0:
+        from nbodygradient_amd.server import Server, serve
+        trace = Wk.c4(count=args.serve)
+        arrivals = np.array([t for t, _ in trace])
+        sspecs = [sp for _, sp in trace]
+        for sp in sspecs:
+            sp.tmax *= args.serve_tmax_scale
+        sstates = Wk.initial_states(sspecs)
+        sv = Server(max_bytes=32 << 30, chunk=args.serve_chunk)
+        sres = serve(sv, sspecs, sstates, arrivals, log=log)
+        sv.close()
+        lat = sres["latency_s"]
+        ssteps = sum(sp.nsteps for sp in sspecs)
+        out["serving"] = {
+            "workload": "C4 trace: first %d requests, Poisson arrivals at 2000/s, n~U{2..10}, tmax log-U[100,8000] d "
+                        "x%g, h=P_in/30, Jacobian + transit times + dt/delements" % (len(sspecs), args.serve_tmax_scale),
+            "chunk_steps": args.serve_chunk, "p50_s": float(np.percentile(lat, 50)),
+            "p99_s": float(np.percentile(lat, 99)), "max_s": float(np.max(lat)),
+            "sustained_requests_per_s": len(sspecs) / sres["wall_s"], "wall_s": sres["wall_s"],
+            "main_steps_per_s": ssteps / sres["wall_s"], "rounds": sres["rounds"],
+            "offered_requests_per_s": len(sspecs) / float(arrivals[-1])}
+        log("serving: %d requests, p50 %.3f s, p99 %.3f s, %.1f requests/s sustained, %.3g main steps/s" %
+            (len(sspecs), out["serving"]["p50_s"], out["serving"]["p99_s"], out["serving"]["sustained_requests_per_s"],
+             out["serving"]["main_steps_per_s"]))
     if not args.no_plain:
         lat = plain_latency(specs, states)
         t_plain = 0.0
