@@ -173,7 +173,8 @@ def main():
     launches = {}
     for cls, L in stats["launches"].items():
         flops = sum(W(n) * u for n, u in L["units"].items())
-        launches[str(cls)] = {"kernel": "nbg_run<%d>" % cls, "ms": L["ms"], "units": sum(L["units"].values()),
+        launches[str(cls)] = {"kernel": "nbg_run<%d>" % cls, "ms": L["ms"], "grid": L["grid"],
+                              "units": sum(L["units"].values()),
                               "units_by_n": {str(n): u for n, u in sorted(L["units"].items())}, "flops": flops,
                               "tflops": flops / (L["ms"] / 1e3) / 1e12 if L["ms"] > 0 else 0.0}
     # the launches of a batch run concurrently and end together (each takes
@@ -185,8 +186,8 @@ def main():
     log("download: %d systems, max count %d" % (len(res), max(int(r["count"].max()) for r in res)))
     for n in sorted(launches, key=int):
         L = launches[n]
-        log("launch %s: %.1f ms, %d units %s, %.3f TFLOP/s" % (L["kernel"], L["ms"], L["units"], L["units_by_n"],
-                                                               L["tflops"]))
+        log("launch %s: grid %d, %.1f ms, %d units %s, %.3f TFLOP/s" % (L["kernel"], L["grid"], L["ms"], L["units"],
+                                                                        L["units_by_n"], L["tflops"]))
     out = {
         "metric": METRIC,
         "value": value,

@@ -93,6 +93,7 @@ typedef struct {
    * smaller classes.  Entries of absent classes are 0. */
   double launch_ms[13];
   int64_t launch_units[13][13];
+  int32_t launch_grid[13];  /* workgroups (one wave each) of that launch */
 } nbg_batch_stats;
 
 int nbg_batch_create(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* tmax,

@@ -13,6 +13,8 @@
 #include <mutex>
 #include <numeric>
 #include <string>
+#include <chrono>
+#include <thread>
 #include <vector>
 
 #include "../../include/nbg.h"
@@ -115,7 +117,7 @@ void group_layout(Layout& L) {
     const auto& d = L.desc[i];
     const double n = d.n;
     const double steps = L.cost.empty() ? (double)(d.nsteps + 1) : L.cost[i];
-    return steps * std::max(10.0, 4.6 * n * n + 39.6 * n - 67.0);
+    return steps * std::max(10.0, 3.0 * n * n + 39.0 * n - 55.0);
   };
   std::stable_sort(L.order.begin(), L.order.end(), [&](int a, int b) {
     const int ca = nbg_class_of(L.desc[a].n), cb = nbg_class_of(L.desc[b].n);
@@ -235,7 +237,7 @@ nbg::SysBufs bufs_of(Store& S) {
 // Launch every group (one persistent kernel per body count) and wait.
 int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* kernel_ms,
                std::vector<float>* group_ms = nullptr, std::vector<unsigned long long>* units = nullptr,
-               const nbg::SysBufs* bufs = nullptr) {
+               const nbg::SysBufs* bufs = nullptr, std::vector<int>* group_grid = nullptr) {
   NBG_HIP(hipSetDevice(ctx->device));
   const size_t G = L.gn.size();
   if (G > (size_t)nbg::MAX_GROUPS) {
@@ -255,9 +257,9 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
   // proportion to each class's estimated work (CU-time: the per-step cost
   // model times the CU fraction one workgroup holds, by LDS or by its wave
   // slot, one wave per SIMD at this kernel's register use), every class but
-  // the smallest with a margin (only larger launches can help a class), so
-  // that every workgroup is resident from the start and the long systems of
-  // the large-n classes start first.
+  // the smallest with a margin -- the largest the widest, since no launch can
+  // take its systems -- so that every workgroup is resident from the start
+  // and the long systems of the large-n classes start first.
   std::vector<int> grids(G);
   std::vector<double> frac(G), demand(G);
   double tot = 0.0;
@@ -274,19 +276,23 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
       continue;
     }
     double cus = ctx->cu_count * (demand[g] / tot);
-    if (g + 1 < G) cus *= 1.15;
+    if (g == 0) cus *= 1.35;
+    else if (g + 1 < G) cus *= 1.1;
     else cus = std::max(ctx->cu_count - used, 0.25 * ctx->cu_count * (demand[g] / tot));
     grids[g] = std::max(1, std::min(full, (int)(cus / frac[g])));
     used += grids[g] * frac[g];
   }
-  // launches longest first, each on the stream with the least queued work
-  // (launches on one stream run one after the other)
+  if (group_grid) *group_grid = grids;
+  // Launch order: the largest class first, alone for a moment, so that its
+  // workgroups (the largest LDS blocks, the hardest to place) are resident
+  // before the smaller classes fill the remaining LDS of every CU; then the
+  // others, each on the stream with the least queued work (launches on one
+  // stream run one after the other).
   std::vector<size_t> gord(G);
   std::iota(gord.begin(), gord.end(), 0);
-  std::stable_sort(gord.begin(), gord.end(),
-                   [&](size_t a, size_t b) { return L.gcost[a] / grids[a] > L.gcost[b] / grids[b]; });
   std::vector<double> load(ns, 0.0);
   for (size_t g : gord) {
+    if (g == 1) std::this_thread::sleep_for(std::chrono::microseconds(2000));
     nbg::GroupList gl{};
     for (size_t h = g; h < G; ++h) {  // own class, then the smaller ones
       gl.nsys[gl.ng] = L.gcnt[h];
@@ -694,7 +700,8 @@ int nbg_batch_run(nbg_batch* b, nbg_batch_stats* stats) {
   double ms = 0.0;
   std::vector<float> gms;
   std::vector<unsigned long long> gun;
-  if (int rc = run_groups(ctx, b->S, b->L, 0, &ms, &gms, &gun)) return rc;
+  std::vector<int> ggrid;
+  if (int rc = run_groups(ctx, b->S, b->L, 0, &ms, &gms, &gun, nullptr, &ggrid)) return rc;
   if (stats) {
     std::memset(stats, 0, sizeof(*stats));
     std::vector<int64_t> hi((size_t)b->L.n_int);
@@ -711,6 +718,7 @@ int nbg_batch_run(nbg_batch* b, nbg_batch_stats* stats) {
     for (size_t g = 0; g < b->L.gn.size(); ++g) {
       const int c = b->L.gn[g];
       stats->launch_ms[c] = gms[g];
+      stats->launch_grid[c] = ggrid[g];
       for (int n = 0; n < 13; ++n) stats->launch_units[c][n] = (int64_t)gun[13 * g + n];
     }
   }

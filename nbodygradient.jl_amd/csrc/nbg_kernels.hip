@@ -52,10 +52,12 @@ template <int NB> struct Smem {
   static constexpr int CH = NP < CH_MIN ? NP : (CH_MIN > 64 ? 64 : CH_MIN);  // pairs per sweep chunk
   // Lifetimes are disjoint: jac_phi/dadq live in the force-gradient stage
   // (dadq also holds jac_kick around the kicks), the pair slots in the sweeps.
+  static constexpr int PQ = 17;  // per-pair factors of phisalpha!, see phi_build
   union {
     struct {
       double phi[N3 * N4];   // jac_phi block: row 3b+k (velocity k of body b), col 3d+p (x_p of d) or N3+d (m_d)
       double dadq[N3 * N4];  // d a / d q: row 3b+k, same columns
+      double pq[(NP > 0 ? NP : 1) * PQ];  // per pair (lexicographic index)
     } f;
     PairSlot slot[CH];
   } u;
@@ -65,7 +67,11 @@ template <int NB> struct Smem {
   uint8_t plist[2][2 * (NP > 0 ? NP : 1)];
   uint8_t lvw[2][NP > 0 ? NP : 1];
   int nlv[2];
-  double jij[14 * 14];   // jac_ij, column-major as in the reference
+  // Source of jac_ij for a pair whose masses sum to zero (the reference leaves
+  // jac_ij from the pair before it, ahl21.jl:712-713): the last pair with mass
+  // when it lies in an earlier chunk, or zero at the start of a step.
+  PairSlot jlast;
+  int seq;  // the system has such a pair: sweeps keep the sequential order
   double dqij[14];       // dqdt_ij
   double x[N3], v[N3], xe[N3], ve[N3], m[NB];
   double px[N3], pv[N3], pxe[N3], pve[N3];  // s_prior
@@ -337,15 +343,39 @@ template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick,
         wsync();
       }
   }
-  // phisalpha! (ahl21.jl:558-700, alpha = 2) for the Kepler pairs, one row
-  // block (the three rows of body b) at a time: a lane keeps the block's rows
-  // of its dadq and jac_phi columns in registers while it accumulates them.
-  // Body b's rows receive contributions only from the pairs that contain b,
-  // and in the reference's lexicographic pair order those come partner by
-  // partner in increasing order; looping over the partner c in increasing
-  // order therefore repeats every accumulation in the reference's order.
-  // a and dadq start at zero (ahl21.jl:559-563); jac_phi keeps phic!'s terms.
+  // phisalpha! (ahl21.jl:558-700, alpha = 2) for the Kepler pairs.
+  // First one lane per pair evaluates the pair's factors (the divisions and
+  // square roots) into pq; then, one row block (the three rows of body b) at
+  // a time, a lane keeps the block's rows of its dadq and jac_phi columns in
+  // registers and accumulates them from pq.  Body b's rows receive
+  // contributions only from the pairs that contain b, and in the reference's
+  // lexicographic pair order those come partner by partner in increasing
+  // order; looping over the partner c in increasing order therefore repeats
+  // every accumulation in the reference's order.  a and dadq start at zero
+  // (ahl21.jl:559-563); jac_phi keeps phic!'s terms.
+  constexpr int NP = Smem<NB>::NP, PQ = Smem<NB>::PQ;
+  double* pq = S.u.f.pq;
+  auto pidx = [](int i, int j) { return i * (2 * NB - i - 1) / 2 + (j - i - 1); };
   const double coeff = (((2.0 * (h * h * h)) / 96.0) * 2.0) * NBG_GNEWT;
+  // loop 1 factors (ahl21.jl:569-575): fac2 = G/r^3, fac_k, f3_k = fac_k * 3/r^2
+  for (int p = lane; p < NP; p += WAVE) {
+    int i = 0, q = p;
+    while (q >= NB - 1 - i) { q -= NB - 1 - i; ++i; }
+    const int j = i + 1 + q;
+    double rij[3];
+    for (int k = 0; k < 3; ++k) rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
+    const double r2 = dot3(rij);
+    const double r3 = r2 * __builtin_sqrt(r2);
+    const double fac2 = NBG_GNEWT / r3;
+    double* o = pq + p * PQ;
+    o[0] = fac2;
+    for (int k = 0; k < 3; ++k) {
+      const double fac = fac2 * rij[k];
+      o[1 + k] = fac;
+      o[4 + k] = fac * (3.0 / r2);
+    }
+  }
+  wsync();
   // loop 1: accelerations and dadq (ahl21.jl:567-602)
 #pragma unroll 1
   for (int b = 0; b < NB; ++b) {
@@ -356,18 +386,16 @@ template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick,
       if (c == b) continue;
       const int i = c < b ? c : b, j = c < b ? b : c;
       if (S.pair[i + NB * j]) continue;
-      double rij[3];
-      for (int k = 0; k < 3; ++k) rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
-      const double r2 = dot3(rij);
-      const double r3 = r2 * __builtin_sqrt(r2);
-      const double fac2 = NBG_GNEWT / r3;
+      const double* o = pq + pidx(i, j) * PQ;
+      const double fac2 = o[0];
+      const int ry = myp < 3 ? myp : 0;
+      const double rmy = S.x[3 * i + ry] - S.x[3 * j + ry];
       const double mi = S.m[i], mj = S.m[j];
-      const double rmy = myp == 0 ? rij[0] : (myp == 1 ? rij[1] : rij[2]);
       const bool bi = (b == i);
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const double fac = fac2 * rij[k];
-        const double f3 = fac * (3.0 / r2);
+        const double fac = o[1 + k];
+        const double f3 = o[4 + k];
         double d = db[k];
         if (bi) {
           ab[k] -= mj * fac;
@@ -393,6 +421,37 @@ template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick,
       for (int k = 0; k < 3; ++k) S.a[3 * b + k] = ab[k];
   }
   wsync();
+  // loop 2 factors (ahl21.jl:609-640), needing the accelerations above
+  for (int p = lane; p < NP; p += WAVE) {
+    int i = 0, q = p;
+    while (q >= NB - 1 - i) { q -= NB - 1 - i; ++i; }
+    const int j = i + 1 + q;
+    double rij[3], aij[3];
+    for (int k = 0; k < 3; ++k) {
+      aij[k] = S.a[3 * i + k] - S.a[3 * j + k];
+      rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
+    }
+    const double r2 = dot3(rij);
+    const double r1 = __builtin_sqrt(r2);
+    const double ardot = dot3(aij, rij);
+    const double fac1 = coeff / ((r2 * r2) * r1);
+    const double mi = S.m[i], mj = S.m[j];
+    const double fac2 = ((2.0 * NBG_GNEWT) * (mi + mj)) / r1 + 3.0 * ardot;
+    double* o = pq + p * PQ;
+    for (int k = 0; k < 3; ++k) {
+      const double fac = fac1 * (rij[k] * fac2 - r2 * aij[k]);
+      o[k] = fac;                                                  // fac
+      o[3 + k] = fac * (5.0 / r2);                                 // f5
+      o[6 + k] = (((2.0 * NBG_GNEWT) * fac1) * rij[k]) / r1;       // fm
+      o[9 + k] = ((-2.0) * fac1) * ((((rij[k] * NBG_GNEWT) * (mi + mj)) / (r2 * r1)) + aij[k]);  // fr
+      // (fac1 * 3) * rij[k] multiplies aij[p] in fac3 and is ff = (3 * fac1) * rij[k]
+      // as well (IEEE multiplication commutes)
+      o[12 + k] = (fac1 * 3.0) * rij[k];
+    }
+    o[15] = fac1 * fac2;   // fd
+    o[16] = (-fac1) * r2;  // fa
+  }
+  wsync();
   // loop 2: velocity corrections, dqdt_phi and jac_phi (ahl21.jl:607-698)
   const double q3h = 3.0 / h;
 #pragma unroll 1
@@ -409,9 +468,9 @@ template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick,
       if (c == b) continue;
       const int i = c < b ? c : b, j = c < b ? b : c;
       if (S.pair[i + NB * j]) continue;
-      double rij[3], aij[3], Di[3], Dj[3];
+      const double* o = pq + pidx(i, j) * PQ;
+      double rij[3], Di[3], Dj[3];
       for (int k = 0; k < 3; ++k) {
-        aij[k] = S.a[3 * i + k] - S.a[3 * j + k];
         rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
         Di[k] = own ? D[(3 * i + k) * N4 + lane] : 0.0;
         Dj[k] = own ? D[(3 * j + k) * N4 + lane] : 0.0;
@@ -419,27 +478,18 @@ template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick,
       double dot = 0.0;
       if (own)
         for (int k = 0; k < 3; ++k) dot += rij[k] * (Di[k] - Dj[k]);
-      const double r2 = dot3(rij);
-      const double r1 = __builtin_sqrt(r2);
-      const double ardot = dot3(aij, rij);
-      const double fac1 = coeff / ((r2 * r2) * r1);
+      const int ry = myp < 3 ? myp : 0;
+      const double rmy = ry == 0 ? rij[0] : (ry == 1 ? rij[1] : rij[2]);
+      const double amy = S.a[3 * i + ry] - S.a[3 * j + ry];
       const double mi = S.m[i], mj = S.m[j];
-      const double fac2 = ((2.0 * NBG_GNEWT) * (mi + mj)) / r1 + 3.0 * ardot;
-      const double rmy = myp == 0 ? rij[0] : (myp == 1 ? rij[1] : rij[2]);
-      const double amy = myp == 0 ? aij[0] : (myp == 1 ? aij[1] : aij[2]);
       const bool ij = myp < 3 && (myd == i || myd == j);
       const bool bi = (b == i);
+      const double fd = o[15], fa = o[16];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const double fac = fac1 * (rij[k] * fac2 - r2 * aij[k]);
-        const double f5 = fac * (5.0 / r2);
-        const double fm = (((2.0 * NBG_GNEWT) * fac1) * rij[k]) / r1;
-        const double fd = fac1 * fac2;
-        const double fr = ((-2.0) * fac1) * ((((rij[k] * NBG_GNEWT) * (mi + mj)) / (r2 * r1)) + aij[k]);
-        const double fac3 = ij ? fr * rmy + (((fac1 * 3.0) * rij[k]) * amy) : 0.0;
-        const double fa = (-fac1) * r2;
+        const double fac = o[k], f5 = o[3 + k], fm = o[6 + k], fr = o[9 + k], ff = o[12 + k];
+        const double fac3 = ij ? fr * rmy + o[12 + k] * amy : 0.0;
         const double diff = Di[k] - Dj[k];
-        const double ff = (3.0 * fac1) * rij[k];
         double pv = pb[k];
         if (bi) {
           kahan(vb[k], veb[k], mj * fac);
@@ -549,9 +599,27 @@ template <int NB> __device__ inline void drift_cols(Smem<NB>& S, int c, double h
     }
 }
 
+// jac_ij (ahl21.jl:734-758) is a function of the pair's slot: rows 0-5 and
+// 7-12 (body i, body j), column k.  jac_ij starts at zero and accumulates
+// +-m*kep (ahl21.jl:735-742); the mass columns 6 and 13 are set from jac_mass,
+// delxv and kep's k-derivative column; rows 6 and 13 stay zero.
+__device__ inline double jij_coef(const PairSlot& sl, bool rowi, int rk, int k) {
+  const double mi = sl.mi, mj = sl.mj, mijinv = sl.mijinv;
+  if (k == 6)
+    return rowi ? sl.jm[rk] * sl.m_j : (((-mj) * sl.delxv[rk]) * mijinv) - ((NBG_GNEWT * mi) * sl.kep[rk + 36]);
+  if (k == 13)
+    return rowi ? ((mi * sl.delxv[rk]) * mijinv) + ((NBG_GNEWT * mj) * sl.kep[rk + 36]) : (-sl.jm[rk]) * sl.m_i;
+  const int ck = k < 7 ? k : k - 7;
+  const double kv = sl.kep[rk + 6 * ck];
+  if (rowi == (k < 7)) return 0.0 + (rowi ? mj : mi) * kv;
+  return 0.0 - (rowi ? mj : mi) * kv;
+}
+
 // Pair map on rows (i, j): copy_submatrix!, mul!, comp_sum_matrix!,
-// ypoc_submatrix! (ahl21.jl:31-42, utils.jl:51-100).
-template <int NB> __device__ inline void pair_cols(Smem<NB>& S, int i, int j, int c) {
+// ypoc_submatrix! (ahl21.jl:31-42, utils.jl:51-100) for column c, with
+// jac_ij evaluated from the pair's slot (every lane forms the same
+// coefficients; each row's products are summed over k in ascending order).
+template <int NB> __device__ inline void pair_cols(Smem<NB>& S, const PairSlot& sl, int i, int j, int c) {
   constexpr int N7 = 7 * NB;
   const int ri = 6 * i, rj = 6 * j;
   double a[14], e[12];
@@ -566,13 +634,11 @@ template <int NB> __device__ inline void pair_cols(Smem<NB>& S, int i, int j, in
   a[13] = (c == 7 * j + 6) ? 1.0 : 0.0;
   double o[12];
 #pragma unroll
-  for (int q = 0; q < 12; ++q) {
-    const int r = q < 6 ? q : q + 1;
-    double s = 0.0;
+  for (int q = 0; q < 12; ++q) o[q] = 0.0;
 #pragma unroll
-    for (int k = 0; k < 14; ++k) s = fmad(S.jij[r + 14 * k], a[k], s);
-    o[q] = s;
-  }
+  for (int k = 0; k < 14; ++k)
+#pragma unroll
+    for (int q = 0; q < 12; ++q) o[q] = fmad(jij_coef(sl, q < 6, q < 6 ? q : q - 6, k), a[k], o[q]);
 #pragma unroll
   for (int q = 0; q < 12; ++q) {
     const int src = q < 6 ? q : q + 1;
@@ -603,7 +669,7 @@ template <int NB> __device__ inline void pair_cols(Smem<NB>& S, int i, int j, in
 // gives the reference's results exactly.
 // ---------------------------------------------------------------------------
 template <int NB>
-__device__ void sweep(Smem<NB>& S, int dir, double hh, bool needJ, bool needQ, int lane) {
+__device__ __attribute__((always_inline)) inline void sweep(Smem<NB>& S, int dir, double hh, bool needJ, bool needQ, int lane) {
   constexpr int N7 = 7 * NB;
   constexpr int CH = Smem<NB>::CH;
   const bool drift_first = (dir == 0);
@@ -659,41 +725,24 @@ __device__ void sweep(Smem<NB>& S, int dir, double hh, bool needJ, bool needQ, i
     if (lane < cnt && S.u.slot[lane].valid) pair_jacobian(&S.u.slot[lane], drift_first);
     wsync();
     // ---- C: jac_ij, Jacobian rows and dq/dt, in sweep order
+    int lastq = -1;  // last slot of this chunk whose pair has mass
 #pragma unroll 1
     for (int q = 0; q < cnt; ++q) {
       const PairSlot& sl = S.u.slot[q];
       const int i = sl.i, j = sl.j;
       if (sl.valid) {
-        // (gm == 0: the reference returns before touching jac_ij / dqdt_ij and
-        // the caller reuses the previous values; S.jij / S.dqij keep them.)
-        const double mi = sl.mi, mj = sl.mj, mijinv = sl.mijinv;
-        for (int e = lane; e < 196; e += WAVE) {
-          const int r = e % 14, c = e / 14;
-          double val = 0.0;
-          if (r != 6 && r != 13) {
-            const int rk = r < 7 ? r : r - 7;
-            if (c == 6) {
-              val = r < 7 ? sl.jm[rk] * sl.m_j : (((-mj) * sl.delxv[rk]) * mijinv) - ((NBG_GNEWT * mi) * sl.kep[rk + 36]);
-            } else if (c == 13) {
-              val = r < 7 ? ((mi * sl.delxv[rk]) * mijinv) + ((NBG_GNEWT * mj) * sl.kep[rk + 36]) : (-sl.jm[rk]) * sl.m_i;
-            } else {
-              const int ck = c < 7 ? c : c - 7;
-              const double kv = sl.kep[rk + 6 * ck];
-              // jac_ij starts at zero and accumulates +-m*kep (ahl21.jl:735-742)
-              if ((r < 7) == (c < 7)) val = 0.0 + (r < 7 ? mj : mi) * kv;
-              else val = 0.0 - (r < 7 ? mj : mi) * kv;
-            }
-          }
-          S.jij[e] = val;
-        }
-        if (lane < 6) {
-          S.dqij[lane] = mj * sl.kep[lane + 42];
-          S.dqij[7 + lane] = (-mi) * sl.kep[lane + 42];
+        lastq = q;
+        if (needQ && lane < 6) {  // dqdt_ij (ahl21.jl:743-748)
+          S.dqij[lane] = sl.mj * sl.kep[lane + 42];
+          S.dqij[7 + lane] = (-sl.mi) * sl.kep[lane + 42];
         }
       }
-      wsync();
+      // (gm == 0: the reference returns before touching jac_ij / dqdt_ij and
+      // the caller reuses the previous values: the last pair with mass, or
+      // jlast when that lies in an earlier chunk; dqij keeps its values.)
+      const PairSlot& src = lastq >= 0 ? S.u.slot[lastq] : S.jlast;
       if (needJ)
-        for (int c = lane; c < N7; c += WAVE) pair_cols<NB>(S, i, j, c);
+        for (int c = lane; c < N7; c += WAVE) pair_cols<NB>(S, src, i, j, c);
       if (needQ) {
         double q1[14];
         for (int k = 0; k < 7; ++k) {
@@ -702,12 +751,25 @@ __device__ void sweep(Smem<NB>& S, int dir, double hh, bool needJ, bool needQ, i
         }
         if (lane < 14) {
           double s2 = 0.0;
-          for (int k = 0; k < 14; ++k) s2 = fmad(S.jij[lane + 14 * k], q1[k], s2);
+          if (lane != 6 && lane != 13) {
+            const bool rowi = lane < 7;
+            const int rk = rowi ? lane : lane - 7;
+            for (int k = 0; k < 14; ++k) s2 = fmad(jij_coef(src, rowi, rk, k), q1[k], s2);
+          } else {
+            for (int k = 0; k < 14; ++k) s2 = fmad(0.0, q1[k], s2);
+          }
           const double nd = (S.dqij[lane] * 0.5) + (q1[lane] + s2);
           S.dqij[lane] = nd;
           S.dqdt[lane < 7 ? 7 * i + lane : 7 * j + lane - 7] = nd;
         }
       }
+      wsync();
+    }
+    // keep the last pair with mass for a massless pair at the start of a later chunk
+    if (S.seq && lastq >= 0) {
+      const double* from = reinterpret_cast<const double*>(&S.u.slot[lastq]);
+      double* to = reinterpret_cast<double*>(&S.jlast);
+      for (int w = lane; w < (int)(sizeof(PairSlot) / sizeof(double)); w += WAVE) to[w] = from[w];
       wsync();
     }
   }
@@ -725,7 +787,9 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
   const double h6 = h / 6.0;
   // zero_out!(d); fill!(s.dqdt, 0)
   if (lane < 14) S.dqij[lane] = 0.0;
-  for (int q = lane; q < 196; q += WAVE) S.jij[q] = 0.0;
+  if (S.seq)  // jac_ij = 0 at the start of the step (zero_out!, ahl21.jl:8)
+    for (int w = lane; w < (int)(sizeof(PairSlot) / sizeof(double)); w += WAVE)
+      reinterpret_cast<double*>(&S.jlast)[w] = 0.0;
   if (needQ)
     for (int r = lane; r < N7; r += WAVE) {
       S.dqdt[r] = 0.0;
@@ -1079,6 +1143,7 @@ template <int NB> __device__ void load_system(Smem<NB>& S, const SysDesc& D, con
     for (int i = 0; i < NB - 1; ++i)
       for (int j = i + 1; j < NB; ++j)
         if (!S.pair[i + NB * j] && NBG_GNEWT * (S.m[i] + S.m[j]) == 0.0) seq = true;
+    S.seq = seq ? 1 : 0;
     for (int dir = 0; dir < 2; ++dir) {
       uint8_t pi_[NPS], pj_[NPS], lv[NPS];
       int np = 0;

@@ -59,7 +59,7 @@ class Batch:
         self.ctx.check(_lib.lib().nbg_batch_run(self._h, C.byref(st)), "nbg_batch_run")
         return {"system_steps": st.system_steps, "transits": st.transits, "newton_iters": st.newton_iters,
                 "kernel_ms": st.kernel_ms,
-                "launches": {c: {"ms": st.launch_ms[c],
+                "launches": {c: {"ms": st.launch_ms[c], "grid": st.launch_grid[c],
                                  "units": {n: st.launch_units[c][n] for n in range(13) if st.launch_units[c][n] > 0}}
                              for c in range(13) if st.launch_ms[c] > 0}}
 
