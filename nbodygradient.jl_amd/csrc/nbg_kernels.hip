@@ -45,7 +45,12 @@ template <int NB> struct Smem {
   static constexpr int JOFF = N6 * N7;
   // jac_step without mass rows (row 6b+k <-> Julia row 7b+k), then jac_error
   // in the same layout at JOFF.  2*N6*N7 >= N7*N7, so jac_init fits here too.
-  double JJ[2 * N6 * N7];
+  // From JGLOBAL_MIN bodies on, these live in an HBM work area (jg) and the
+  // LDS keeps everything else: four such systems fit a CU instead of two.
+  // Every access to them is made by the lane that owns the column.
+  static constexpr bool JG = NB >= JGLOBAL_MIN;
+  double JJ[JG ? 1 : 2 * N6 * N7];
+  double* jg;
   static constexpr int NP = NB * (NB - 1) / 2;  // pairs per sweep
   static constexpr int CH_FIT = (int)((2 * N3 * N4 * sizeof(double)) / sizeof(PairSlot));
   static constexpr int CH_MIN = CH_FIT < 4 ? 4 : CH_FIT;
@@ -82,6 +87,15 @@ template <int NB> struct Smem {
   int64_t count[NB];
   uint8_t pair[NB * NB];
 };
+
+template <int NB> __device__ __forceinline__ double* jj(Smem<NB>& S) {
+  if constexpr (Smem<NB>::JG) return S.jg;
+  else return S.JJ;
+}
+template <int NB> __device__ __forceinline__ const double* jj(const Smem<NB>& S) {
+  if constexpr (Smem<NB>::JG) return S.jg;
+  else return S.JJ;
+}
 
 __device__ inline void wsync() { __syncthreads(); }
 
@@ -543,17 +557,19 @@ template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick,
 // ---------------------------------------------------------------------------
 // Position rows of column c (pre-update), the operand of K*J and Phi*J.
 template <int NB> __device__ inline void load_pos(const Smem<NB>& S, int c, double* P) {
+  const double* const J_ = jj<NB>(S);
   constexpr int N7 = 7 * NB;
 #pragma unroll
   for (int d = 0; d < NB; ++d)
 #pragma unroll
-    for (int p = 0; p < 3; ++p) P[3 * d + p] = S.JJ[(6 * d + p) * N7 + c];
+    for (int p = 0; p < 3; ++p) P[3 * d + p] = J_[(6 * d + p) * N7 + c];
 }
 
 // J[velocity rows, c] += (B * J)[velocity rows, c] with Kahan compensation,
 // B = K or Phi (3n x 4n block); then the zero-addend Kahan pass over the
 // position rows that comp_sum_matrix! applies to the rest of the matrix.
 template <int NB> __device__ inline void vel_product_add(Smem<NB>& S, const double* B, int c, const double* P) {
+  double* const J_ = jj<NB>(S);
   constexpr int N3 = 3 * NB, N4 = 4 * NB, N7 = 7 * NB;
 #pragma unroll 1
   for (int rv = 0; rv < N3; ++rv) {
@@ -568,19 +584,20 @@ template <int NB> __device__ inline void vel_product_add(Smem<NB>& S, const doub
     }
     const int b = rv / 3, k = rv - 3 * b;
     const int r = 6 * b + 3 + k;
-    kahan(S.JJ[r * N7 + c], S.JJ[Smem<NB>::JOFF + r * N7 + c], s);
+    kahan(J_[r * N7 + c], J_[Smem<NB>::JOFF + r * N7 + c], s);
   }
 }
 
 template <int NB> __device__ inline void zero_add_rows(Smem<NB>& S, int c, bool pos, bool vel) {
+  double* const J_ = jj<NB>(S);
   constexpr int N7 = 7 * NB;
 #pragma unroll 1
   for (int b = 0; b < NB; ++b)
     for (int k = 0; k < 6; ++k) {
       if ((k < 3 && !pos) || (k >= 3 && !vel)) continue;
       const int r = 6 * b + k;
-      double& J = S.JJ[r * N7 + c];
-      double& E = S.JJ[Smem<NB>::JOFF + r * N7 + c];
+      double& J = J_[r * N7 + c];
+      double& E = J_[Smem<NB>::JOFF + r * N7 + c];
       E += 0.0;
       const double t = J + E;
       E += J - t;
@@ -590,12 +607,13 @@ template <int NB> __device__ inline void zero_add_rows(Smem<NB>& S, int c, bool 
 
 // drift_grad! Jacobian part (ahl21.jl:326-328)
 template <int NB> __device__ inline void drift_cols(Smem<NB>& S, int c, double hh) {
+  double* const J_ = jj<NB>(S);
   constexpr int N7 = 7 * NB;
 #pragma unroll 1
   for (int b = 0; b < NB; ++b)
     for (int k = 0; k < 3; ++k) {
       const int r = 6 * b + k;
-      kahan(S.JJ[r * N7 + c], S.JJ[Smem<NB>::JOFF + r * N7 + c], hh * S.JJ[(r + 3) * N7 + c]);
+      kahan(J_[r * N7 + c], J_[Smem<NB>::JOFF + r * N7 + c], hh * J_[(r + 3) * N7 + c]);
     }
 }
 
@@ -620,15 +638,16 @@ __device__ inline double jij_coef(const PairSlot& sl, bool rowi, int rk, int k) 
 // jac_ij evaluated from the pair's slot (every lane forms the same
 // coefficients; each row's products are summed over k in ascending order).
 template <int NB> __device__ inline void pair_cols(Smem<NB>& S, const PairSlot& sl, int i, int j, int c) {
+  double* const J_ = jj<NB>(S);
   constexpr int N7 = 7 * NB;
   const int ri = 6 * i, rj = 6 * j;
   double a[14], e[12];
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
-    a[k] = S.JJ[(ri + k) * N7 + c];
-    a[7 + k] = S.JJ[(rj + k) * N7 + c];
-    e[k] = S.JJ[Smem<NB>::JOFF + (ri + k) * N7 + c];
-    e[6 + k] = S.JJ[Smem<NB>::JOFF + (rj + k) * N7 + c];
+    a[k] = J_[(ri + k) * N7 + c];
+    a[7 + k] = J_[(rj + k) * N7 + c];
+    e[k] = J_[Smem<NB>::JOFF + (ri + k) * N7 + c];
+    e[6 + k] = J_[Smem<NB>::JOFF + (rj + k) * N7 + c];
   }
   a[6] = (c == 7 * i + 6) ? 1.0 : 0.0;
   a[13] = (c == 7 * j + 6) ? 1.0 : 0.0;
@@ -648,8 +667,8 @@ template <int NB> __device__ inline void pair_cols(Smem<NB>& S, const PairSlot& 
     se += sv - t;
     sv = t;
     const int r = q < 6 ? ri + q : rj + (q - 6);
-    S.JJ[r * N7 + c] = sv;
-    S.JJ[Smem<NB>::JOFF + r * N7 + c] = se;
+    J_[r * N7 + c] = sv;
+    J_[Smem<NB>::JOFF + r * N7 + c] = se;
   }
 }
 
@@ -968,6 +987,7 @@ template <int NB> __device__ inline void restore_prior(Smem<NB>& S, int lane) {
 template <int NB>
 __device__ void findtransit(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int ti, int jb, double dt0, int lane,
                             double* ws) {
+  double* const J_ = jj<NB>(S);
   constexpr int N6 = 6 * NB, N7 = 7 * NB;
   const int n = NB;
   double stmp = 0.0;
@@ -994,8 +1014,8 @@ __device__ void findtransit(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int
     // keep the prior Jacobian: the step below updates J in place
     for (int col = lane; col < N7; col += WAVE)
       for (int r = 0; r < N6; ++r) {
-        ws[r * N7 + col] = S.JJ[r * N7 + col];
-        ws[N6 * N7 + r * N7 + col] = S.JJ[Smem<NB>::JOFF + r * N7 + col];
+        ws[r * N7 + col] = J_[r * N7 + col];
+        ws[N6 * N7 + r * N7 + col] = J_[Smem<NB>::JOFF + r * N7 + col];
       }
     ahl21_step<NB>(S, dt0, true, true, D.any_kick != 0, lane);
     const double gd = gdot<NB>(S, ti, jb);
@@ -1005,10 +1025,10 @@ __device__ void findtransit(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int
     const int64_t ntt = D.ntt;
     for (int col = lane; col < N7; col += WAVE) {
       const int rj = 6 * jb, ri = 6 * ti;
-      const double d0 = -((((S.JJ[rj * N7 + col] - S.JJ[ri * N7 + col]) * dvx +
-                            (S.JJ[(rj + 1) * N7 + col] - S.JJ[(ri + 1) * N7 + col]) * dvy) +
-                           (S.JJ[(rj + 3) * N7 + col] - S.JJ[(ri + 3) * N7 + col]) * dx) +
-                          (S.JJ[(rj + 4) * N7 + col] - S.JJ[(ri + 4) * N7 + col]) * dy) /
+      const double d0 = -((((J_[rj * N7 + col] - J_[ri * N7 + col]) * dvx +
+                            (J_[(rj + 1) * N7 + col] - J_[(ri + 1) * N7 + col]) * dvy) +
+                           (J_[(rj + 3) * N7 + col] - J_[(ri + 3) * N7 + col]) * dx) +
+                          (J_[(rj + 4) * N7 + col] - J_[(ri + 4) * N7 + col]) * dy) /
                         gd;
       const int64_t base = (int64_t)jb + n * (c + ntt * (int64_t)col);
       dq[0 + nb * base] = d0;
@@ -1016,11 +1036,11 @@ __device__ void findtransit(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int
         const double vsky = __builtin_sqrt(dvx * dvx + dvy * dvy);
         const double dvdt = (dvx * (S.dqdt[7 * jb + 3] - S.dqdt[7 * ti + 3]) +
                              dvy * (S.dqdt[7 * jb + 4] - S.dqdt[7 * ti + 4])) / vsky;
-        dq[1 + nb * base] = ((S.JJ[(rj + 3) * N7 + col] - S.JJ[(ri + 3) * N7 + col]) * dvx +
-                             (S.JJ[(rj + 4) * N7 + col] - S.JJ[(ri + 4) * N7 + col]) * dvy) / vsky +
+        dq[1 + nb * base] = ((J_[(rj + 3) * N7 + col] - J_[(ri + 3) * N7 + col]) * dvx +
+                             (J_[(rj + 4) * N7 + col] - J_[(ri + 4) * N7 + col]) * dvy) / vsky +
                             dvdt * d0;
-        dq[2 + nb * base] = 2.0 * ((S.JJ[rj * N7 + col] - S.JJ[ri * N7 + col]) * dx +
-                                   (S.JJ[(rj + 1) * N7 + col] - S.JJ[(ri + 1) * N7 + col]) * dy);
+        dq[2 + nb * base] = 2.0 * ((J_[rj * N7 + col] - J_[ri * N7 + col]) * dx +
+                                   (J_[(rj + 1) * N7 + col] - J_[(ri + 1) * N7 + col]) * dy);
       }
     }
     if (lane == 0) {
@@ -1032,8 +1052,8 @@ __device__ void findtransit(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int
     }
     for (int col = lane; col < N7; col += WAVE)
       for (int r = 0; r < N6; ++r) {
-        S.JJ[r * N7 + col] = ws[r * N7 + col];
-        S.JJ[Smem<NB>::JOFF + r * N7 + col] = ws[N6 * N7 + r * N7 + col];
+        J_[r * N7 + col] = ws[r * N7 + col];
+        J_[Smem<NB>::JOFF + r * N7 + col] = ws[N6 * N7 + r * N7 + col];
       }
   } else if (lane == 0) {
     tt[0 + nb * (jb + n * c)] = S.t + dt0;
@@ -1088,9 +1108,10 @@ __device__ void detect_transits(Smem<NB>& S, const SysDesc& D, const SysBufs& B,
 // calc_dtdelements! (timing.jl:112-138) for one system; jac_init is staged
 // transposed in the Jacobian's LDS space (the Jacobian is already stored).
 template <int NB> __device__ void dtdelements(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int lane) {
+  double* const J_ = jj<NB>(S);
   constexpr int N7 = 7 * NB;
   const int n = NB;
-  double* jt = S.JJ;  // N7*N7 <= 2*N6*N7 doubles
+  double* jt = J_;  // N7*N7 <= 2*N6*N7 doubles
   const double* jinit = B.jac + D.off_jac + 2 * (int64_t)N7 * N7;
   for (int col = lane; col < N7; col += WAVE)
     for (int cp = 0; cp < N7; ++cp) jt[col + N7 * cp] = jinit[cp + (int64_t)N7 * col];
@@ -1118,6 +1139,11 @@ template <int NB> __device__ void dtdelements(Smem<NB>& S, const SysDesc& D, con
 // Load / store of a system between its HBM record and LDS.
 // ---------------------------------------------------------------------------
 template <int NB> __device__ void load_system(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int lane) {
+  double* J_ = S.JJ;
+  if constexpr (Smem<NB>::JG) {
+    J_ = B.ws + D.off_ws + 2 * (int64_t)(6 * NB) * (7 * NB);  // after the transit-step copy
+    if (lane == 0) S.jg = J_;
+  }
   constexpr int N3 = 3 * NB, N6 = 6 * NB, N7 = 7 * NB;
   const double* st = B.state + D.off_state;
   for (int q = lane; q < N3; q += WAVE) {
@@ -1188,8 +1214,8 @@ template <int NB> __device__ void load_system(Smem<NB>& S, const SysDesc& D, con
     for (int c = lane; c < N7; c += WAVE)
       for (int b = 0; b < NB; ++b)
         for (int k = 0; k < 6; ++k) {
-          S.JJ[(6 * b + k) * N7 + c] = J[(7 * b + k) + (int64_t)N7 * c];
-          S.JJ[Smem<NB>::JOFF + (6 * b + k) * N7 + c] = JE[(7 * b + k) + (int64_t)N7 * c];
+          J_[(6 * b + k) * N7 + c] = J[(7 * b + k) + (int64_t)N7 * c];
+          J_[Smem<NB>::JOFF + (6 * b + k) * N7 + c] = JE[(7 * b + k) + (int64_t)N7 * c];
         }
   }
   (void)N6;
@@ -1197,6 +1223,7 @@ template <int NB> __device__ void load_system(Smem<NB>& S, const SysDesc& D, con
 }
 
 template <int NB> __device__ void store_system(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int lane) {
+  double* const J_ = jj<NB>(S);
   constexpr int N3 = 3 * NB, N7 = 7 * NB;
   wsync();
   double* st = B.state + D.off_state;
@@ -1218,8 +1245,8 @@ template <int NB> __device__ void store_system(Smem<NB>& S, const SysDesc& D, co
     for (int c = lane; c < N7; c += WAVE)
       for (int b = 0; b < NB; ++b)
         for (int k = 0; k < 6; ++k) {
-          J[(7 * b + k) + (int64_t)N7 * c] = S.JJ[(6 * b + k) * N7 + c];
-          JE[(7 * b + k) + (int64_t)N7 * c] = S.JJ[Smem<NB>::JOFF + (6 * b + k) * N7 + c];
+          J[(7 * b + k) + (int64_t)N7 * c] = J_[(6 * b + k) * N7 + c];
+          JE[(7 * b + k) + (int64_t)N7 * c] = J_[Smem<NB>::JOFF + (6 * b + k) * N7 + c];
         }
   }
   wsync();
@@ -1276,6 +1303,12 @@ __device__ __forceinline__ void dispatch_n(unsigned char* raw, const SysDesc& D,
   }
 }
 
+// LDS of a launch that may run any body count up to HI.
+template <int N> constexpr size_t lds_bytes_upto() {
+  if constexpr (N <= 2) return sizeof(Smem<2>);
+  else return sizeof(Smem<N>) > lds_bytes_upto<N - 1>() ? sizeof(Smem<N>) : lds_bytes_upto<N - 1>();
+}
+
 // Persistent kernel of launch class HI: every workgroup (one wave) takes
 // systems from the queues of its group list in order -- its own class first,
 // then classes of fewer bodies, each ordered longest first -- and runs each
@@ -1286,7 +1319,7 @@ __device__ __forceinline__ void dispatch_n(unsigned char* raw, const SysDesc& D,
 template <int HI>
 __global__ __launch_bounds__(WAVE) void nbg_run(GroupList gl, SysBufs B, int64_t chunk,
                                                 unsigned long long* __restrict__ units) {
-  __shared__ __attribute__((aligned(16))) unsigned char raw[sizeof(Smem<HI>)];
+  __shared__ __attribute__((aligned(16))) unsigned char raw[lds_bytes_upto<HI>()];
   const int lane = threadIdx.x;
   int g = 0;
   while (g < gl.ng) {
@@ -1318,11 +1351,13 @@ __global__ __launch_bounds__(WAVE) void nbg_run(GroupList gl, SysBufs B, int64_t
 }  // namespace nbg
 
 // ---------------------------------------------------------------------------
-// Host launcher.  Body counts are served by launch classes of similar LDS
-// footprint: n = 2..6 share one kernel (sized for n = 6: four workgroups per
-// CU, which their register use allows anyway), every larger n has its own.
+// Host launcher.  Body counts are served by launch classes: n = 2..9 share
+// one kernel (its LDS, the largest of their layouts -- n = 6's, with the
+// Jacobian in LDS up to n = 6 and in HBM from n = 7 on -- lets four
+// workgroups share a CU, which is also what their register use allows);
+// every larger n has its own.
 // ---------------------------------------------------------------------------
-extern "C" int nbg_class_of(int n) { return n <= 6 ? 6 : n; }
+extern "C" int nbg_class_of(int n) { return n <= 9 ? 9 : n; }
 
 extern "C" hipError_t nbg_launch_group(int cls, nbg::GroupList gl, nbg::SysBufs bufs, int64_t chunk, int grid,
                                        unsigned long long* units, hipStream_t stream) {
@@ -1331,9 +1366,6 @@ extern "C" hipError_t nbg_launch_group(int cls, nbg::GroupList gl, nbg::SysBufs 
   case HI:                                                                                                    \
     hipLaunchKernelGGL(nbg::nbg_run<HI>, dim3(grid), dim3(nbg::WAVE), 0, stream, gl, bufs, chunk, units);    \
     return hipGetLastError();
-    NBG_CASE(6)
-    NBG_CASE(7)
-    NBG_CASE(8)
     NBG_CASE(9)
     NBG_CASE(10)
     NBG_CASE(11)
@@ -1358,11 +1390,12 @@ extern "C" hipError_t nbg_launch_copy(const void* src, void* dst, size_t bytes, 
   return hipGetLastError();
 }
 
-extern "C" size_t nbg_lds_bytes(int nb) {
-  switch (nb) {
-#define NBG_SZ(N) case N: return sizeof(nbg::Smem<N>);
-    NBG_SZ(2) NBG_SZ(3) NBG_SZ(4) NBG_SZ(5) NBG_SZ(6) NBG_SZ(7) NBG_SZ(8) NBG_SZ(9) NBG_SZ(10) NBG_SZ(11) NBG_SZ(12)
-#undef NBG_SZ
+extern "C" size_t nbg_lds_bytes(int cls) {
+  switch (cls) {
+    case 9: return nbg::lds_bytes_upto<9>();
+    case 10: return nbg::lds_bytes_upto<10>();
+    case 11: return nbg::lds_bytes_upto<11>();
+    case 12: return nbg::lds_bytes_upto<12>();
     default: return 0;
   }
 }

@@ -8,7 +8,8 @@
 //                   t  (scalars at the end, STATE_T0_SLOT..)
 //  jac    (double): jac_step[7n*7n] jac_error[7n*7n] jac_init[7n*7n]
 //  ws     (double): two compressed 6n x 7n blocks (Jacobian copy during the
-//                   final gradient step of a transit)
+//                   final gradient step of a transit); for n >= JGLOBAL_MIN two
+//                   more: the Jacobian and its error during the run
 //  tt     (double): ntbv x n x ntt   (TransitTiming.tt / TransitParameters.ttbv)
 //  dq     (double): ntbv x n x ntt x 7 x n (dtdq0); del (double) same (dtdelements,
 //                   at off_del)
@@ -85,7 +86,10 @@ NBG_TD int64_t off_dqdt(int n) { return 13 * (int64_t)n; }
 NBG_TD int64_t off_gsave(int n) { return 20 * (int64_t)n; }
 NBG_TD int64_t off_t(int n) { return 21 * (int64_t)n; }
 NBG_TD int64_t jac_len(int n) { return 3 * 49 * (int64_t)n * n; }
-NBG_TD int64_t ws_len(int n) { return 2 * 42 * (int64_t)n * n; }
+// Systems of at least this many bodies keep their Jacobian (and its Kahan
+// error) in an HBM work area at the end of their ws record instead of LDS.
+constexpr int JGLOBAL_MIN = 7;
+NBG_TD int64_t ws_len(int n) { return (n >= JGLOBAL_MIN ? 4 : 2) * 42 * (int64_t)n * n; }
 NBG_TD int64_t int_len(int n) { return (int64_t)n + 3; }
 
 }  // namespace nbg

@@ -52,12 +52,14 @@ def test_no_device_raises():
 
 def test_server_footprint_matches_the_record_layout():
     """Device bytes of one served request: state, Jacobians (J, Jerr,
-    jac_init), the transit-step Jacobian copy, tt, dtdq0, dtdelements, counters
-    and the pair mask, rounded up to 256 bytes (no GPU needed)."""
+    jac_init), the transit-step Jacobian copy (and, from n = 7 on, the working
+    J/Jerr in HBM), tt, dtdq0, dtdelements, counters and the pair mask, rounded
+    up to 256 bytes (no GPU needed)."""
     if not os.path.exists(_lib.LIB_PATH):
         pytest.skip("libnbg_hip.so not built")
     L = _lib.lib()
     for n, ntbv, ntt in [(2, 1, 5), (3, 1, 103), (9, 1, 2003), (10, 3, 40)]:
-        b = 8 * ((21 * n + 4) + 3 * 49 * n * n + 84 * n * n + ntbv * n * ntt + 2 * ntbv * n * ntt * 7 * n + (n + 3))
+        ws = (168 if n >= 7 else 84) * n * n
+        b = 8 * ((21 * n + 4) + 3 * 49 * n * n + ws + ntbv * n * ntt + 2 * ntbv * n * ntt * 7 * n + (n + 3))
         b += (n * n + 7) // 8 * 8
         assert L.nbg_server_footprint(n, ntbv, ntt) == (b + 255) // 256 * 256
