@@ -38,6 +38,7 @@ struct PairSlot {
   double mi, mj, mijinv, m_i, m_j;
   int valid, i, j, pad;
   double kep[48], jm[6];
+  double mc[24];  // jac_ij's mass columns 6 (rows 0-5, 7-12) and 13 (same rows)
 };
 
 template <int NB> struct Smem {
@@ -52,12 +53,13 @@ template <int NB> struct Smem {
   double JJ[JG ? 1 : 2 * N6 * N7];
   double* jg;
   static constexpr int NP = NB * (NB - 1) / 2;  // pairs per sweep
-  static constexpr int CH_FIT = (int)((2 * N3 * N4 * sizeof(double)) / sizeof(PairSlot));
+  static constexpr int PQ = 17;  // per-pair factors of phisalpha!, see phi_build
+  // pair slots share the space of the force-gradient arrays (union below)
+  static constexpr int CH_FIT = (int)(((2 * N3 * N4 + (NP > 0 ? NP : 1) * PQ) * sizeof(double)) / sizeof(PairSlot));
   static constexpr int CH_MIN = CH_FIT < 4 ? 4 : CH_FIT;
   static constexpr int CH = NP < CH_MIN ? NP : (CH_MIN > 64 ? 64 : CH_MIN);  // pairs per sweep chunk
   // Lifetimes are disjoint: jac_phi/dadq live in the force-gradient stage
   // (dadq also holds jac_kick around the kicks), the pair slots in the sweeps.
-  static constexpr int PQ = 17;  // per-pair factors of phisalpha!, see phi_build
   union {
     struct {
       double phi[N3 * N4];   // jac_phi block: row 3b+k (velocity k of body b), col 3d+p (x_p of d) or N3+d (m_d)
@@ -107,6 +109,22 @@ __device__ inline void kahan(double& s, double& e, double add) {
   s = t;
 }
 
+// jac_ij (ahl21.jl:734-758) is a function of the pair's slot: rows 0-5 and
+// 7-12 (body i, body j), column k.  jac_ij starts at zero and accumulates
+// +-m*kep (ahl21.jl:735-742); the mass columns 6 and 13 are set from jac_mass,
+// delxv and kep's k-derivative column; rows 6 and 13 stay zero.
+__device__ inline double jij_coef(const PairSlot& sl, bool rowi, int rk, int k) {
+  const double mi = sl.mi, mj = sl.mj, mijinv = sl.mijinv;
+  if (k == 6)
+    return rowi ? sl.jm[rk] * sl.m_j : (((-mj) * sl.delxv[rk]) * mijinv) - ((NBG_GNEWT * mi) * sl.kep[rk + 36]);
+  if (k == 13)
+    return rowi ? ((mi * sl.delxv[rk]) * mijinv) + ((NBG_GNEWT * mj) * sl.kep[rk + 36]) : (-sl.jm[rk]) * sl.m_i;
+  const int ck = k < 7 ? k : k - 7;
+  const double kv = sl.kep[rk + 6 * ck];
+  if (rowi == (k < 7)) return 0.0 + (rowi ? mj : mi) * kv;
+  return 0.0 - (rowi ? mj : mi) * kv;
+}
+
 // ---------------------------------------------------------------------------
 // The pair Kepler drift (jac_delxv_gamma!, ahl21.jl:766-890) and its analytic
 // Jacobian (compute_jacobian_gamma!, ahl21.jl:896-1139) as single out-of-line
@@ -116,6 +134,7 @@ __device__ inline void kahan(double& s, double& e, double add) {
 // pair's slot in LDS; the arithmetic runs in registers.
 // ---------------------------------------------------------------------------
 __device__ __attribute__((noinline)) void pair_drift(PairSlot* sl, double gm, double hh, bool drift_first) {
+  drift_first = __builtin_amdgcn_readfirstlane((int)drift_first) != 0;  // the same for the whole sweep
   double x0[3], v0[3], delxv[6];
   for (int k = 0; k < 3; ++k) {
     x0[k] = sl->x0[k];
@@ -127,6 +146,7 @@ __device__ __attribute__((noinline)) void pair_drift(PairSlot* sl, double gm, do
 }
 
 __device__ __attribute__((noinline)) void pair_jacobian(PairSlot* sl, bool drift_first) {
+  drift_first = __builtin_amdgcn_readfirstlane((int)drift_first) != 0;  // the same for the whole sweep
   double x0[3], v0[3], kep[48], jm[6];
   for (int k = 0; k < 3; ++k) {
     x0[k] = sl->x0[k];
@@ -138,6 +158,10 @@ __device__ __attribute__((noinline)) void pair_jacobian(PairSlot* sl, bool drift
   compute_jacobian_gamma(P, x0, v0, kep, jm, drift_first);
   for (int q = 0; q < 48; ++q) sl->kep[q] = kep[q];
   for (int q = 0; q < 6; ++q) sl->jm[q] = jm[q];
+  for (int q = 0; q < 12; ++q) {
+    sl->mc[q] = jij_coef(*sl, q < 6, q < 6 ? q : q - 6, 6);
+    sl->mc[12 + q] = jij_coef(*sl, q < 6, q < 6 ? q : q - 6, 13);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -390,7 +414,14 @@ template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick,
     }
   }
   wsync();
-  // loop 1: accelerations and dadq (ahl21.jl:567-602)
+  // loop 1: accelerations and dadq (ahl21.jl:567-602).  For the lane's
+  // column (body myd, coordinate myp; myp = 3 for the mass) the reference's
+  // conditional updates of pair (i, j) reduce, per role of myd in the pair,
+  // to at most two additions in a fixed order: t1 then t2, each -0.0 (which
+  // leaves any sum unchanged) when it does not apply.  "self" is the body b
+  // of the row block, "other" its partner c.
+  const bool mpos = myp >= 0 && myp < 3, mmass = myp == 3;
+  const int ry = mpos ? myp : 0;
 #pragma unroll 1
   for (int b = 0; b < NB; ++b) {
     double ab[3] = {0.0, 0.0, 0.0};
@@ -402,31 +433,23 @@ template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick,
       if (S.pair[i + NB * j]) continue;
       const double* o = pq + pidx(i, j) * PQ;
       const double fac2 = o[0];
-      const int ry = myp < 3 ? myp : 0;
       const double rmy = S.x[3 * i + ry] - S.x[3 * j + ry];
-      const double mi = S.m[i], mj = S.m[j];
       const bool bi = (b == i);
+      const double m = bi ? S.m[j] : S.m[i];  // the partner's mass
+      const bool self_pos = mpos && myd == b, other_pos = mpos && myd == c;
+      const bool other_mass = mmass && myd == c;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const double fac = o[1 + k];
         const double f3 = o[4 + k];
-        double d = db[k];
-        if (bi) {
-          ab[k] -= mj * fac;
-          if (myp == 3 && myd == j) d -= fac;
-          if (myp < 3 && myd == i) d += (f3 * mj) * rmy;
-          if (myp < 3 && myd == j) d -= (f3 * mj) * rmy;
-          if (myp == k && myd == i) d -= fac2 * mj;
-          if (myp == k && myd == j) d += fac2 * mj;
-        } else {
-          ab[k] += mi * fac;
-          if (myp == 3 && myd == i) d += fac;
-          if (myp < 3 && myd == i) d -= (f3 * mi) * rmy;
-          if (myp < 3 && myd == j) d += (f3 * mi) * rmy;
-          if (myp == k && myd == i) d += fac2 * mi;
-          if (myp == k && myd == j) d -= fac2 * mi;
-        }
-        db[k] = d;
+        const double mf = m * fac;
+        ab[k] += bi ? -mf : mf;
+        const double x3 = (f3 * m) * rmy;
+        const double f2m = fac2 * m;
+        const bool pk = myp == k;
+        const double t1 = other_mass ? (bi ? -fac : fac) : (self_pos ? x3 : (other_pos ? -x3 : -0.0));
+        const double t2 = (pk && self_pos) ? -f2m : ((pk && other_pos) ? f2m : -0.0);
+        db[k] = (db[k] + t1) + t2;
       }
     }
     if (own)
@@ -466,7 +489,9 @@ template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick,
     o[16] = (-fac1) * r2;  // fa
   }
   wsync();
-  // loop 2: velocity corrections, dqdt_phi and jac_phi (ahl21.jl:607-698)
+  // loop 2: velocity corrections, dqdt_phi and jac_phi (ahl21.jl:607-698),
+  // with the same role reduction as loop 1: per k, at most three
+  // role-dependent additions (t1, t2, t3), then the dadq terms.
   const double q3h = 3.0 / h;
 #pragma unroll 1
   for (int b = 0; b < NB; ++b) {
@@ -492,53 +517,35 @@ template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick,
       double dot = 0.0;
       if (own)
         for (int k = 0; k < 3; ++k) dot += rij[k] * (Di[k] - Dj[k]);
-      const int ry = myp < 3 ? myp : 0;
       const double rmy = ry == 0 ? rij[0] : (ry == 1 ? rij[1] : rij[2]);
       const double amy = S.a[3 * i + ry] - S.a[3 * j + ry];
-      const double mi = S.m[i], mj = S.m[j];
-      const bool ij = myp < 3 && (myd == i || myd == j);
       const bool bi = (b == i);
+      const double m = bi ? S.m[j] : S.m[i];  // the partner's mass
+      const bool self_pos = mpos && myd == b, other_pos = mpos && myd == c;
+      const bool self_mass = mmass && myd == b, other_mass = mmass && myd == c;
       const double fd = o[15], fa = o[16];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
         const double fac = o[k], f5 = o[3 + k], fm = o[6 + k], fr = o[9 + k], ff = o[12 + k];
-        const double fac3 = ij ? fr * rmy + o[12 + k] * amy : 0.0;
-        const double diff = Di[k] - Dj[k];
-        double pv = pb[k];
-        if (bi) {
-          kahan(vb[k], veb[k], mj * fac);
-          if (needQ) qb[k] += (q3h * mj) * fac;
-          if (myp == 3 && myd == j) pv += fac;
-          if (myp < 3 && myd == i) pv -= (f5 * mj) * rmy;
-          if (myp < 3 && myd == j) pv += (f5 * mj) * rmy;
-          if (myp == 3 && myd == i) pv += fm * mj;
-          if (myp == 3 && myd == j) pv += fm * mj;
-          if (myp == k && myd == i) pv += fd * mj;
-          if (myp == k && myd == j) pv -= fd * mj;
-          if (ij) {
-            if (myd == i) pv += mj * fac3;
-            else pv -= mj * fac3;
-          }
-          pv += (fa * mj) * diff;
-          pv += (ff * mj) * dot;
-        } else {
-          kahan(vb[k], veb[k], (-mi) * fac);
-          if (needQ) qb[k] -= (q3h * mi) * fac;
-          if (myp == 3 && myd == i) pv -= fac;
-          if (myp < 3 && myd == i) pv += (f5 * mi) * rmy;
-          if (myp < 3 && myd == j) pv -= (f5 * mi) * rmy;
-          if (myp == 3 && myd == i) pv -= fm * mi;
-          if (myp == 3 && myd == j) pv -= fm * mi;
-          if (myp == k && myd == i) pv -= fd * mi;
-          if (myp == k && myd == j) pv += fd * mi;
-          if (ij) {
-            if (myd == i) pv -= mi * fac3;
-            else pv += mi * fac3;
-          }
-          pv -= (fa * mi) * diff;
-          pv -= (ff * mi) * dot;
+        const double mf = m * fac;
+        kahan(vb[k], veb[k], bi ? mf : -mf);
+        if (needQ) {
+          const double qm = (q3h * m) * fac;
+          qb[k] += bi ? qm : -qm;
         }
-        pb[k] = pv;
+        const double diff = Di[k] - Dj[k];
+        const double x5 = (f5 * m) * rmy;
+        const double y = fm * m;
+        const double sy = bi ? y : -y, sfac = bi ? fac : -fac;
+        const double fdm = fd * m;
+        const double fac3 = fr * rmy + o[12 + k] * amy;
+        const double mf3 = m * fac3;
+        const bool pk = myp == k;
+        const double t1 = self_pos ? -x5 : (other_pos ? x5 : (self_mass ? sy : (other_mass ? sfac : -0.0)));
+        const double t2 = (pk && self_pos) ? fdm : ((pk && other_pos) ? -fdm : (other_mass ? sy : -0.0));
+        const double t3 = self_pos ? mf3 : (other_pos ? -mf3 : -0.0);
+        const double td = (fa * m) * diff, tf = (ff * m) * dot;
+        pb[k] = (((((pb[k] + t1) + t2) + t3) + (bi ? td : -td)) + (bi ? tf : -tf));
       }
     }
     if (own)
@@ -617,51 +624,57 @@ template <int NB> __device__ inline void drift_cols(Smem<NB>& S, int c, double h
     }
 }
 
-// jac_ij (ahl21.jl:734-758) is a function of the pair's slot: rows 0-5 and
-// 7-12 (body i, body j), column k.  jac_ij starts at zero and accumulates
-// +-m*kep (ahl21.jl:735-742); the mass columns 6 and 13 are set from jac_mass,
-// delxv and kep's k-derivative column; rows 6 and 13 stay zero.
-__device__ inline double jij_coef(const PairSlot& sl, bool rowi, int rk, int k) {
-  const double mi = sl.mi, mj = sl.mj, mijinv = sl.mijinv;
-  if (k == 6)
-    return rowi ? sl.jm[rk] * sl.m_j : (((-mj) * sl.delxv[rk]) * mijinv) - ((NBG_GNEWT * mi) * sl.kep[rk + 36]);
-  if (k == 13)
-    return rowi ? ((mi * sl.delxv[rk]) * mijinv) + ((NBG_GNEWT * mj) * sl.kep[rk + 36]) : (-sl.jm[rk]) * sl.m_i;
-  const int ck = k < 7 ? k : k - 7;
-  const double kv = sl.kep[rk + 6 * ck];
-  if (rowi == (k < 7)) return 0.0 + (rowi ? mj : mi) * kv;
-  return 0.0 - (rowi ? mj : mi) * kv;
-}
-
 // Pair map on rows (i, j): copy_submatrix!, mul!, comp_sum_matrix!,
-// ypoc_submatrix! (ahl21.jl:31-42, utils.jl:51-100) for column c, with
-// jac_ij evaluated from the pair's slot (every lane forms the same
-// coefficients; each row's products are summed over k in ascending order).
+// ypoc_submatrix! (ahl21.jl:31-42, utils.jl:51-100) for column c.  Row q of
+// jac_ij * a is summed over k = 0..13 in ascending order with one fused
+// multiply-add per term, as jij_coef's coefficients would give it: the
+// Kepler-block coefficient +-(0 + m*kep) enters as m*kep times +-a[k] (the
+// running sum starts at +0 and can never become -0, so the sign of a zero
+// coefficient cannot change it), and the mass columns 6 and 13 take their
+// coefficients from the slot, where pair_jacobian stored them.
 template <int NB> __device__ inline void pair_cols(Smem<NB>& S, const PairSlot& sl, int i, int j, int c) {
   double* const J_ = jj<NB>(S);
   constexpr int N7 = 7 * NB;
   const int ri = 6 * i, rj = 6 * j;
-  double a[14], e[12];
+  double a[12], e[12];
 #pragma unroll
   for (int k = 0; k < 6; ++k) {
     a[k] = J_[(ri + k) * N7 + c];
-    a[7 + k] = J_[(rj + k) * N7 + c];
+    a[6 + k] = J_[(rj + k) * N7 + c];
     e[k] = J_[Smem<NB>::JOFF + (ri + k) * N7 + c];
     e[6 + k] = J_[Smem<NB>::JOFF + (rj + k) * N7 + c];
   }
-  a[6] = (c == 7 * i + 6) ? 1.0 : 0.0;
-  a[13] = (c == 7 * j + 6) ? 1.0 : 0.0;
+  const double a6 = (c == 7 * i + 6) ? 1.0 : 0.0;
+  const double a13 = (c == 7 * j + 6) ? 1.0 : 0.0;
+  const double mi = sl.mi, mj = sl.mj;
   double o[12];
 #pragma unroll
   for (int q = 0; q < 12; ++q) o[q] = 0.0;
+  // k = 0..5: rows of i get +mj*kep * a_i[k], rows of j -mi*kep * a_i[k]
 #pragma unroll
-  for (int k = 0; k < 14; ++k)
+  for (int k = 0; k < 6; ++k)
 #pragma unroll
-    for (int q = 0; q < 12; ++q) o[q] = fmad(jij_coef(sl, q < 6, q < 6 ? q : q - 6, k), a[k], o[q]);
+    for (int rk = 0; rk < 6; ++rk) {
+      const double kv = sl.kep[rk + 6 * k];
+      o[rk] = fmad(mj * kv, a[k], o[rk]);
+      o[6 + rk] = fmad(mi * kv, -a[k], o[6 + rk]);
+    }
+#pragma unroll
+  for (int q = 0; q < 12; ++q) o[q] = fmad(sl.mc[q], a6, o[q]);
+  // k = 7..12: rows of i get -mj*kep * a_j[k], rows of j +mi*kep * a_j[k]
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+#pragma unroll
+    for (int rk = 0; rk < 6; ++rk) {
+      const double kv = sl.kep[rk + 6 * k];
+      o[rk] = fmad(mj * kv, -a[6 + k], o[rk]);
+      o[6 + rk] = fmad(mi * kv, a[6 + k], o[6 + rk]);
+    }
+#pragma unroll
+  for (int q = 0; q < 12; ++q) o[q] = fmad(sl.mc[12 + q], a13, o[q]);
 #pragma unroll
   for (int q = 0; q < 12; ++q) {
-    const int src = q < 6 ? q : q + 1;
-    double sv = a[src], se = e[q];
+    double sv = a[q], se = e[q];
     se += o[q];
     const double t = sv + se;
     se += sv - t;

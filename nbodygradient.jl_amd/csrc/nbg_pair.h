@@ -39,10 +39,22 @@ NBG_HD double cubic1(double a, double b, double c) {
   double R = a3 * a3 * a3 + (double)(0.5) * ((-a3) * b + c);
   double R2 = R * R;
   double Q3 = Q * Q * Q;
-  if (R2 < Q3) return -c / b;
-  double A = (-nbg::sign(R)) * nbg::cbrt(__builtin_fabs(R) + __builtin_sqrt(R2 - Q3));
-  double B = (A == (double)(0)) ? (double)(0) : Q / A;
-  return (A + B) - a3;
+  // Each lane returns one of the two formulas; a formula is evaluated only
+  // when some lane of the wave needs it (on the host: when this lane does).
+  const bool three = R2 < Q3;
+  double res = 0.0;
+  if (NBG_ANY(three)) {
+    const double v = -c / b;
+    if (three) res = v;
+  }
+  if (NBG_ANY(!three)) {
+    const double w = three ? 1.0 : __builtin_fabs(R) + __builtin_sqrt(R2 - Q3);
+    double A = (-nbg::sign(R)) * nbg::cbrt(w);
+    double B = (A == (double)(0)) ? (double)(0) : Q / A;
+    const double v = (A + B) - a3;
+    if (!three) res = v;
+  }
+  return res;
 }
 
 // G3 / G3_series: utils.jl:124-165
@@ -198,6 +210,134 @@ NBG_HD double H6(double gamma, double beta) {
          ((double)(2) * (beta * beta));
 }
 
+// The series branches (gamma < 0.5) of G3, H1, H2, H3, H5 and H6 evaluated
+// together: each series keeps its own recurrence, iteration count and
+// stopping test exactly as in its single-function form above, and stops
+// updating once its own test fires; running them side by side only lets
+// their independent divisions overlap.  MASK selects the series wanted.
+enum : int { SER_G3 = 1, SER_H1 = 2, SER_H2 = 4, SER_H3 = 8, SER_H5 = 16, SER_H6 = 32 };
+struct SeriesOut {
+  double g3, h1, h2, h3, h5, h6;
+};
+template <int MASK>
+NBG_HD SeriesOut series_set(double gamma, double beta, double sqb) {
+  const double x2 = (-nbg::sign(beta)) * (gamma * gamma);
+  // G3: term, sum, previous two sums, done
+  double tg = 1.0, sg = 1.0, sg1 = 2.0 * sg, sg2 = 2.0 * sg;
+  double t1 = 1.0, s1 = 1.0, s11 = 2.0 * s1, s12 = 2.0 * s1;
+  double t2 = 1.0, s2 = 1.0, s21 = 2.0 * s2, s22 = 2.0 * s2;
+  double t3 = (double)(1) / (double)(30), s3 = (double)(1) / (double)(10), s31 = 2.0 * s3, s32 = 2.0 * s3;
+  double f3 = (double)(1) * (double)(4);
+  double t5 = (double)(1) / (double)(60), s5 = t5, s51 = 2.0 * s5, s52 = 2.0 * s5;
+  double t6 = (double)(1) / (double)(360), s6 = (double)(1) / (double)(40), s61 = 2.0 * s6, s62 = 2.0 * s6;
+  double f6 = (double)(1) * (double)(16);
+  bool dg = !(MASK & SER_G3), d1 = !(MASK & SER_H1), d2 = !(MASK & SER_H2);
+  bool d3 = !(MASK & SER_H3), d5 = !(MASK & SER_H5), d6 = !(MASK & SER_H6);
+  // n is each running series' iteration count.  A series that has stopped
+  // keeps its values (the candidate update is computed and discarded), so
+  // the loop can run until every series of every lane of the wave is done.
+  for (int n = 1;; ++n) {
+    if (MASK & SER_G3) {
+      const double t = tg * (x2 / double((2 * n + 3) * (2 * n + 2)));
+      const double v = sg + t;
+      const bool u = !dg;
+      sg2 = u ? sg1 : sg2; sg1 = u ? sg : sg1; tg = u ? t : tg; sg = u ? v : sg;
+      dg = dg || (n >= 100 || sg == sg2 || sg == sg1);
+    }
+    if (MASK & SER_H1) {
+      const double t = (t1 * (x2 * (double)(n + 1))) / double((2 * n + 4) * (2 * n + 3) * n);
+      const double v = s1 + t;
+      const bool u = !d1;
+      s12 = u ? s11 : s12; s11 = u ? s1 : s11; t1 = u ? t : t1; s1 = u ? v : s1;
+      d1 = d1 || (n >= 100 || s1 == s12 || s1 == s11);
+    }
+    if (MASK & SER_H2) {
+      const double t = (t2 * x2) / double((4 * n + 6) * n);
+      const double v = s2 + t;
+      const bool u = !d2;
+      s22 = u ? s21 : s22; s21 = u ? s2 : s21; t2 = u ? t : t2; s2 = u ? v : s2;
+      d2 = d2 || (n >= 100 || s2 == s22 || s2 == s21);
+    }
+    if (MASK & SER_H3) {
+      const double t = (t3 * x2) / double((2 * n + 4) * (2 * n + 5));
+      const double f = f3 * (double)(4);
+      const double v = s3 + t * (f - (double)(1));
+      const bool u = !d3;
+      s32 = u ? s31 : s32; s31 = u ? s3 : s31; t3 = u ? t : t3; f3 = u ? f : f3; s3 = u ? v : s3;
+      d3 = d3 || (n >= 100 || s3 == s32 || s3 == s31);
+    }
+    if (MASK & SER_H5) {
+      const double t = (t5 * (x2 * (double)(n + 1))) / double((2 * n + 5) * (2 * n + 4) * n);
+      const double v = s5 + t;
+      const bool u = !d5;
+      s52 = u ? s51 : s52; s51 = u ? s5 : s51; t5 = u ? t : t5; s5 = u ? v : s5;
+      d5 = d5 || (n >= 100 || s5 == s52 || s5 == s51);
+    }
+    if (MASK & SER_H6) {
+      const double t = (t6 * x2) / double((2 * n + 5) * (2 * n + 6));
+      const double f = f6 * (double)(4);
+      const double v = s6 + t * ((f - (double)(3 * n)) - (double)(7));
+      const bool u = !d6;
+      s62 = u ? s61 : s62; s61 = u ? s6 : s61; t6 = u ? t : t6; f6 = u ? f : f6; s6 = u ? v : s6;
+      d6 = d6 || (n >= 100 || s6 == s62 || s6 == s61);
+    }
+    if (NBG_ALL(dg && d1 && d2 && d3 && d5 && d6)) break;
+  }
+  SeriesOut o;
+  o.g3 = sg * (((-x2) * gamma) / (((double)(6) * beta) * sqb));
+  o.h1 = s1 * ((x2 * x2) / ((double)(12) * (beta * beta)));
+  o.h2 = s2 * (((-x2) * gamma) / (((double)(3) * beta) * sqb));
+  o.h3 = s3 * (((-(x2 * x2)) * gamma) / (beta * sqb));
+  o.h5 = s5 * (((-(x2 * x2)) * gamma) / (beta * sqb));
+  o.h6 = s6 * ((-(x2 * x2 * x2)) / (beta * beta));
+  return o;
+}
+// The functions of MASK at (gamma, beta): the joint series below 0.5, the
+// closed forms above (each function's own branch).
+template <int MASK>
+NBG_HD SeriesOut gfuncs(double gamma, double beta, double sqb) {
+  const bool ser = gamma < (double)(0.5);
+  SeriesOut o = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (NBG_ANY(ser)) {  // lanes of the closed forms run the series on a dummy argument
+    const SeriesOut v = series_set<MASK>(ser ? gamma : 0.25, beta, sqb);
+    if (ser) o = v;
+  }
+  if (NBG_ANY(!ser)) {
+    const double g = ser ? 1.0 : gamma;
+    SeriesOut v;
+    v.g3 = (MASK & SER_G3) ? G3(g, beta, sqb) : 0.0;
+    v.h1 = (MASK & SER_H1) ? H1(g, beta) : 0.0;
+    v.h2 = (MASK & SER_H2) ? H2(g, beta, sqb) : 0.0;
+    v.h3 = (MASK & SER_H3) ? H3(g, beta, sqb) : 0.0;
+    v.h5 = (MASK & SER_H5) ? H5(g, beta, sqb) : 0.0;
+    v.h6 = (MASK & SER_H6) ? H6(g, beta) : 0.0;
+    if (!ser) o = v;
+  }
+  return o;
+}
+
+// (sin, cos)(xx) on lanes with ell, (sinh, e^-xx + sinh)(xx) on the others:
+// the two branches of the Kepler solve's G-function evaluation.  Each is
+// evaluated only if some lane of the wave needs it, with a harmless argument
+// on the lanes that do not, so no lane's inputs push the others off the
+// libm fast paths.
+NBG_HD void sc_pair(bool ell, double xx, double* sx, double* cx) {
+  double s = 0.0, c = 0.0;
+  if (NBG_ANY(ell)) {
+    double s1, c1;
+    nbg::sincos(ell ? xx : 0.125, &s1, &c1);
+    if (ell) { s = s1; c = c1; }
+  }
+  if (NBG_ANY(!ell)) {
+    const double y = ell ? 0.125 : xx;
+    const double s2 = nbg::sinh(y);
+    const double c2 = nbg::exp(-y) + s2;
+    if (!ell) { s = s2; c = c2; }
+  }
+  *sx = s;
+  *cx = c;
+}
+
 struct KParams {
   double gamma, g0, g1, g2, g3, h1, h2, dfdt, fm1, gmh, dgdtm1, r0, r, r0inv, rinv, k, h, beta, betainv, eta, sqb, zeta;
   int iters;
@@ -219,18 +359,23 @@ NBG_HD KParams jac_delxv_gamma(const double* x0, const double* v0, double* delxv
   double zeta = k - r0 * beta0;
   double gamma_guess = (double)(0);
   double eta = drift_first ? dot3(rtmp, v0) : dot3(x0, v0);
-  if (zeta != (double)(0)) {
+  const bool zn = zeta != (double)(0);
+  if (NBG_ANY(zn)) {
     double zinv = (double)(6) / zeta;
-    gamma_guess = cubic1((((double)(0.5) * eta) * sqb) * zinv, ((r0 * signb) * beta0) * zinv,
-                         ((((-h) * signb) * beta0) * sqb) * zinv);
-  } else {
+    const double g = cubic1((((double)(0.5) * eta) * sqb) * zinv, ((r0 * signb) * beta0) * zinv,
+                            ((((-h) * signb) * beta0) * sqb) * zinv);
+    if (zn) gamma_guess = g;
+  }
+  if (NBG_ANY(!zn)) {
+    double g;
     if (eta != (double)(0)) {
       double reta = r0 / eta;
       double disc = reta * reta + ((double)(2) * h) / eta;
-      gamma_guess = (disc > (double)(0)) ? sqb * ((-reta) + __builtin_sqrt(disc)) : (h * r0inv) * sqb;
+      g = (disc > (double)(0)) ? sqb * ((-reta) + __builtin_sqrt(disc)) : (h * r0inv) * sqb;
     } else {
-      gamma_guess = (h * r0inv) * sqb;
+      g = (h * r0inv) * sqb;
     }
+    if (!zn) gamma_guess = g;
   }
   double gamma = gamma_guess;
   double gamma1 = (double)(2) * gamma;
@@ -240,35 +385,40 @@ NBG_HD KParams jac_delxv_gamma(const double* x0, const double* v0, double* delxv
   const double c3 = (((double)(2) * eta) * signb) * sqb;
   const double c4 = ((-sqb) * h) * beta0;
   const double c5 = (((double)(2) * eta) * signb) * sqb;
+  const bool ell = beta0 > (double)(0);
   double sx, cx, xx;
-  while (iter < 20) {
-    gamma2 = gamma1;
-    gamma1 = gamma;
+  // Newton on gamma (at most 20 iterations, until gamma repeats one of the
+  // previous two).  A lane that has stopped keeps its values while the
+  // others finish, so the loop runs until the whole wave is done.
+  bool run = true;
+  do {
     xx = (double)(0.5) * gamma;
-    if (beta0 > (double)(0)) {
-      nbg::sincos(xx, &sx, &cx);
-    } else {
-      sx = nbg::sinh(xx);
-      cx = nbg::exp(-xx) + sx;
+    sc_pair(ell, xx, &sx, &cx);
+    const double gn = gamma - (((k * gamma + (c2 * sx) * cx) + c3 * (sx * sx)) + c4) /
+                                  (((((double)(2) * signb) * zeta) * (sx * sx) + (c5 * sx) * cx) + r0 * beta0);
+    if (run) {
+      gamma2 = gamma1;
+      gamma1 = gamma;
+      gamma = gn;
+      iter += 1;
+      run = !(gamma == gamma2 || gamma == gamma1) && iter < 20;
     }
-    gamma -= (((k * gamma + (c2 * sx) * cx) + c3 * (sx * sx)) + c4) /
-             (((((double)(2) * signb) * zeta) * (sx * sx) + (c5 * sx) * cx) + r0 * beta0);
-    iter += 1;
-    if (gamma == gamma2 || gamma == gamma1) break;
-  }
+  } while (NBG_ANY(run));
   for (int j = 0; j < 6; ++j) delxv[j] = (double)(0);
   xx = (double)(0.5) * gamma;
-  if (beta0 > (double)(0)) {
-    nbg::sincos(xx, &sx, &cx);
-  } else {
-    sx = nbg::sinh(xx);
-    cx = nbg::exp(-xx) + sx;
-  }
+  sc_pair(ell, xx, &sx, &cx);
   double g1bs = (((double)(2) * sx) * cx) / sqb;
   double g2bs = (((double)(2) * signb) * (sx * sx)) * beta0inv;
   double g0bs = (double)(1) - beta0 * g2bs;
-  double g3bs = G3(gamma, beta0, sqb);
-  double h1 = (double)(0), h2 = (double)(0);
+  double g3bs, h1 = (double)(0), h2 = (double)(0);
+  if (drift_first) {
+    g3bs = gfuncs<SER_G3>(gamma, beta0, sqb).g3;
+  } else {
+    const SeriesOut so = gfuncs<SER_G3 | SER_H1 | SER_H2>(gamma, beta0, sqb);
+    g3bs = so.g3;
+    h1 = so.h1;
+    h2 = so.h2;
+  }
   double r = (r0 * g0bs + eta * g1bs) + k * g2bs;
   double rinv = (double)(1) / r;
   double dfdt = (((-k) * g1bs) * rinv) * r0inv;
@@ -277,8 +427,6 @@ NBG_HD KParams jac_delxv_gamma(const double* x0, const double* v0, double* delxv
     fm1 = ((-k) * r0inv) * g2bs;
     gmh = (k * r0inv) * (h * g2bs - r0 * g3bs);
   } else {
-    h1 = H1(gamma, beta0);
-    h2 = H2(gamma, beta0, sqb);
     fm1 = (k * rinv) * (g2bs - (k * r0inv) * h1);
     gmh = (k * rinv) * (r0 * h2 + eta * h1);
   }
@@ -326,20 +474,21 @@ void compute_jacobian_gamma(const KParams& P, const double* x0, const double* v0
     double c9 = (((double)(2) * g2) * h) - (((double)(3) * g3) * r0);
     double c10 = (k * (r0inv2 * r0inv2)) * ((((-g2) * r0) * h + ((k * c9) * betainv)) - ((c3 * c13) * rinv));
     double c24 = r0inv3 * ((r0 * (((double)(2) * k) * r0inv - beta)) * betainv - ((g1 * c3) * rinv) / g2);
-    double h6 = H6(gamma, beta);
+    const SeriesOut so = gfuncs<SER_H1 | SER_H2 | SER_H3 | SER_H5 | SER_H6>(gamma, beta, sqb);
+    double h6 = so.h6;
     double dfm1dxx = fm1 * c24;
     double dfm1dxv = (-fm1) * (g1 * rinv + h * c24);
     double dfm1dvx = dfm1dxv;
     double dfm1dvv = (fm1 * rinv) * (((-r0) * g2 + ((k * h6) * betainv) / g2) + h * ((double)(2) * g1 + ((h * r) * c24)));
     double dfm1dh = fm1 * ((g1 * rinv) * (((double)(1) / g2 + ((double)(2) * k) * r0inv) - beta) - eta * c24);
     double dfm1dk = fm1 * (((double)(1) / k + ((((g1 * c1) * rinv) * r0inv) / g2)) - (((double)(2) * betainv) * r0inv));
-    double h4 = (-H1(gamma, beta)) * beta;
-    double h5 = H5(gamma, beta, sqb);
+    double h4 = (-so.h1) * beta;
+    double h5 = so.h5;
     double dfm1dk2 = r0 * h4 + k * h6;
     double dgmhdxx = c10;
     double dgmhdxv = (((((-g2) * k) * c13) * rinv) * r0inv) - h * c10;
     double dgmhdvx = dgmhdxv;
-    double h3 = H3(gamma, beta, sqb);
+    double h3 = so.h3;
     double h8 = (-((double)(2) * h3)) + (double)(3) * h5;
     double dgmhdvv = ((((((((double)(2) * g2) * h) * k) * c13) * rinv) * r0inv) + hsq * c10) +
                 (((k * betainv) * rinv) * r0inv) *
@@ -400,7 +549,7 @@ void compute_jacobian_gamma(const KParams& P, const double* x0, const double* v0
     double dgdtmhdfdtm1dxx = c25;
     double dgdtmhdfdtm1dxv = c26 - h * c25;
     double dgdtmhdfdtm1dvx = c26 - h * c25;
-    h2 = H2(gamma, beta, sqb);
+    h2 = so.h2;
     double c33 = ((((((d * k) * rinv3) * r0inv) * k) * (h * g2 - r0 * g3)) +
              ((((k * ((((((((-eta) * k) * g1) * (g2 * g2)) - (((g1 * g2) * g3) * ksq)) -
                          ((((r0 * eta) * beta) * g1) * (g2 * g2))) -
@@ -454,9 +603,10 @@ void compute_jacobian_gamma(const KParams& P, const double* x0, const double* v0
     double c18 = eta * g1 + ((double)(2) * g2) * k;
     double c19 = ((double)(4) * eta) * h1 + ((double)(3) * h2) * r0;
     double c23 = h2 * k - r0 * g1;
-    double h6 = H6(gamma, beta);
-    double h3 = H3(gamma, beta, sqb);
-    double h5 = H5(gamma, beta, sqb);
+    const SeriesOut so = gfuncs<SER_H3 | SER_H5 | SER_H6>(gamma, beta, sqb);
+    double h6 = so.h6;
+    double h3 = so.h3;
+    double h5 = so.h5;
     double h8 = (-((double)(2) * h3)) + (double)(3) * h5;
     double dfm1dxx = ((((k * rinv3) * betainv) * pow4(r0inv)) *
                  (((((((k * h1) * (r * r)) * r0) * (beta - ((double)(2) * k) * r0inv)) +
