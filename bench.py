@@ -31,11 +31,12 @@ sys.path.insert(0, os.path.join(ROOT, "nbodygradient.jl_amd"))
 import numpy as np  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per system-step of nbg_run<9>, from the rocprofv3 FETCH_SIZE and
-# WRITE_SIZE passes scaled by the same counters' reading of a copy of known
-# size (profiles/nbg_run_c2_x0.05_counters.md: 22.22 GB for 799,221 n = 9
-# system-steps in one dispatch)
-HBM_BYTES_PER_UNIT = 22.22e9 / 799221
+# HBM bytes per system-step of nbg_run<9> (every n <= 9 in one launch), from
+# the rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the same counters'
+# reading of a copy of known size (profiles/nbg_run_c2_x0.05_counters.md:
+# 696.1 GB read + 901.6 GB written for the 4,310,895 system-steps of the
+# C2 x0.05 batch in one dispatch)
+HBM_BYTES_PER_UNIT = (696.1e9 + 901.6e9) / 4310895
 METRIC = "system-steps/s w/ Jacobian, batched vs plain GPU; p50/p99 request latency"
 
 
@@ -205,11 +206,11 @@ def main():
                                "Jacobian + transit times + dt/delements" %
                                (len(specs), "" if args.tmax_scale == 1.0 else " x%g" % args.tmax_scale),
                    "systems": len(specs), "system_steps": sys_steps, "transits": stats["transits"],
-                   "newton_iters": stats["newton_iters"], "parallelism": "1 GPU, persistent launch per n"},
+                   "newton_iters": stats["newton_iters"], "parallelism": "1 GPU, one persistent launch per body-count class (n <= 9 in one)"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": HBM_BYTES_PER_UNIT * launches[dom]["units"],
-                     "traffic_basis": "rocprofv3 FETCH_SIZE/WRITE_SIZE of nbg_run<9>, calibrated on nbg_copy_kernel: "
+                     "traffic_basis": "rocprofv3 FETCH_SIZE/WRITE_SIZE of nbg_run<9> on C2 x0.05, calibrated on nbg_copy_kernel: "
                                       "%.0f B per system-step x this launch's units" % HBM_BYTES_PER_UNIT,
                      "kernel": launches[dom]["kernel"], "launch_ms": launches[dom]["ms"],
                      "note": "FP64 compute bound (VALU); MI355X FP64 vector and matrix peaks are both 78.6 "
