@@ -125,6 +125,9 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (read MIB, write MIB) through an elementwise "
                          "kernel: the reference reading for the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--coop", default="auto", metavar="auto|off|K",
+                    help="two-wave workgroups for the longest systems: automatic selection (default), off, or the "
+                         "K longest systems")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (systems are independent: a "
@@ -142,6 +145,10 @@ def main():
             s.tmax *= args.tmax_scale
     states = Wk.initial_states(specs)
     ctx = nb.default_context()
+    if args.coop == "off":
+        ctx.set_coop(0, 0)
+    elif args.coop != "auto":
+        ctx.set_coop(2, int(args.coop))
     name, cus = ctx.info()
     log("device %s, %d CUs; %d systems prepared in %.1fs" % (name, cus, len(specs), time.perf_counter() - t_setup))
 
@@ -174,7 +181,8 @@ def main():
     launches = {}
     for cls, L in stats["launches"].items():
         flops = sum(W(n) * u for n, u in L["units"].items())
-        launches[str(cls)] = {"kernel": "nbg_run<%d>" % cls, "ms": L["ms"], "grid": L["grid"],
+        name = "nbg_run_coop<9>" if cls == 1 else "nbg_run<%d>" % cls  # class 1: two-wave workgroups
+        launches[str(cls)] = {"kernel": name, "ms": L["ms"], "grid": L["grid"],
                               "units": sum(L["units"].values()),
                               "units_by_n": {str(n): u for n, u in sorted(L["units"].items())}, "flops": flops,
                               "tflops": flops / (L["ms"] / 1e3) / 1e12 if L["ms"] > 0 else 0.0}

@@ -153,6 +153,15 @@ int64_t nbg_server_footprint(int32_t n, int32_t ntbv, int64_t ntt);
 int nbg_set_launch(nbg_ctx* ctx, int32_t max_groups_per_cu, int32_t streams);
 /* Upper bound on persistent workgroups per launch (0 = no bound). */
 int nbg_set_grid_cap(nbg_ctx* ctx, int32_t grid_cap);
+/* Cooperative workgroups of batch runs (no counterpart in the reference):
+ * a system of at most 9 bodies with a transit search can run on a workgroup
+ * of two waves, the second running its transit searches (findtransit!,
+ * timing.jl:30-110) from snapshots while the first integrates; results are
+ * bitwise those of the one-wave kernel.  mode 0: off; 1 (default): the
+ * systems whose estimated time exceeds the batch's work spread over every
+ * wave slot of the GPU (the ones that would set the batch's end); 2: the
+ * longest ones regardless.  At most max_systems per batch (default 128). */
+int nbg_set_coop(nbg_ctx* ctx, int32_t mode, int32_t max_systems);
 
 /* Diagnostic only (no counterpart in the reference): a plain device copy of
  * `bytes` (a multiple of 16) through a grid-stride kernel, run `reps` times;

@@ -26,6 +26,9 @@ extern "C" hipError_t nbg_launch_group(int cls, nbg::GroupList gl, nbg::SysBufs 
 extern "C" size_t nbg_lds_bytes(int nb);
 extern "C" int nbg_class_of(int n);
 extern "C" hipError_t nbg_launch_copy(const void* src, void* dst, size_t bytes, int cu_count, hipStream_t stream);
+extern "C" hipError_t nbg_launch_coop(int cls, nbg::GroupList gl, nbg::SysBufs bufs, nbg::CoopBufs coop, int grid,
+                                      unsigned long long* units, hipStream_t stream);
+extern "C" int64_t nbg_coop_stride(int cls);
 
 namespace {
 
@@ -59,10 +62,11 @@ struct Store {
   DevVec<nbg::SysDesc> desc;
   DevVec<int> queue;
   DevVec<unsigned long long> units;  // work units completed, [launch][n]
+  DevVec<double> coop;               // Jacobian snapshot slots of the cooperative workgroups
   void release() {
     state.release(); state0.release(); jac.release(); jac0.release(); ws.release();
     tt.release(); dq.release(); del.release(); ints.release(); pair.release(); desc.release(); queue.release();
-    units.release();
+    units.release(); coop.release();
   }
 };
 
@@ -88,6 +92,17 @@ long long jround(double x) { return (long long)std::nearbyint(x); }  // Julia ro
 
 void group_layout(Layout& L);
 
+// Estimated device time of system i (microseconds): steps x per-step time
+// (transit search included) measured on MI355X for n = 3, 5, 7, 9 and fitted
+// by a quadratic.  It only orders work (longest first), shares the GPU
+// between launches and picks the systems worth a second wave.
+double sys_cost(const Layout& L, int i) {
+  const auto& d = L.desc[i];
+  const double n = d.n;
+  const double steps = L.cost.empty() ? (double)(d.nsteps + 1) : L.cost[i];
+  return steps * std::max(10.0, 3.0 * n * n + 39.0 * n - 55.0);
+}
+
 void finalize_layout(Layout& L) {
   int64_t s = 0, j = 0, w = 0, t = 0, q = 0, in = 0, pr = 0;
   for (auto& d : L.desc) {
@@ -110,15 +125,7 @@ void group_layout(Layout& L) {
   const int ns = (int)L.desc.size();
   L.order.resize(ns);
   std::iota(L.order.begin(), L.order.end(), 0);
-  // device time per step (microseconds, transit search included) measured on
-  // MI355X for n = 3, 5, 7, 9 and fitted by a quadratic; it only orders work
-  // (longest first) and shares the GPU between launches
-  auto cost = [&](int i) {
-    const auto& d = L.desc[i];
-    const double n = d.n;
-    const double steps = L.cost.empty() ? (double)(d.nsteps + 1) : L.cost[i];
-    return steps * std::max(10.0, 3.0 * n * n + 39.0 * n - 55.0);
-  };
+  auto cost = [&](int i) { return sys_cost(L, i); };
   std::stable_sort(L.order.begin(), L.order.end(), [&](int a, int b) {
     const int ca = nbg_class_of(L.desc[a].n), cb = nbg_class_of(L.desc[b].n);
     if (ca != cb) return ca > cb;
@@ -160,6 +167,8 @@ struct nbg_ctx {
   std::string name;
   int max_groups_per_cu = 0;
   int grid_cap = 0;
+  int coop_mode = 1;   // 0 off, 1 systems longer than the batch's throughput bound, 2 the longest ones
+  int coop_max = 128;  // at most this many cooperative systems per batch
   int nstreams = 4;
   std::vector<hipStream_t> streams;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -283,6 +292,47 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
     used += grids[g] * frac[g];
   }
   if (group_grid) *group_grid = grids;
+  // Cooperative systems (batch runs only): the longest transit-search systems
+  // of class 9, from the front of its cost-ordered group, get a workgroup of
+  // two waves (the second runs their transit searches).  Auto mode takes the
+  // systems whose estimated time exceeds the whole batch's work spread over
+  // every wave slot of the GPU -- the ones that would otherwise set the end.
+  int gk = -1, K = 0;
+  if (!bufs && chunk == 0 && ctx->coop_mode > 0 && ctx->grid_cap == 0 && ctx->max_groups_per_cu == 0 &&
+      ctx->streams.size() > 1) {
+    for (size_t g = 0; g < G; ++g)
+      if (L.gn[g] == 9) gk = (int)g;
+    if (gk >= 0) {
+      double total = 0.0;
+      for (size_t g = 0; g < G; ++g) total += L.gcost[g];
+      const double floor_cost = total / (4.0 * ctx->cu_count);
+      const int cap = std::min({ctx->coop_max, ctx->cu_count / 2, L.gcnt[gk]});
+      while (K < cap) {
+        const int i = L.order[L.goff[gk] + K];
+        if (L.desc[i].mode != nbg::MODE_TRANSITS) break;
+        if (ctx->coop_mode == 1 && sys_cost(L, i) <= floor_cost) break;
+        ++K;
+      }
+    }
+  }
+  const bool timed_coop = K > 0 && group_ms != nullptr;
+  if (K > 0) {
+    NBG_HIP(S.coop.ensure((int64_t)K * nbg_coop_stride(9)));
+    nbg::GroupList cg{};
+    cg.ng = 1;
+    cg.nsys[0] = K;
+    cg.descs[0] = S.desc.p + L.goff[gk];
+    cg.queue[0] = S.queue.p + G;
+    const nbg::CoopBufs cb{S.coop.p, nbg_coop_stride(9)};
+    hipStream_t st = ctx->streams[0];
+    if (timed_coop) NBG_HIP(hipEventRecord(ctx->gev[2 * 15], st));
+    NBG_HIP(nbg_launch_coop(9, cg, B, cb, K, S.units.p + 13 * G, st));
+    if (timed_coop) NBG_HIP(hipEventRecord(ctx->gev[2 * 15 + 1], st));
+    // each cooperative workgroup holds two wave slots of the class-9 launch
+    grids[gk] = std::max(1, grids[gk] - 2 * K);
+    if (group_grid) *group_grid = grids;
+    std::this_thread::sleep_for(std::chrono::microseconds(2000));
+  }
   // Launch order: the largest class first, alone for a moment, so that its
   // workgroups (the largest LDS blocks, the hardest to place) are resident
   // before the smaller classes fill the remaining LDS of every CU; then the
@@ -295,12 +345,14 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
     if (g == 1) std::this_thread::sleep_for(std::chrono::microseconds(2000));
     nbg::GroupList gl{};
     for (size_t h = g; h < G; ++h) {  // own class, then the smaller ones
-      gl.nsys[gl.ng] = L.gcnt[h];
-      gl.descs[gl.ng] = S.desc.p + L.goff[h];
+      const int skip = ((int)h == gk) ? K : 0;  // the cooperative systems
+      gl.nsys[gl.ng] = L.gcnt[h] - skip;
+      gl.descs[gl.ng] = S.desc.p + L.goff[h] + skip;
       gl.queue[gl.ng] = S.queue.p + h;
       gl.ng += 1;
     }
     const int grid = grids[g];
+    if (K > 0) load[0] += 1e30;  // the cooperative launch holds stream 0
     const int si = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     load[si] += L.gcost[g] / grid;
     hipStream_t st = ctx->streams[si];
@@ -328,10 +380,24 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
     group_ms->assign(G, 0.0f);
     for (size_t g = 0; g < G; ++g) NBG_HIP(hipEventElapsedTime(&(*group_ms)[g], ctx->gev[2 * g], ctx->gev[2 * g + 1]));
   }
-  if (units) {
-    units->assign(13 * G, 0);
-    NBG_HIP(hipMemcpy(units->data(), S.units.p, sizeof(unsigned long long) * 13 * G, hipMemcpyDeviceToHost));
+  if (K > 0) {
+    unsigned long long werr = 0;
+    NBG_HIP(hipMemcpy(&werr, S.units.p + 13 * G + 12, sizeof(werr), hipMemcpyDeviceToHost));
+    if (werr) {
+      ctx->err = "cooperative workgroup: a wait between its two waves timed out";
+      return -2;
+    }
   }
+  if (units) {
+    units->assign(13 * (G + 1), 0);
+    NBG_HIP(hipMemcpy(units->data(), S.units.p, sizeof(unsigned long long) * 13 * (G + 1), hipMemcpyDeviceToHost));
+  }
+  if (group_ms) {
+    group_ms->resize(G + 1, 0.0f);
+    (*group_ms)[G] = 0.0f;
+    if (timed_coop) NBG_HIP(hipEventElapsedTime(&(*group_ms)[G], ctx->gev[2 * 15], ctx->gev[2 * 15 + 1]));
+  }
+  if (group_grid) group_grid->push_back(K);
   return 0;
 }
 
@@ -507,6 +573,16 @@ int nbg_set_launch(nbg_ctx* ctx, int32_t max_groups_per_cu, int32_t streams) {
     ctx->streams.assign(streams, nullptr);
     for (auto& s : ctx->streams) NBG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
   }
+  return 0;
+}
+
+int nbg_set_coop(nbg_ctx* ctx, int32_t mode, int32_t max_systems) {
+  if (!ctx) return -1;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(mode >= 0 && mode <= 2, "coop mode must be 0, 1 or 2");
+  NBG_ARG(max_systems >= 0, "max_systems must be >= 0");
+  ctx->coop_mode = mode;
+  ctx->coop_max = max_systems;
   return 0;
 }
 
@@ -715,11 +791,17 @@ int nbg_batch_run(nbg_batch* b, nbg_batch_stats* stats) {
     stats->newton_iters = it;
     stats->system_steps = b->steps_main + (b->L.desc.empty() || !b->L.desc[0].grad ? 0 : tr);
     stats->kernel_ms = ms;
-    for (size_t g = 0; g < b->L.gn.size(); ++g) {
+    const size_t G = b->L.gn.size();
+    for (size_t g = 0; g < G; ++g) {
       const int c = b->L.gn[g];
       stats->launch_ms[c] = gms[g];
       stats->launch_grid[c] = ggrid[g];
       for (int n = 0; n < 13; ++n) stats->launch_units[c][n] = (int64_t)gun[13 * g + n];
+    }
+    if (ggrid.size() > G && ggrid[G] > 0) {  // the cooperative launch, reported as class 1
+      stats->launch_ms[1] = gms[G];
+      stats->launch_grid[1] = ggrid[G];
+      for (int n = 0; n < 13; ++n) stats->launch_units[1][n] = (int64_t)gun[13 * G + n];
     }
   }
   return 0;

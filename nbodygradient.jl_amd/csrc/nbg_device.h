@@ -1,0 +1,1651 @@
+// nbg_device.h -- device code of the AHL21 step with Jacobian propagation
+// and transit search on gfx950 (included by the translation units that
+// instantiate its kernels: nbg_kernels.hip for the one-wave launches,
+// nbg_coop.hip for the two-wave ones, so that neither's register allocation
+// of the shared functions depends on the other's workgroup size).
+//
+// One wavefront (64 lanes) integrates one system; a persistent grid pulls
+// systems from a work queue.  Inside a wavefront:
+//  * the state (x, v, Kahan errors, masses, dq/dt) lives in LDS; the scalar
+//    pair work (universal-variable Kepler solve and its analytic Jacobian)
+//    runs one lane per pair of a level of disjoint pairs;
+//  * the 7n x 7n Jacobian without its mass rows, which the AHL21 map never
+//    changes (they stay unit rows): J and its compensation error are 6n x 7n,
+//    row-major, in LDS up to n = 6 and in a per-system HBM work area from
+//    n = 7 on; lane c owns column c for every update (pair maps, drifts, the
+//    kick and force-gradient products, Kahan sums);
+//  * the kick matrix K, the force-gradient matrix Phi and d(acceleration)/dq
+//    have non-zeros only in velocity rows x (position, mass) columns; they are
+//    stored as 3n x 4n blocks and built with one lane per column.
+// Operation order follows NbodyGradient.jl src/integrator/ahl21/ahl21.jl:5-95
+// (mul! as a k-ascending fused multiply-add accumulation), so results match
+// the CPU restatement bit for bit.  Build with -ffp-contract=off.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "nbg_pair.h"
+#include "nbg_types.h"
+
+namespace nbg {
+
+constexpr int WAVE = 64;
+
+template <int NB> struct Shape {
+  static constexpr int N3 = 3 * NB, N4 = 4 * NB, N6 = 6 * NB, N7 = 7 * NB;
+};
+
+// One pair of a sweep chunk: inputs and results of the universal-variable
+// Kepler drift (phase A) and its analytic 6x8 Jacobian (phase B).
+struct PairSlot {
+  double x0[3], v0[3];
+  KParams P;
+  double delxv[6];
+  double mi, mj, mijinv, m_i, m_j;
+  int valid, i, j, pad;
+  double kep[48], jm[6];
+  double mc[24];  // jac_ij's mass columns 6 (rows 0-5, 7-12) and 13 (same rows)
+};
+
+template <int NB> struct Smem {
+  static constexpr int N3 = 3 * NB, N4 = 4 * NB, N6 = 6 * NB, N7 = 7 * NB;
+  static constexpr int JOFF = N6 * N7;
+  // jac_step without mass rows (row 6b+k <-> Julia row 7b+k), then jac_error
+  // in the same layout at JOFF.  2*N6*N7 >= N7*N7, so jac_init fits here too.
+  // From JGLOBAL_MIN bodies on, these live in an HBM work area (jg) and the
+  // LDS keeps everything else: four such systems fit a CU instead of two.
+  // Every access to them is made by the lane that owns the column.
+  static constexpr bool JG = NB >= JGLOBAL_MIN;
+  double JJ[JG ? 1 : 2 * N6 * N7];
+  double* jg;
+  static constexpr int NP = NB * (NB - 1) / 2;  // pairs per sweep
+  static constexpr int PQ = 17;  // per-pair factors of phisalpha!, see phi_build
+  // pair slots share the space of the force-gradient arrays (union below)
+  static constexpr int CH_FIT = (int)(((2 * N3 * N4 + (NP > 0 ? NP : 1) * PQ) * sizeof(double)) / sizeof(PairSlot));
+  static constexpr int CH_MIN = CH_FIT < 4 ? 4 : CH_FIT;
+  static constexpr int CH = NP < CH_MIN ? NP : (CH_MIN > 64 ? 64 : CH_MIN);  // pairs per sweep chunk
+  // Lifetimes are disjoint: jac_phi/dadq live in the force-gradient stage
+  // (dadq also holds jac_kick around the kicks), the pair slots in the sweeps.
+  union {
+    struct {
+      double phi[N3 * N4];   // jac_phi block: row 3b+k (velocity k of body b), col 3d+p (x_p of d) or N3+d (m_d)
+      double dadq[N3 * N4];  // d a / d q: row 3b+k, same columns
+      double pq[(NP > 0 ? NP : 1) * PQ];  // per pair (lexicographic index)
+    } f;
+    PairSlot slot[CH];
+  } u;
+  // Kepler pairs (i, j) of the forward / backward sweep grouped by level:
+  // a pair's level is one more than the latest earlier pair sharing a body,
+  // so the pairs of a level are disjoint; lvw holds the level widths.
+  uint8_t plist[2][2 * (NP > 0 ? NP : 1)];
+  uint8_t lvw[2][NP > 0 ? NP : 1];
+  int nlv[2];
+  // Source of jac_ij for a pair whose masses sum to zero (the reference leaves
+  // jac_ij from the pair before it, ahl21.jl:712-713): the last pair with mass
+  // when it lies in an earlier chunk, or zero at the start of a step.
+  PairSlot jlast;
+  int seq;  // the system has such a pair: sweeps keep the sequential order
+  double dqij[14];       // dqdt_ij
+  double x[N3], v[N3], xe[N3], ve[N3], m[NB];
+  double px[N3], pv[N3], pxe[N3], pve[N3];  // s_prior
+  double a[N3];
+  double dqdt[N7], pdqdt[N7], dqdt_phi[N7], dqdt_kick[N7];
+  double gsave[NB];
+  double t, pt;
+  int64_t count[NB];
+  uint8_t pair[NB * NB];
+};
+
+// The Jacobian store of a system: an LDS array, or (from JGLOBAL_MIN bodies
+// on) the HBM work area, addressed as global memory from a wave-uniform
+// base so that accesses are global loads/stores with scalar bases rather
+// than flat ones.
+typedef __attribute__((address_space(1))) double gdouble;
+__device__ __forceinline__ gdouble* uniform_global(const double* p) {
+  const uint64_t a = reinterpret_cast<uint64_t>(p);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+  return (gdouble*)((((uint64_t)hi) << 32) | lo);
+}
+template <int NB> __device__ __forceinline__ auto jj(Smem<NB>& S) {
+  if constexpr (Smem<NB>::JG) return uniform_global(S.jg);
+  else return static_cast<double*>(S.JJ);
+}
+template <int NB> __device__ __forceinline__ auto jj(const Smem<NB>& S) {
+  if constexpr (Smem<NB>::JG) return (const gdouble*)uniform_global(S.jg);
+  else return static_cast<const double*>(S.JJ);
+}
+
+// Synchronisation of the lanes of one wave (a system's lanes are one wave):
+// LDS (and scalar) accesses complete, no reordering across.  A one-wave
+// workgroup's __syncthreads() lowers to exactly this; in a cooperative
+// workgroup the two waves run different code and sync only with themselves.
+__device__ inline void wsync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0), vmcnt/expcnt unchanged (gfx9 encoding)
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <class RS, class RE> __device__ inline void kahan(RS& s, RE& e, double add) {
+  // comp_sum: utils.jl:16-23
+  e += add;
+  const double t = s + e;
+  e = (s - t) + e;
+  s = t;
+}
+
+// jac_ij (ahl21.jl:734-758) is a function of the pair's slot: rows 0-5 and
+// 7-12 (body i, body j), column k.  jac_ij starts at zero and accumulates
+// +-m*kep (ahl21.jl:735-742); the mass columns 6 and 13 are set from jac_mass,
+// delxv and kep's k-derivative column; rows 6 and 13 stay zero.
+__device__ inline double jij_coef(const PairSlot& sl, bool rowi, int rk, int k) {
+  const double mi = sl.mi, mj = sl.mj, mijinv = sl.mijinv;
+  if (k == 6)
+    return rowi ? sl.jm[rk] * sl.m_j : (((-mj) * sl.delxv[rk]) * mijinv) - ((NBG_GNEWT * mi) * sl.kep[rk + 36]);
+  if (k == 13)
+    return rowi ? ((mi * sl.delxv[rk]) * mijinv) + ((NBG_GNEWT * mj) * sl.kep[rk + 36]) : (-sl.jm[rk]) * sl.m_i;
+  const int ck = k < 7 ? k : k - 7;
+  const double kv = sl.kep[rk + 6 * ck];
+  if (rowi == (k < 7)) return 0.0 + (rowi ? mj : mi) * kv;
+  return 0.0 - (rowi ? mj : mi) * kv;
+}
+
+// ---------------------------------------------------------------------------
+// The pair Kepler drift (jac_delxv_gamma!, ahl21.jl:766-890) and its analytic
+// Jacobian (compute_jacobian_gamma!, ahl21.jl:896-1139) as single out-of-line
+// copies shared by every body count's kernel: inlined, each would be
+// duplicated per kernel and per call site, and the hot loop of a step would
+// no longer fit the instruction cache.  Inputs and outputs pass through the
+// pair's slot in LDS; the arithmetic runs in registers.
+// ---------------------------------------------------------------------------
+static __device__ __attribute__((noinline)) void pair_drift(PairSlot* sl, double gm, double hh, bool drift_first) {
+  drift_first = __builtin_amdgcn_readfirstlane((int)drift_first) != 0;  // the same for the whole sweep
+  double x0[3], v0[3], delxv[6];
+  for (int k = 0; k < 3; ++k) {
+    x0[k] = sl->x0[k];
+    v0[k] = sl->v0[k];
+  }
+  const KParams P = jac_delxv_gamma(x0, v0, delxv, gm, hh, drift_first);
+  sl->P = P;
+  for (int k = 0; k < 6; ++k) sl->delxv[k] = delxv[k];
+}
+
+static __device__ __attribute__((noinline)) void pair_jacobian(PairSlot* sl, bool drift_first) {
+  drift_first = __builtin_amdgcn_readfirstlane((int)drift_first) != 0;  // the same for the whole sweep
+  double x0[3], v0[3], kep[48], jm[6];
+  for (int k = 0; k < 3; ++k) {
+    x0[k] = sl->x0[k];
+    v0[k] = sl->v0[k];
+  }
+  for (int q = 0; q < 48; ++q) kep[q] = 0.0;
+  for (int q = 0; q < 6; ++q) jm[q] = 0.0;
+  const KParams P = sl->P;
+  compute_jacobian_gamma(P, x0, v0, kep, jm, drift_first);
+  for (int q = 0; q < 48; ++q) sl->kep[q] = kep[q];
+  for (int q = 0; q < 6; ++q) sl->jm[q] = jm[q];
+  for (int q = 0; q < 12; ++q) {
+    sl->mc[q] = jij_coef(*sl, q < 6, q < 6 ? q : q - 6, 6);
+    sl->mc[12 + q] = jij_coef(*sl, q < 6, q < 6 ? q : q - 6, 13);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Lane <-> column helpers for the 3n x 4n blocks: column cc is (d, p) with
+// p in 0..2 for positions and p = 3 for the mass.
+// ---------------------------------------------------------------------------
+template <int NB> __device__ inline void col_dp(int cc, int& d, int& p) {
+  constexpr int N3 = 3 * NB;
+  if (cc < N3) { d = cc / 3; p = cc - 3 * d; }
+  else { d = cc - N3; p = 3; }
+}
+
+// ---------------------------------------------------------------------------
+// kickfast! with Jacobian (ahl21.jl:337-386): impulses for kicked pairs.
+// v (Kahan) and dqdt_kick are uniform (lane 0 stores); jac_kick is built with
+// one lane per column into S.u.f.dadq.
+// ---------------------------------------------------------------------------
+template <int NB> __device__ void kick_build(Smem<NB>& S, double h, bool needQ, int lane) {
+  constexpr int N3 = 3 * NB, N4 = 4 * NB;
+  double* K = S.u.f.dadq;
+  if (lane < N4)
+    for (int r = 0; r < N3; ++r) K[r * N4 + lane] = 0.0;
+  int myd = -1, myp = -1;
+  if (lane < N4) col_dp<NB>(lane, myd, myp);
+  for (int i = 0; i < NB - 1; ++i) {
+    for (int j = i + 1; j < NB; ++j) {
+      if (!S.pair[i + NB * j]) continue;
+      double rij[3];
+      for (int k = 0; k < 3; ++k) rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
+      const double r2inv = 1.0 / dot3(rij);
+      const double r3inv = r2inv * __builtin_sqrt(r2inv);
+      const double fac2 = (h * NBG_GNEWT) * r3inv;
+      const double mi = S.m[i], mj = S.m[j];
+      for (int k = 0; k < 3; ++k) {
+        const double fac = fac2 * rij[k];
+        if (lane == 0) {
+          kahan(S.v[3 * i + k], S.ve[3 * i + k], (-mj) * fac);
+          kahan(S.v[3 * j + k], S.ve[3 * j + k], mi * fac);
+          if (needQ) {
+            S.dqdt_kick[7 * i + 3 + k] -= (mj * fac) / h;
+            S.dqdt_kick[7 * j + 3 + k] += (mi * fac) / h;
+          }
+        }
+        if (lane < N4) {
+          const int ri = 3 * i + k, rj = 3 * j + k;
+          if (myp == 3 && myd == j) K[ri * N4 + lane] -= fac;
+          if (myp == 3 && myd == i) K[rj * N4 + lane] += fac;
+          const double facx = fac * (3.0 * r2inv);
+          if (myp < 3 && myd == i) {
+            K[ri * N4 + lane] += (facx * mj) * rij[myp];
+            K[rj * N4 + lane] -= (facx * mi) * rij[myp];
+          }
+          if (myp < 3 && myd == j) {
+            K[ri * N4 + lane] -= (facx * mj) * rij[myp];
+            K[rj * N4 + lane] += (facx * mi) * rij[myp];
+          }
+          if (myp == k && myd == i) {
+            K[ri * N4 + lane] -= fac2 * mj;
+            K[rj * N4 + lane] += fac2 * mi;
+          }
+          if (myp == k && myd == j) {
+            K[ri * N4 + lane] += fac2 * mj;
+            K[rj * N4 + lane] -= fac2 * mi;
+          }
+        }
+      }
+      wsync();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// phic! (ahl21.jl:392-552, kicked pairs) and phisalpha! (ahl21.jl:558-700,
+// Kepler pairs, alpha = 2) with Jacobian.  Phi and dadq columns are owned by
+// lanes; a, v and dqdt_phi are uniform (lane 0 stores).
+// ---------------------------------------------------------------------------
+template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick, bool needQ, int lane) {
+  constexpr int N3 = 3 * NB, N4 = 4 * NB;
+  const bool own = lane < N4;
+  int myd = -1, myp = -1;
+  if (own) col_dp<NB>(lane, myd, myp);
+  double* PHI = S.u.f.phi;
+  double* D = S.u.f.dadq;
+  if (anyKick) {
+    // phic!: a, dadq, jac_phi set to zero (ahl21.jl:393-399)
+    if (own)
+      for (int r = 0; r < N3; ++r) { PHI[r * N4 + lane] = 0.0; D[r * N4 + lane] = 0.0; }
+    if (lane == 0)
+      for (int r = 0; r < N3; ++r) S.a[r] = 0.0;
+    wsync();
+  }
+  if (anyKick) {
+    const double coeff = ((h * h * h) / 36.0) * NBG_GNEWT;
+    // loop 1: kicks 2h/3, accelerations, dadq (ahl21.jl:403-466)
+    for (int i = 0; i < NB - 1; ++i)
+      for (int j = i + 1; j < NB; ++j) {
+        if (!S.pair[i + NB * j]) continue;
+        double rij[3];
+        for (int k = 0; k < 3; ++k) rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
+        const double r2inv = 1.0 / dot3(rij);
+        const double r3inv = r2inv * __builtin_sqrt(r2inv);
+        const double mi = S.m[i], mj = S.m[j];
+        for (int k = 0; k < 3; ++k) {
+          double fac = (NBG_GNEWT * rij[k]) * r3inv;
+          double facv = ((fac * 2.0) * h) / 3.0;
+          const int ri = 3 * i + k, rj = 3 * j + k;
+          if (lane == 0) {
+            kahan(S.v[ri], S.ve[ri], (-mj) * facv);
+            kahan(S.v[rj], S.ve[rj], mi * facv);
+            if (needQ) {
+              S.dqdt_phi[7 * i + 3 + k] -= ((1.0 / h) * mj) * facv;
+              S.dqdt_phi[7 * j + 3 + k] += ((1.0 / h) * mi) * facv;
+            }
+            S.a[ri] -= mj * fac;
+            S.a[rj] += mi * fac;
+          }
+          if (own) {
+            if (myp == 3 && myd == j) PHI[ri * N4 + lane] -= facv;
+            if (myp == 3 && myd == i) PHI[rj * N4 + lane] += facv;
+            const double fv3 = facv * (3.0 * r2inv);
+            if (myp < 3 && myd == i) {
+              PHI[ri * N4 + lane] += (fv3 * mj) * rij[myp];
+              PHI[rj * N4 + lane] -= (fv3 * mi) * rij[myp];
+            }
+            if (myp < 3 && myd == j) {
+              PHI[ri * N4 + lane] -= (fv3 * mj) * rij[myp];
+              PHI[rj * N4 + lane] += (fv3 * mi) * rij[myp];
+            }
+            const double facd = (((2.0 * h) / 3.0) * NBG_GNEWT) * r3inv;
+            if (myp == k && myd == i) {
+              PHI[ri * N4 + lane] -= facd * mj;
+              PHI[rj * N4 + lane] += facd * mi;
+            }
+            if (myp == k && myd == j) {
+              PHI[ri * N4 + lane] += facd * mj;
+              PHI[rj * N4 + lane] -= facd * mi;
+            }
+            if (myp == 3 && myd == j) D[ri * N4 + lane] -= fac;
+            if (myp == 3 && myd == i) D[rj * N4 + lane] += fac;
+            const double f3 = fac * (3.0 * r2inv);
+            if (myp < 3 && myd == i) {
+              D[ri * N4 + lane] += (f3 * mj) * rij[myp];
+              D[rj * N4 + lane] -= (f3 * mi) * rij[myp];
+            }
+            if (myp < 3 && myd == j) {
+              D[ri * N4 + lane] -= (f3 * mj) * rij[myp];
+              D[rj * N4 + lane] += (f3 * mi) * rij[myp];
+            }
+            const double fg = NBG_GNEWT * r3inv;
+            if (myp == k && myd == i) {
+              D[ri * N4 + lane] -= fg * mj;
+              D[rj * N4 + lane] += fg * mi;
+            }
+            if (myp == k && myd == j) {
+              D[ri * N4 + lane] += fg * mj;
+              D[rj * N4 + lane] -= fg * mi;
+            }
+          }
+        }
+        wsync();
+      }
+    // loop 2: fourth-order correction for kicked pairs (ahl21.jl:471-550)
+    for (int i = 0; i < NB - 1; ++i)
+      for (int j = i + 1; j < NB; ++j) {
+        if (!S.pair[i + NB * j]) continue;
+        double rij[3], aij[3];
+        for (int k = 0; k < 3; ++k) {
+          aij[k] = S.a[3 * i + k] - S.a[3 * j + k];
+          rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
+        }
+        double dot = 0.0;
+        if (own)
+          for (int k = 0; k < 3; ++k) dot += rij[k] * (D[(3 * i + k) * N4 + lane] - D[(3 * j + k) * N4 + lane]);
+        const double r2 = dot3(rij);
+        const double r1 = __builtin_sqrt(r2);
+        const double ardot = dot3(aij, rij);
+        const double fac1 = coeff / ((r2 * r2) * r1);
+        const double fac2 = 3.0 * ardot;
+        const double mi = S.m[i], mj = S.m[j];
+        for (int k = 0; k < 3; ++k) {
+          const double fac = fac1 * (rij[k] * fac2 - r2 * aij[k]);
+          const int ri = 3 * i + k, rj = 3 * j + k;
+          if (lane == 0) {
+            kahan(S.v[ri], S.ve[ri], mj * fac);
+            kahan(S.v[rj], S.ve[rj], (-mi) * fac);
+            if (needQ) {
+              S.dqdt_phi[7 * i + 3 + k] += ((3.0 / h) * mj) * fac;
+              S.dqdt_phi[7 * j + 3 + k] -= ((3.0 / h) * mi) * fac;
+            }
+          }
+          if (own) {
+            double* pi_ = &PHI[ri * N4 + lane];
+            double* pj_ = &PHI[rj * N4 + lane];
+            if (myp == 3 && myd == j) *pi_ += fac;
+            if (myp == 3 && myd == i) *pj_ -= fac;
+            const double f5 = fac * (5.0 / r2);
+            if (myp < 3 && myd == i) { *pi_ -= (f5 * mj) * rij[myp]; *pj_ += (f5 * mi) * rij[myp]; }
+            if (myp < 3 && myd == j) { *pi_ += (f5 * mj) * rij[myp]; *pj_ -= (f5 * mi) * rij[myp]; }
+            const double fd = fac1 * fac2;
+            if (myp == k && myd == i) { *pi_ += fd * mj; *pj_ -= fd * mi; }
+            if (myp == k && myd == j) { *pi_ -= fd * mj; *pj_ += fd * mi; }
+            const double fr = ((-2.0) * fac1) * aij[k];
+            if (myp < 3 && (myd == i || myd == j)) {
+              const double fac3 = fr * rij[myp] + (((fac1 * 3.0) * rij[k]) * aij[myp]);
+              if (myd == i) { *pi_ += mj * fac3; *pj_ -= mi * fac3; }
+              else { *pi_ -= mj * fac3; *pj_ += mi * fac3; }
+            }
+            const double fa = (-fac1) * r2;
+            const double diff = D[ri * N4 + lane] - D[rj * N4 + lane];
+            *pi_ += (fa * mj) * diff;
+            *pj_ -= (fa * mi) * diff;
+            const double ff = (3.0 * fac1) * rij[k];
+            *pi_ += (ff * mj) * dot;
+            *pj_ -= (ff * mi) * dot;
+          }
+        }
+        wsync();
+      }
+  }
+  // phisalpha! (ahl21.jl:558-700, alpha = 2) for the Kepler pairs.
+  // First one lane per pair evaluates the pair's factors (the divisions and
+  // square roots) into pq; then, one row block (the three rows of body b) at
+  // a time, a lane keeps the block's rows of its dadq and jac_phi columns in
+  // registers and accumulates them from pq.  Body b's rows receive
+  // contributions only from the pairs that contain b, and in the reference's
+  // lexicographic pair order those come partner by partner in increasing
+  // order; looping over the partner c in increasing order therefore repeats
+  // every accumulation in the reference's order.  a and dadq start at zero
+  // (ahl21.jl:559-563); jac_phi keeps phic!'s terms.
+  constexpr int NP = Smem<NB>::NP, PQ = Smem<NB>::PQ;
+  double* pq = S.u.f.pq;
+  auto pidx = [](int i, int j) { return i * (2 * NB - i - 1) / 2 + (j - i - 1); };
+  const double coeff = (((2.0 * (h * h * h)) / 96.0) * 2.0) * NBG_GNEWT;
+  // loop 1 factors (ahl21.jl:569-575): fac2 = G/r^3, fac_k, f3_k = fac_k * 3/r^2
+  for (int p = lane; p < NP; p += WAVE) {
+    int i = 0, q = p;
+    while (q >= NB - 1 - i) { q -= NB - 1 - i; ++i; }
+    const int j = i + 1 + q;
+    double rij[3];
+    for (int k = 0; k < 3; ++k) rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
+    const double r2 = dot3(rij);
+    const double r3 = r2 * __builtin_sqrt(r2);
+    const double fac2 = NBG_GNEWT / r3;
+    double* o = pq + p * PQ;
+    o[0] = fac2;
+    for (int k = 0; k < 3; ++k) {
+      const double fac = fac2 * rij[k];
+      o[1 + k] = fac;
+      o[4 + k] = fac * (3.0 / r2);
+    }
+  }
+  wsync();
+  // loop 1: accelerations and dadq (ahl21.jl:567-602).  For the lane's
+  // column (body myd, coordinate myp; myp = 3 for the mass) the reference's
+  // conditional updates of pair (i, j) reduce, per role of myd in the pair,
+  // to at most two additions in a fixed order: t1 then t2, each -0.0 (which
+  // leaves any sum unchanged) when it does not apply.  "self" is the body b
+  // of the row block, "other" its partner c.
+  const bool mpos = myp >= 0 && myp < 3, mmass = myp == 3;
+  const int ry = mpos ? myp : 0;
+#pragma unroll 1
+  for (int b = 0; b < NB; ++b) {
+    double ab[3] = {0.0, 0.0, 0.0};
+    double db[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int c = 0; c < NB; ++c) {
+      if (c == b) continue;
+      const int i = c < b ? c : b, j = c < b ? b : c;
+      if (S.pair[i + NB * j]) continue;
+      const double* o = pq + pidx(i, j) * PQ;
+      const double fac2 = o[0];
+      const double rmy = S.x[3 * i + ry] - S.x[3 * j + ry];
+      const bool bi = (b == i);
+      const double m = bi ? S.m[j] : S.m[i];  // the partner's mass
+      const bool self_pos = mpos && myd == b, other_pos = mpos && myd == c;
+      const bool other_mass = mmass && myd == c;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double fac = o[1 + k];
+        const double f3 = o[4 + k];
+        const double mf = m * fac;
+        ab[k] += bi ? -mf : mf;
+        const double x3 = (f3 * m) * rmy;
+        const double f2m = fac2 * m;
+        const bool pk = myp == k;
+        const double t1 = other_mass ? (bi ? -fac : fac) : (self_pos ? x3 : (other_pos ? -x3 : -0.0));
+        const double t2 = (pk && self_pos) ? -f2m : ((pk && other_pos) ? f2m : -0.0);
+        db[k] = (db[k] + t1) + t2;
+      }
+    }
+    if (own)
+      for (int k = 0; k < 3; ++k) D[(3 * b + k) * N4 + lane] = db[k];
+    if (lane == 0)
+      for (int k = 0; k < 3; ++k) S.a[3 * b + k] = ab[k];
+  }
+  wsync();
+  // loop 2 factors (ahl21.jl:609-640), needing the accelerations above
+  for (int p = lane; p < NP; p += WAVE) {
+    int i = 0, q = p;
+    while (q >= NB - 1 - i) { q -= NB - 1 - i; ++i; }
+    const int j = i + 1 + q;
+    double rij[3], aij[3];
+    for (int k = 0; k < 3; ++k) {
+      aij[k] = S.a[3 * i + k] - S.a[3 * j + k];
+      rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
+    }
+    const double r2 = dot3(rij);
+    const double r1 = __builtin_sqrt(r2);
+    const double ardot = dot3(aij, rij);
+    const double fac1 = coeff / ((r2 * r2) * r1);
+    const double mi = S.m[i], mj = S.m[j];
+    const double fac2 = ((2.0 * NBG_GNEWT) * (mi + mj)) / r1 + 3.0 * ardot;
+    double* o = pq + p * PQ;
+    for (int k = 0; k < 3; ++k) {
+      const double fac = fac1 * (rij[k] * fac2 - r2 * aij[k]);
+      o[k] = fac;                                                  // fac
+      o[3 + k] = fac * (5.0 / r2);                                 // f5
+      o[6 + k] = (((2.0 * NBG_GNEWT) * fac1) * rij[k]) / r1;       // fm
+      o[9 + k] = ((-2.0) * fac1) * ((((rij[k] * NBG_GNEWT) * (mi + mj)) / (r2 * r1)) + aij[k]);  // fr
+      // (fac1 * 3) * rij[k] multiplies aij[p] in fac3 and is ff = (3 * fac1) * rij[k]
+      // as well (IEEE multiplication commutes)
+      o[12 + k] = (fac1 * 3.0) * rij[k];
+    }
+    o[15] = fac1 * fac2;   // fd
+    o[16] = (-fac1) * r2;  // fa
+  }
+  wsync();
+  // loop 2: velocity corrections, dqdt_phi and jac_phi (ahl21.jl:607-698),
+  // with the same role reduction as loop 1: per k, at most three
+  // role-dependent additions (t1, t2, t3), then the dadq terms.
+  const double q3h = 3.0 / h;
+#pragma unroll 1
+  for (int b = 0; b < NB; ++b) {
+    double vb[3], veb[3], qb[3], pb[3];
+    for (int k = 0; k < 3; ++k) {
+      vb[k] = S.v[3 * b + k];
+      veb[k] = S.ve[3 * b + k];
+      qb[k] = needQ ? S.dqdt_phi[7 * b + 3 + k] : 0.0;
+      pb[k] = (anyKick && own) ? PHI[(3 * b + k) * N4 + lane] : 0.0;
+    }
+#pragma unroll 1
+    for (int c = 0; c < NB; ++c) {
+      if (c == b) continue;
+      const int i = c < b ? c : b, j = c < b ? b : c;
+      if (S.pair[i + NB * j]) continue;
+      const double* o = pq + pidx(i, j) * PQ;
+      double rij[3], Di[3], Dj[3];
+      for (int k = 0; k < 3; ++k) {
+        rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
+        Di[k] = own ? D[(3 * i + k) * N4 + lane] : 0.0;
+        Dj[k] = own ? D[(3 * j + k) * N4 + lane] : 0.0;
+      }
+      double dot = 0.0;
+      if (own)
+        for (int k = 0; k < 3; ++k) dot += rij[k] * (Di[k] - Dj[k]);
+      const double rmy = ry == 0 ? rij[0] : (ry == 1 ? rij[1] : rij[2]);
+      const double amy = S.a[3 * i + ry] - S.a[3 * j + ry];
+      const bool bi = (b == i);
+      const double m = bi ? S.m[j] : S.m[i];  // the partner's mass
+      const bool self_pos = mpos && myd == b, other_pos = mpos && myd == c;
+      const bool self_mass = mmass && myd == b, other_mass = mmass && myd == c;
+      const double fd = o[15], fa = o[16];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double fac = o[k], f5 = o[3 + k], fm = o[6 + k], fr = o[9 + k], ff = o[12 + k];
+        const double mf = m * fac;
+        kahan(vb[k], veb[k], bi ? mf : -mf);
+        if (needQ) {
+          const double qm = (q3h * m) * fac;
+          qb[k] += bi ? qm : -qm;
+        }
+        const double diff = Di[k] - Dj[k];
+        const double x5 = (f5 * m) * rmy;
+        const double y = fm * m;
+        const double sy = bi ? y : -y, sfac = bi ? fac : -fac;
+        const double fdm = fd * m;
+        const double fac3 = fr * rmy + o[12 + k] * amy;
+        const double mf3 = m * fac3;
+        const bool pk = myp == k;
+        const double t1 = self_pos ? -x5 : (other_pos ? x5 : (self_mass ? sy : (other_mass ? sfac : -0.0)));
+        const double t2 = (pk && self_pos) ? fdm : ((pk && other_pos) ? -fdm : (other_mass ? sy : -0.0));
+        const double t3 = self_pos ? mf3 : (other_pos ? -mf3 : -0.0);
+        const double td = (fa * m) * diff, tf = (ff * m) * dot;
+        pb[k] = (((((pb[k] + t1) + t2) + t3) + (bi ? td : -td)) + (bi ? tf : -tf));
+      }
+    }
+    if (own)
+      for (int k = 0; k < 3; ++k) PHI[(3 * b + k) * N4 + lane] = pb[k];
+    if (lane == 0)
+      for (int k = 0; k < 3; ++k) {
+        S.v[3 * b + k] = vb[k];
+        S.ve[3 * b + k] = veb[k];
+        if (needQ) S.dqdt_phi[7 * b + 3 + k] = qb[k];
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Jacobian column kernels (lane owns column c).
+// ---------------------------------------------------------------------------
+// Position rows of column c (pre-update), the operand of K*J and Phi*J.
+template <int NB> __device__ inline void load_pos(const Smem<NB>& S, int c, double* P) {
+  const auto J_ = jj<NB>(S);
+  constexpr int N7 = 7 * NB;
+#pragma unroll
+  for (int d = 0; d < NB; ++d)
+#pragma unroll
+    for (int p = 0; p < 3; ++p) P[3 * d + p] = J_[(unsigned)((6 * d + p) * N7 + c)];
+}
+
+// J[velocity rows, c] += (B * J)[velocity rows, c] with Kahan compensation,
+// B = K or Phi (3n x 4n block); then the zero-addend Kahan pass over the
+// position rows that comp_sum_matrix! applies to the rest of the matrix.
+template <int NB> __device__ inline void vel_product_add(Smem<NB>& S, const double* B, int c, const double* P) {
+  const auto J_ = jj<NB>(S);
+  constexpr int N3 = 3 * NB, N4 = 4 * NB, N7 = 7 * NB;
+#pragma unroll 1
+  for (int rv = 0; rv < N3; ++rv) {
+    double s = 0.0;
+    const double* Br = B + rv * N4;
+#pragma unroll
+    for (int d = 0; d < NB; ++d) {
+      s = fmad(Br[3 * d + 0], P[3 * d + 0], s);
+      s = fmad(Br[3 * d + 1], P[3 * d + 1], s);
+      s = fmad(Br[3 * d + 2], P[3 * d + 2], s);
+      s = fmad(Br[N3 + d], (c == 7 * d + 6) ? 1.0 : 0.0, s);
+    }
+    const int b = rv / 3, k = rv - 3 * b;
+    const int r = 6 * b + 3 + k;
+    kahan(J_[(unsigned)(r * N7 + c)], J_[(unsigned)(Smem<NB>::JOFF + r * N7 + c)], s);
+  }
+}
+
+template <int NB> __device__ inline void zero_add_rows(Smem<NB>& S, int c, bool pos, bool vel) {
+  const auto J_ = jj<NB>(S);
+  constexpr int N7 = 7 * NB;
+#pragma unroll 1
+  for (int b = 0; b < NB; ++b)
+    for (int k = 0; k < 6; ++k) {
+      if ((k < 3 && !pos) || (k >= 3 && !vel)) continue;
+      const int r = 6 * b + k;
+      auto& J = J_[(unsigned)(r * N7 + c)];
+      auto& E = J_[(unsigned)(Smem<NB>::JOFF + r * N7 + c)];
+      E += 0.0;
+      const double t = J + E;
+      E += J - t;
+      J = t;
+    }
+}
+
+// drift_grad! Jacobian part (ahl21.jl:326-328)
+template <int NB> __device__ inline void drift_cols(Smem<NB>& S, int c, double hh) {
+  const auto J_ = jj<NB>(S);
+  constexpr int N7 = 7 * NB;
+#pragma unroll 1
+  for (int b = 0; b < NB; ++b)
+    for (int k = 0; k < 3; ++k) {
+      const int r = 6 * b + k;
+      kahan(J_[(unsigned)(r * N7 + c)], J_[(unsigned)(Smem<NB>::JOFF + r * N7 + c)], hh * J_[(unsigned)((r + 3) * N7 + c)]);
+    }
+}
+
+// Pair map on rows (i, j): copy_submatrix!, mul!, comp_sum_matrix!,
+// ypoc_submatrix! (ahl21.jl:31-42, utils.jl:51-100) for column c.  Row q of
+// jac_ij * a is summed over k = 0..13 in ascending order with one fused
+// multiply-add per term, as jij_coef's coefficients would give it: the
+// Kepler-block coefficient +-(0 + m*kep) enters as m*kep times +-a[k] (the
+// running sum starts at +0 and can never become -0, so the sign of a zero
+// coefficient cannot change it), and the mass columns 6 and 13 take their
+// coefficients from the slot, where pair_jacobian stored them.
+template <int NB> __device__ inline void pair_cols(Smem<NB>& S, const PairSlot& sl, int i, int j, int c) {
+  const auto J_ = jj<NB>(S);
+  constexpr int N7 = 7 * NB;
+  const int ri = 6 * i, rj = 6 * j;
+  double a[12], e[12];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    a[k] = J_[(unsigned)((ri + k) * N7 + c)];
+    a[6 + k] = J_[(unsigned)((rj + k) * N7 + c)];
+    e[k] = J_[(unsigned)(Smem<NB>::JOFF + (ri + k) * N7 + c)];
+    e[6 + k] = J_[(unsigned)(Smem<NB>::JOFF + (rj + k) * N7 + c)];
+  }
+  const double a6 = (c == 7 * i + 6) ? 1.0 : 0.0;
+  const double a13 = (c == 7 * j + 6) ? 1.0 : 0.0;
+  const double mi = sl.mi, mj = sl.mj;
+  double o[12];
+#pragma unroll
+  for (int q = 0; q < 12; ++q) o[q] = 0.0;
+  // k = 0..5: rows of i get +mj*kep * a_i[k], rows of j -mi*kep * a_i[k]
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+#pragma unroll
+    for (int rk = 0; rk < 6; ++rk) {
+      const double kv = sl.kep[rk + 6 * k];
+      o[rk] = fmad(mj * kv, a[k], o[rk]);
+      o[6 + rk] = fmad(mi * kv, -a[k], o[6 + rk]);
+    }
+#pragma unroll
+  for (int q = 0; q < 12; ++q) o[q] = fmad(sl.mc[q], a6, o[q]);
+  // k = 7..12: rows of i get -mj*kep * a_j[k], rows of j +mi*kep * a_j[k]
+#pragma unroll
+  for (int k = 0; k < 6; ++k)
+#pragma unroll
+    for (int rk = 0; rk < 6; ++rk) {
+      const double kv = sl.kep[rk + 6 * k];
+      o[rk] = fmad(mj * kv, -a[6 + k], o[rk]);
+      o[6 + rk] = fmad(mi * kv, a[6 + k], o[6 + rk]);
+    }
+#pragma unroll
+  for (int q = 0; q < 12; ++q) o[q] = fmad(sl.mc[12 + q], a13, o[q]);
+#pragma unroll
+  for (int q = 0; q < 12; ++q) {
+    double sv = a[q], se = e[q];
+    se += o[q];
+    const double t = sv + se;
+    se += sv - t;
+    sv = t;
+    const int r = q < 6 ? ri + q : rj + (q - 6);
+    J_[(unsigned)(r * N7 + c)] = sv;
+    J_[(unsigned)(Smem<NB>::JOFF + r * N7 + c)] = se;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One pair sweep of ahl21! (forward: ahl21.jl:25-45, drift_first = true;
+// backward: :56-78, drift_first = false) in chunks of pairs:
+//  A. sequentially per pair: the universal-variable Kepler drift
+//     (jac_delxv_gamma!, ahl21.jl:766-890) and the Kahan state update of
+//     kepler_driftij_gamma! (:706-733), all lanes alike, lane 0 stores;
+//  B. the analytic Jacobians (compute_jacobian_gamma!, :896-1139) of all the
+//     chunk's pairs at once, one lane per pair;
+//  C. sequentially per pair: jac_ij (:734-758), the Jacobian row update
+//     (copy_submatrix!, mul!, comp_sum_matrix!, ypoc_submatrix!) with one lane
+//     per column, and the dq/dt update.
+// The state never reads the Jacobian and the Jacobian updates never read the
+// state, so running all state updates of a chunk before its Jacobian updates
+// gives the reference's results exactly.
+// ---------------------------------------------------------------------------
+template <int NB>
+__device__ __attribute__((always_inline)) inline void sweep(Smem<NB>& S, int dir, double hh, bool needJ, bool needQ, int lane) {
+  constexpr int N7 = 7 * NB;
+  constexpr int CH = Smem<NB>::CH;
+  const bool drift_first = (dir == 0);
+  const int nlev = S.nlv[dir];
+  const uint8_t* pl = S.plist[dir];
+  const uint8_t* lw = S.lvw[dir];
+  int lev = 0, c0 = 0;
+  while (lev < nlev) {
+    // a chunk is a run of whole levels holding at most CH pairs
+    int cnt = 0, lev_end = lev;
+    while (lev_end < nlev && cnt + lw[lev_end] <= CH) cnt += lw[lev_end++];
+    // ---- A: Kepler drifts and state updates, one level at a time: the
+    // pairs of a level share no body, so lane l solves pair l of the level
+    int q0 = 0;
+#pragma unroll 1
+    for (int L = lev; L < lev_end; ++L) {
+      const int w = lw[L];
+      const bool mine = lane < w;
+      if (mine) {
+        const int q = q0 + lane;
+        const int i = pl[2 * (c0 + q)], j = pl[2 * (c0 + q) + 1];
+        PairSlot& sl = S.u.slot[q];
+        for (int k = 0; k < 3; ++k) {
+          sl.x0[k] = S.x[3 * i + k] - S.x[3 * j + k];
+          sl.v0[k] = S.v[3 * i + k] - S.v[3 * j + k];
+        }
+        const double m_i = S.m[i], m_j = S.m[j];
+        const double gm = NBG_GNEWT * (m_i + m_j);
+        if (gm != 0.0) {
+          pair_drift(&sl, gm, hh, drift_first);
+          const double mijinv = 1.0 / (m_i + m_j);
+          const double mi = m_i * mijinv;
+          const double mj = m_j * mijinv;
+          for (int k = 0; k < 3; ++k) {
+            kahan(S.x[3 * i + k], S.xe[3 * i + k], mj * sl.delxv[k]);
+            kahan(S.x[3 * j + k], S.xe[3 * j + k], (-mi) * sl.delxv[k]);
+          }
+          for (int k = 0; k < 3; ++k) {
+            kahan(S.v[3 * i + k], S.ve[3 * i + k], mj * sl.delxv[3 + k]);
+            kahan(S.v[3 * j + k], S.ve[3 * j + k], (-mi) * sl.delxv[3 + k]);
+          }
+          sl.mi = mi; sl.mj = mj; sl.mijinv = mijinv; sl.m_i = m_i; sl.m_j = m_j;
+        }
+        sl.valid = (gm != 0.0); sl.i = i; sl.j = j;
+      }
+      q0 += w;
+      wsync();
+    }
+    lev = lev_end;
+    c0 += cnt;
+    if (!(needJ || needQ)) continue;
+    // ---- B: analytic Kepler-drift Jacobians, one lane per pair
+    if (lane < cnt && S.u.slot[lane].valid) pair_jacobian(&S.u.slot[lane], drift_first);
+    wsync();
+    // ---- C: jac_ij, Jacobian rows and dq/dt, in sweep order
+    int lastq = -1;  // last slot of this chunk whose pair has mass
+#pragma unroll 1
+    for (int q = 0; q < cnt; ++q) {
+      const PairSlot& sl = S.u.slot[q];
+      const int i = sl.i, j = sl.j;
+      if (sl.valid) {
+        lastq = q;
+        if (needQ && lane < 6) {  // dqdt_ij (ahl21.jl:743-748)
+          S.dqij[lane] = sl.mj * sl.kep[lane + 42];
+          S.dqij[7 + lane] = (-sl.mi) * sl.kep[lane + 42];
+        }
+      }
+      // (gm == 0: the reference returns before touching jac_ij / dqdt_ij and
+      // the caller reuses the previous values: the last pair with mass, or
+      // jlast when that lies in an earlier chunk; dqij keeps its values.)
+      const PairSlot& src = lastq >= 0 ? S.u.slot[lastq] : S.jlast;
+      if (needJ)
+        for (int c = lane; c < N7; c += WAVE) pair_cols<NB>(S, src, i, j, c);
+      if (needQ) {
+        double q1[14];
+        for (int k = 0; k < 7; ++k) {
+          q1[k] = S.dqdt[7 * i + k];
+          q1[7 + k] = S.dqdt[7 * j + k];
+        }
+        if (lane < 14) {
+          double s2 = 0.0;
+          if (lane != 6 && lane != 13) {
+            const bool rowi = lane < 7;
+            const int rk = rowi ? lane : lane - 7;
+            for (int k = 0; k < 14; ++k) s2 = fmad(jij_coef(src, rowi, rk, k), q1[k], s2);
+          } else {
+            for (int k = 0; k < 14; ++k) s2 = fmad(0.0, q1[k], s2);
+          }
+          const double nd = (S.dqij[lane] * 0.5) + (q1[lane] + s2);
+          S.dqij[lane] = nd;
+          S.dqdt[lane < 7 ? 7 * i + lane : 7 * j + lane - 7] = nd;
+        }
+      }
+      wsync();
+    }
+    // keep the last pair with mass for a massless pair at the start of a later chunk
+    if (S.seq && lastq >= 0) {
+      const double* from = reinterpret_cast<const double*>(&S.u.slot[lastq]);
+      double* to = reinterpret_cast<double*>(&S.jlast);
+      for (int w = lane; w < (int)(sizeof(PairSlot) / sizeof(double)); w += WAVE) to[w] = from[w];
+      wsync();
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One AHL21 step: ahl21!(s, d, h) (ahl21.jl:5-95).  needJ: propagate the
+// Jacobian; needQ: compute dq/dt.  The state arithmetic is identical for
+// every flag combination (so grad and no-grad runs agree bit for bit, as the
+// reference's no-grad map ahl21_no_grad.jl:6-18 does).
+// ---------------------------------------------------------------------------
+template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, bool needQ, bool anyKick, int lane) {
+  constexpr int N3 = 3 * NB, N4 = 4 * NB, N7 = 7 * NB;
+  const double h2 = 0.5 * h;
+  const double h6 = h / 6.0;
+  // zero_out!(d); fill!(s.dqdt, 0)
+  if (lane < 14) S.dqij[lane] = 0.0;
+  if (S.seq)  // jac_ij = 0 at the start of the step (zero_out!, ahl21.jl:8)
+    for (int w = lane; w < (int)(sizeof(PairSlot) / sizeof(double)); w += WAVE)
+      reinterpret_cast<double*>(&S.jlast)[w] = 0.0;
+  if (needQ)
+    for (int r = lane; r < N7; r += WAVE) {
+      S.dqdt[r] = 0.0;
+      S.dqdt_phi[r] = 0.0;
+      S.dqdt_kick[r] = 0.0;
+    }
+  wsync();
+  // ---- kickfast!(s, d, h/6)
+  if (anyKick) kick_build<NB>(S, h6, needQ, lane);
+  wsync();
+  if (needQ) {
+    // dqdt_kick ./= 6; tmp7n = K*dqdt (dqdt = 0 here); dqdt .+= dqdt_kick .+ tmp7n
+    for (int r = lane; r < N7; r += WAVE) {
+      double tmp = 0.0;
+      if (anyKick) {
+        const int b = r / 7, k = r - 7 * b;
+        if (k >= 3 && k < 6) {
+          const double* Kr = S.u.f.dadq + (3 * b + k - 3) * N4;
+          for (int d = 0; d < NB; ++d) {
+            tmp = fmad(Kr[3 * d + 0], S.dqdt[7 * d + 0], tmp);
+            tmp = fmad(Kr[3 * d + 1], S.dqdt[7 * d + 1], tmp);
+            tmp = fmad(Kr[3 * d + 2], S.dqdt[7 * d + 2], tmp);
+            tmp = fmad(Kr[N3 + d], S.dqdt[7 * d + 6], tmp);
+          }
+        }
+      }
+      const double dk = S.dqdt_kick[r] / 6.0;
+      S.dqdt[r] = S.dqdt[r] + (dk + tmp);
+    }
+  }
+  // ---- Jacobian: jac_copy = K*J (pre-drift), drift_grad!, J += jac_copy
+  if (needJ)
+    for (int c = lane; c < N7; c += WAVE) {
+      double P[N3];
+      load_pos<NB>(S, c, P);
+      drift_cols<NB>(S, c, h2);
+      zero_add_rows<NB>(S, c, true, !anyKick);
+      if (anyKick) vel_product_add<NB>(S, S.u.f.dadq, c, P);
+    }
+  // ---- drift of the state (drift_grad! / drift!)
+  if (lane == 0)
+    for (int q = 0; q < N3; ++q) kahan(S.x[q], S.xe[q], h2 * S.v[q]);
+  wsync();
+  if (needQ) {
+    for (int q = lane; q < N3; q += WAVE) {
+      const int b = q / 3, k = q - 3 * b;
+      S.dqdt[7 * b + k] = 0.5 * S.v[q] + h2 * S.dqdt[7 * b + 3 + k];
+    }
+  }
+  wsync();
+  // ---- forward pair sweep (drift first)
+  sweep<NB>(S, 0, h2, needJ, needQ, lane);
+  // ---- phic!, phisalpha!
+  phi_build<NB>(S, h, anyKick, needQ, lane);
+  wsync();
+  if (needJ)
+    for (int c = lane; c < N7; c += WAVE) {
+      double P[N3];
+      load_pos<NB>(S, c, P);
+      vel_product_add<NB>(S, S.u.f.phi, c, P);
+      zero_add_rows<NB>(S, c, true, false);
+    }
+  if (needQ) {
+    // tmp7n = jac_phi*dqdt; dqdt .+= dqdt_phi .+ tmp7n
+    double nv[2];
+    int nr = 0;
+    for (int r = lane; r < N7; r += WAVE, ++nr) {
+      double tmp = 0.0;
+      const int b = r / 7, k = r - 7 * b;
+      if (k >= 3 && k < 6) {
+        const double* Pr = S.u.f.phi + (3 * b + k - 3) * N4;
+        for (int d = 0; d < NB; ++d) {
+          tmp = fmad(Pr[3 * d + 0], S.dqdt[7 * d + 0], tmp);
+          tmp = fmad(Pr[3 * d + 1], S.dqdt[7 * d + 1], tmp);
+          tmp = fmad(Pr[3 * d + 2], S.dqdt[7 * d + 2], tmp);
+          tmp = fmad(Pr[N3 + d], S.dqdt[7 * d + 6], tmp);
+        }
+      }
+      nv[nr] = S.dqdt[r] + (S.dqdt_phi[r] + tmp);
+    }
+    wsync();
+    nr = 0;
+    for (int r = lane; r < N7; r += WAVE, ++nr) S.dqdt[r] = nv[nr];
+  }
+  wsync();
+  // ---- backward pair sweep (drift after)
+  sweep<NB>(S, 1, h2, needJ, needQ, lane);
+  // ---- drift
+  if (needJ)
+    for (int c = lane; c < N7; c += WAVE) drift_cols<NB>(S, c, h2);
+  if (lane == 0)
+    for (int q = 0; q < N3; ++q) kahan(S.x[q], S.xe[q], h2 * S.v[q]);
+  wsync();
+  if (needQ) {
+    for (int q = lane; q < N3; q += WAVE) {
+      const int b = q / 3, k = q - 3 * b;
+      S.dqdt[7 * b + k] += 0.5 * S.v[q] + h2 * S.dqdt[7 * b + 3 + k];
+    }
+    for (int r = lane; r < N7; r += WAVE) S.dqdt_kick[r] = 0.0;
+  }
+  wsync();
+  // ---- kickfast!(s, d, h/6)
+  if (anyKick) kick_build<NB>(S, h6, needQ, lane);
+  wsync();
+  if (needQ) {
+    double nv[2];
+    int nr = 0;
+    for (int r = lane; r < N7; r += WAVE, ++nr) {
+      double tmp = 0.0;
+      if (anyKick) {
+        const int b = r / 7, k = r - 7 * b;
+        if (k >= 3 && k < 6) {
+          const double* Kr = S.u.f.dadq + (3 * b + k - 3) * N4;
+          for (int d = 0; d < NB; ++d) {
+            tmp = fmad(Kr[3 * d + 0], S.dqdt[7 * d + 0], tmp);
+            tmp = fmad(Kr[3 * d + 1], S.dqdt[7 * d + 1], tmp);
+            tmp = fmad(Kr[3 * d + 2], S.dqdt[7 * d + 2], tmp);
+            tmp = fmad(Kr[N3 + d], S.dqdt[7 * d + 6], tmp);
+          }
+        }
+      }
+      const double dk = S.dqdt_kick[r] / 6.0;
+      nv[nr] = S.dqdt[r] + (dk + tmp);
+    }
+    wsync();
+    nr = 0;
+    for (int r = lane; r < N7; r += WAVE, ++nr) S.dqdt[r] = nv[nr];
+  }
+  if (needJ)
+    for (int c = lane; c < N7; c += WAVE) {
+      if (anyKick) {
+        double P[N3];
+        load_pos<NB>(S, c, P);
+        vel_product_add<NB>(S, S.u.f.dadq, c, P);
+        zero_add_rows<NB>(S, c, true, false);
+      } else {
+        zero_add_rows<NB>(S, c, true, true);
+      }
+    }
+  wsync();
+}
+
+// ---------------------------------------------------------------------------
+// Transit search (timing.jl:3-110) and element gradients (timing.jl:112-138)
+// ---------------------------------------------------------------------------
+template <int NB> __device__ inline double gsky(const Smem<NB>& S, int i, int j) {
+  return (S.x[3 * j] - S.x[3 * i]) * (S.v[3 * j] - S.v[3 * i]) +
+         (S.x[3 * j + 1] - S.x[3 * i + 1]) * (S.v[3 * j + 1] - S.v[3 * i + 1]);
+}
+template <int NB> __device__ inline double gdot(const Smem<NB>& S, int i, int j) {
+  return (((S.x[3 * j] - S.x[3 * i]) * (S.dqdt[7 * j + 3] - S.dqdt[7 * i + 3]) +
+           (S.x[3 * j + 1] - S.x[3 * i + 1]) * (S.dqdt[7 * j + 4] - S.dqdt[7 * i + 4])) +
+          (S.v[3 * j] - S.v[3 * i]) * (S.dqdt[7 * j + 0] - S.dqdt[7 * i + 0])) +
+         (S.v[3 * j + 1] - S.v[3 * i + 1]) * (S.dqdt[7 * j + 1] - S.dqdt[7 * i + 1]);
+}
+
+template <int NB> __device__ inline void save_prior(Smem<NB>& S, int lane) {
+  constexpr int N3 = 3 * NB, N7 = 7 * NB;
+  for (int q = lane; q < N3; q += WAVE) {
+    S.px[q] = S.x[q]; S.pv[q] = S.v[q]; S.pxe[q] = S.xe[q]; S.pve[q] = S.ve[q];
+  }
+  for (int q = lane; q < N7; q += WAVE) S.pdqdt[q] = S.dqdt[q];
+  if (lane == 0) S.pt = S.t;
+  wsync();
+}
+template <int NB> __device__ inline void restore_prior(Smem<NB>& S, int lane) {
+  constexpr int N3 = 3 * NB, N7 = 7 * NB;
+  for (int q = lane; q < N3; q += WAVE) {
+    S.x[q] = S.px[q]; S.v[q] = S.pv[q]; S.xe[q] = S.pxe[q]; S.ve[q] = S.pve[q];
+  }
+  for (int q = lane; q < N7; q += WAVE) S.dqdt[q] = S.pdqdt[q];
+  if (lane == 0) S.t = S.pt;
+  wsync();
+}
+
+template <int NB>
+__device__ void findtransit(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int ti, int jb, double dt0, int lane,
+                            double* ws) {
+  const auto J_ = jj<NB>(S);
+  constexpr int N6 = 6 * NB, N7 = 7 * NB;
+  const int n = NB;
+  double stmp = 0.0;
+  double tt1 = dt0 + 1.0;
+  double tt2 = dt0 + 2.0;
+  int iter = 0;
+  while (true) {
+    tt2 = tt1;
+    tt1 = dt0;
+    restore_prior<NB>(S, lane);
+    ahl21_step<NB>(S, dt0, false, true, D.any_kick != 0, lane);
+    const double gs = gsky<NB>(S, ti, jb);
+    const double gd = gdot<NB>(S, ti, jb);
+    const double dt = -gs / gd;
+    kahan(dt0, stmp, dt);
+    iter += 1;
+    if (iter >= 20 || dt0 == tt1 || dt0 == tt2) break;
+  }
+  const int64_t c = S.count[jb] - 1;
+  const int nb = D.ntbv;
+  double* tt = B.tt + D.off_tt;
+  if (D.grad) {
+    restore_prior<NB>(S, lane);
+    // keep the prior Jacobian: the step below updates J in place
+    for (int col = lane; col < N7; col += WAVE)
+      for (int r = 0; r < N6; ++r) {
+        ws[r * N7 + col] = J_[(unsigned)(r * N7 + col)];
+        ws[N6 * N7 + r * N7 + col] = J_[(unsigned)(Smem<NB>::JOFF + r * N7 + col)];
+      }
+    ahl21_step<NB>(S, dt0, true, true, D.any_kick != 0, lane);
+    const double gd = gdot<NB>(S, ti, jb);
+    const double dx = S.x[3 * jb] - S.x[3 * ti], dy = S.x[3 * jb + 1] - S.x[3 * ti + 1];
+    const double dvx = S.v[3 * jb] - S.v[3 * ti], dvy = S.v[3 * jb + 1] - S.v[3 * ti + 1];
+    double* dq = B.dq + D.off_dq;
+    const int64_t ntt = D.ntt;
+    for (int col = lane; col < N7; col += WAVE) {
+      const int rj = 6 * jb, ri = 6 * ti;
+      const double d0 = -((((J_[(unsigned)(rj * N7 + col)] - J_[(unsigned)(ri * N7 + col)]) * dvx +
+                            (J_[(unsigned)((rj + 1) * N7 + col)] - J_[(unsigned)((ri + 1) * N7 + col)]) * dvy) +
+                           (J_[(unsigned)((rj + 3) * N7 + col)] - J_[(unsigned)((ri + 3) * N7 + col)]) * dx) +
+                          (J_[(unsigned)((rj + 4) * N7 + col)] - J_[(unsigned)((ri + 4) * N7 + col)]) * dy) /
+                        gd;
+      const int64_t base = (int64_t)jb + n * (c + ntt * (int64_t)col);
+      dq[0 + nb * base] = d0;
+      if (nb == 3) {
+        const double vsky = __builtin_sqrt(dvx * dvx + dvy * dvy);
+        const double dvdt = (dvx * (S.dqdt[7 * jb + 3] - S.dqdt[7 * ti + 3]) +
+                             dvy * (S.dqdt[7 * jb + 4] - S.dqdt[7 * ti + 4])) / vsky;
+        dq[1 + nb * base] = ((J_[(unsigned)((rj + 3) * N7 + col)] - J_[(unsigned)((ri + 3) * N7 + col)]) * dvx +
+                             (J_[(unsigned)((rj + 4) * N7 + col)] - J_[(unsigned)((ri + 4) * N7 + col)]) * dvy) / vsky +
+                            dvdt * d0;
+        dq[2 + nb * base] = 2.0 * ((J_[(unsigned)(rj * N7 + col)] - J_[(unsigned)(ri * N7 + col)]) * dx +
+                                   (J_[(unsigned)((rj + 1) * N7 + col)] - J_[(unsigned)((ri + 1) * N7 + col)]) * dy);
+      }
+    }
+    if (lane == 0) {
+      tt[0 + nb * (jb + n * c)] = S.t + dt0;
+      if (nb == 3) {
+        tt[1 + nb * (jb + n * c)] = __builtin_sqrt(dvx * dvx + dvy * dvy);
+        tt[2 + nb * (jb + n * c)] = dx * dx + dy * dy;
+      }
+    }
+    for (int col = lane; col < N7; col += WAVE)
+      for (int r = 0; r < N6; ++r) {
+        J_[(unsigned)(r * N7 + col)] = ws[r * N7 + col];
+        J_[(unsigned)(Smem<NB>::JOFF + r * N7 + col)] = ws[N6 * N7 + r * N7 + col];
+      }
+  } else if (lane == 0) {
+    tt[0 + nb * (jb + n * c)] = S.t + dt0;
+    if (nb == 3) {
+      const double dx = S.x[3 * jb] - S.x[3 * ti], dy = S.x[3 * jb + 1] - S.x[3 * ti + 1];
+      const double dvx = S.v[3 * jb] - S.v[3 * ti], dvy = S.v[3 * jb + 1] - S.v[3 * ti + 1];
+      tt[1 + nb * (jb + n * c)] = __builtin_sqrt(dvx * dvx + dvy * dvy);
+      tt[2 + nb * (jb + n * c)] = dx * dx + dy * dy;
+    }
+  }
+  if (lane == 0) {
+    int64_t* ints = B.ints + D.off_int;
+    ints[NB + 1] += 1;
+    ints[NB + 2] += iter;
+  }
+  wsync();
+}
+
+template <int NB>
+__device__ void detect_transits(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int lane, double* ws) {
+  // set_state!(s_prior, s) (timing.jl:6) is taken lazily, before the first
+  // search of this step; a search leaves the state modified and the prior
+  // state is restored right after it (timing.jl:19-20), so bodies without a
+  // transit see the unmodified state either way.
+  bool saved = false;
+  const int ti = D.ti;
+  for (int i = 0; i < NB; ++i) {
+    if (i == ti) continue;
+    const double gi = gsky<NB>(S, i, ti);
+    const double ri = __builtin_sqrt((S.x[3 * i] * S.x[3 * i] + S.x[3 * i + 1] * S.x[3 * i + 1]) + S.x[3 * i + 2] * S.x[3 * i + 2]);
+    if (gi > 0.0 && S.gsave[i] < 0.0 && -S.x[3 * i + 2] > 0.25 * ri && ri < 1e12) {
+      const int64_t cnt = S.count[i] + 1;
+      wsync();
+      if (lane == 0) S.count[i] = cnt;
+      wsync();
+      if (cnt >= 1 && cnt <= D.ntt) {  // count starts at 0; the lower bound keeps tt/dq indexing in range
+        if (!saved) {
+          save_prior<NB>(S, lane);
+          saved = true;
+        }
+        const double dt0 = ((-gi) * D.hstep) / (gi - S.gsave[i]);
+        findtransit<NB>(S, D, B, ti, i, dt0, lane, ws);
+        restore_prior<NB>(S, lane);
+      }
+    }
+    wsync();
+    if (lane == 0) S.gsave[i] = gi;
+    wsync();
+  }
+}
+
+// calc_dtdelements! (timing.jl:112-138) for one system; jac_init is staged
+// transposed in the Jacobian's LDS space (the Jacobian is already stored).
+template <int NB> __device__ void dtdelements(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int lane) {
+  const auto J_ = jj<NB>(S);
+  constexpr int N7 = 7 * NB;
+  const int n = NB;
+  const auto jt = J_;  // N7*N7 <= 2*N6*N7 doubles
+  const double* jinit = B.jac + D.off_jac + 2 * (int64_t)N7 * N7;
+  for (int col = lane; col < N7; col += WAVE)
+    for (int cp = 0; cp < N7; ++cp) jt[col + N7 * cp] = jinit[cp + (int64_t)N7 * col];
+  wsync();
+  const double* dq = B.dq + D.off_dq;
+  double* del = B.del + D.off_del;
+  const int nb = D.ntbv;
+  const int64_t ntt = D.ntt;
+  for (int b = 0; b < nb; ++b)
+    for (int i = 0; i < n; ++i) {
+      const int64_t cnt = S.count[i] < ntt ? S.count[i] : ntt;
+      for (int64_t jj = 0; jj < cnt; ++jj) {
+        for (int col = lane; col < N7; col += WAVE) {
+          double acc = 0.0;
+          for (int cp = 0; cp < N7; ++cp)
+            acc = acc + dq[b + nb * ((int64_t)i + n * (jj + ntt * cp))] * jt[col + N7 * cp];
+          del[b + nb * ((int64_t)i + n * (jj + ntt * col))] = acc;
+        }
+      }
+    }
+  wsync();
+}
+
+// ---------------------------------------------------------------------------
+// Load / store of a system between its HBM record and LDS.
+// ---------------------------------------------------------------------------
+template <int NB> __device__ void load_system(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int lane,
+                                              bool loadJ = true) {
+  double* J_ = S.JJ;
+  if constexpr (Smem<NB>::JG) {
+    J_ = B.ws + D.off_ws + 2 * (int64_t)(6 * NB) * (7 * NB);  // after the transit-step copy
+    if (lane == 0) S.jg = J_;
+  }
+  constexpr int N3 = 3 * NB, N6 = 6 * NB, N7 = 7 * NB;
+  const double* st = B.state + D.off_state;
+  for (int q = lane; q < N3; q += WAVE) {
+    S.x[q] = st[off_x(NB) + q];
+    S.v[q] = st[off_v(NB) + q];
+    S.xe[q] = st[off_xe(NB) + q];
+    S.ve[q] = st[off_ve(NB) + q];
+  }
+  for (int q = lane; q < NB; q += WAVE) {
+    S.m[q] = st[off_m(NB) + q];
+    S.gsave[q] = st[off_gsave(NB) + q];
+    S.count[q] = B.ints[D.off_int + q];
+  }
+  for (int q = lane; q < N7; q += WAVE) S.dqdt[q] = st[off_dqdt(NB) + q];
+  for (int q = lane; q < NB * NB; q += WAVE) S.pair[q] = B.pair[D.off_pair + q];
+  if (lane == 0) S.t = st[off_t(NB)];
+  wsync();
+  if (lane == 0) {
+    constexpr int NPS = Smem<NB>::NP > 0 ? Smem<NB>::NP : 1;
+    // a massless pair (gm == 0) leaves jac_ij from the pair before it in
+    // place (ahl21.jl:712-713), so systems with one keep the sequential order
+    bool seq = false;
+    for (int i = 0; i < NB - 1; ++i)
+      for (int j = i + 1; j < NB; ++j)
+        if (!S.pair[i + NB * j] && NBG_GNEWT * (S.m[i] + S.m[j]) == 0.0) seq = true;
+    S.seq = seq ? 1 : 0;
+    for (int dir = 0; dir < 2; ++dir) {
+      uint8_t pi_[NPS], pj_[NPS], lv[NPS];
+      int np = 0;
+      if (dir == 0) {
+        for (int i = 0; i < NB - 1; ++i)
+          for (int j = i + 1; j < NB; ++j)
+            if (!S.pair[i + NB * j]) { pi_[np] = (uint8_t)i; pj_[np] = (uint8_t)j; ++np; }
+      } else {
+        for (int i = NB - 2; i >= 0; --i)
+          for (int j = NB - 1; j >= i + 1; --j)
+            if (!S.pair[i + NB * j]) { pi_[np] = (uint8_t)i; pj_[np] = (uint8_t)j; ++np; }
+      }
+      int last[NB];
+      for (int b = 0; b < NB; ++b) last[b] = 0;
+      int nl = 0;
+      for (int p = 0; p < np; ++p) {
+        const int a = last[pi_[p]], b = last[pj_[p]];
+        const int l = seq ? p + 1 : (a > b ? a : b) + 1;
+        lv[p] = (uint8_t)l;
+        last[pi_[p]] = l;
+        last[pj_[p]] = l;
+        nl = l > nl ? l : nl;
+      }
+      int w = 0;
+      for (int l = 1; l <= nl; ++l) {
+        int cntl = 0;
+        for (int p = 0; p < np; ++p)
+          if (lv[p] == l) {
+            S.plist[dir][2 * w] = pi_[p];
+            S.plist[dir][2 * w + 1] = pj_[p];
+            ++w;
+            ++cntl;
+          }
+        S.lvw[dir][l - 1] = (uint8_t)cntl;
+      }
+      S.nlv[dir] = nl;
+    }
+  }
+  if (D.grad && loadJ) {
+    const double* J = B.jac + D.off_jac;
+    const double* JE = J + (int64_t)N7 * N7;
+    for (int c = lane; c < N7; c += WAVE)
+      for (int b = 0; b < NB; ++b)
+        for (int k = 0; k < 6; ++k) {
+          J_[(unsigned)((6 * b + k) * N7 + c)] = J[(7 * b + k) + (int64_t)N7 * c];
+          J_[(unsigned)(Smem<NB>::JOFF + (6 * b + k) * N7 + c)] = JE[(7 * b + k) + (int64_t)N7 * c];
+        }
+  }
+  (void)N6;
+  wsync();
+}
+
+template <int NB> __device__ void store_system(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int lane) {
+  const auto J_ = jj<NB>(S);
+  constexpr int N3 = 3 * NB, N7 = 7 * NB;
+  wsync();
+  double* st = B.state + D.off_state;
+  for (int q = lane; q < N3; q += WAVE) {
+    st[off_x(NB) + q] = S.x[q];
+    st[off_v(NB) + q] = S.v[q];
+    st[off_xe(NB) + q] = S.xe[q];
+    st[off_ve(NB) + q] = S.ve[q];
+  }
+  for (int q = lane; q < NB; q += WAVE) {
+    st[off_gsave(NB) + q] = S.gsave[q];
+    B.ints[D.off_int + q] = S.count[q];
+  }
+  for (int q = lane; q < N7; q += WAVE) st[off_dqdt(NB) + q] = S.dqdt[q];
+  if (lane == 0) st[off_t(NB)] = S.t;
+  if (D.grad) {
+    double* J = B.jac + D.off_jac;
+    double* JE = J + (int64_t)N7 * N7;
+    for (int c = lane; c < N7; c += WAVE)
+      for (int b = 0; b < NB; ++b)
+        for (int k = 0; k < 6; ++k) {
+          J[(7 * b + k) + (int64_t)N7 * c] = J_[(unsigned)((6 * b + k) * N7 + c)];
+          JE[(7 * b + k) + (int64_t)N7 * c] = J_[(unsigned)(Smem<NB>::JOFF + (6 * b + k) * N7 + c)];
+        }
+  }
+  wsync();
+}
+
+// Run (or resume) one system for at most `chunk` steps.  Returns true when
+// the system has finished.
+template <int NB>
+__device__ void run_system(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int64_t chunk, int lane) {
+  load_system<NB>(S, D, B, lane);
+  int64_t* ints = B.ints + D.off_int;
+  const int64_t istep0 = ints[NB];
+  if (istep0 > D.nsteps) return;  // finished earlier
+  const int64_t iend = (chunk > 0 && istep0 + chunk < D.nsteps) ? istep0 + chunk : D.nsteps;
+  const bool anyKick = D.any_kick != 0;
+  const bool grad = D.grad != 0;
+  double* ws = B.ws + D.off_ws;
+  if (D.mode == MODE_TRANSITS && istep0 == 0) {
+    // gsave[i] = g!(i, ti, x, v) before the loop (Transits.jl:146-149)
+    if (lane == 0)
+      for (int i = 0; i < NB; ++i)
+        if (i != D.ti) S.gsave[i] = gsky<NB>(S, i, D.ti);
+    wsync();
+  }
+#pragma unroll 1
+  for (int64_t is = istep0; is < iend; ++is) {
+    const bool last = (is + 1 == D.nsteps);
+    const bool needQ = last && (D.mode == MODE_TRANSITS || D.hlast == 0.0);
+    ahl21_step<NB>(S, D.h, grad, needQ, anyKick, lane);
+    if (D.mode == MODE_TRANSITS) {
+      if (lane == 0) S.t = D.t0 + ((double)(is + 1) * D.h);
+      wsync();
+      detect_transits<NB>(S, D, B, lane, ws);
+    }
+  }
+  const bool done = (iend == D.nsteps);
+  if (done && D.mode == MODE_STEPS && D.hlast != 0.0) ahl21_step<NB>(S, D.hlast, grad, true, anyKick, lane);
+  store_system<NB>(S, D, B, lane);
+  if (lane == 0) ints[NB] = done ? D.nsteps + 1 : iend;
+  if (done && D.mode == MODE_TRANSITS && grad) dtdelements<NB>(S, D, B, lane);
+}
+
+// Run system D with the LDS layout of its body count N (a compile-time
+// chain of scalar branches on the uniform D.n).
+template <int N, int HI>
+__device__ __forceinline__ void dispatch_n(unsigned char* raw, const SysDesc& D, const SysBufs& B, int64_t chunk,
+                                           int lane) {
+  if constexpr (N <= HI) {
+    if (D.n == N) {
+      run_system<N>(*reinterpret_cast<Smem<N>*>(raw), D, B, chunk, lane);
+      return;
+    }
+    dispatch_n<N + 1, HI>(raw, D, B, chunk, lane);
+  }
+}
+
+
+// LDS of a launch that may run any body count up to HI.
+template <int N> constexpr size_t lds_bytes_upto() {
+  if constexpr (N <= 2) return sizeof(Smem<2>);
+  else return sizeof(Smem<N>) > lds_bytes_upto<N - 1>() ? sizeof(Smem<N>) : lds_bytes_upto<N - 1>();
+}
+
+// ---------------------------------------------------------------------------
+// Cooperative workgroups: two waves per system, for the longest systems of a
+// batch.  Wave 0 integrates; at each detected transit it pushes a job -- the
+// body, the Newton start dt0, the transit's index, a snapshot of the state at
+// the step's end and of J/Jerr -- into a ring in LDS (its J part into the
+// workgroup's HBM slots) and goes on.  Wave 1 runs each job's findtransit on
+// its own LDS layout, from the snapshot, exactly as the one-wave kernel runs
+// it from its saved prior state, so the results are the same bits; the
+// transit search leaves the integrating wave's critical path.  Ring protocol:
+// wave 0 owns `tail` and `done`, wave 1 owns `head`; data are published with
+// a workgroup-scope release before the counter store and read after an
+// acquire.  Every wait has a partner that always makes progress: wave 1
+// consumes jobs unconditionally, and wave 0 sets `done` after its last step.
+// Each wait is also bounded (COOP_SPIN_* polls of ~500 cycles): should one
+// ever run out, the wave records the failure in `err` (the host turns it
+// into an error) and goes on, so that every workgroup still finishes.
+// ---------------------------------------------------------------------------
+constexpr long long COOP_SPIN_SLOT = 1LL << 22;  // ~1 s: a slot frees within one search
+constexpr long long COOP_SPIN_JOB = 1LL << 29;   // ~2 min: a whole integration without transits
+template <int HI> struct TransitRing {
+  int head, tail, done, pad;
+  int jb[COOP_RQ];
+  int64_t cidx[COOP_RQ];
+  double dt0[COOP_RQ];
+  double st[COOP_RQ][19 * HI + 1];  // x, v, xe, ve (3n each), dqdt (7n), t
+};
+
+__device__ __forceinline__ int ring_load(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void ring_store(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Wave 0: hand one transit search to wave 1.
+template <int NB, int HI>
+__device__ void push_transit(Smem<NB>& S, const SysDesc& D, TransitRing<HI>& R, double* jobJ, int jb, double dt0,
+                             int64_t c, int lane, unsigned long long* err) {
+  constexpr int N3 = 3 * NB, N6 = 6 * NB, N7 = 7 * NB;
+  const int tail = R.tail;  // only this wave writes it
+  long long spins = 0;
+  while (tail - ring_load(&R.head) >= COOP_RQ) {
+    if (++spins > COOP_SPIN_SLOT) {
+      if (lane == 0) atomicOr(err, 1ull);
+      return;
+    }
+    __builtin_amdgcn_s_sleep(4);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");  // wave 1 is done with the slot
+  const int slot = tail % COOP_RQ;
+  double* st = R.st[slot];
+  for (int q = lane; q < N3; q += WAVE) {
+    st[q] = S.x[q];
+    st[N3 + q] = S.v[q];
+    st[2 * N3 + q] = S.xe[q];
+    st[3 * N3 + q] = S.ve[q];
+  }
+  for (int q = lane; q < N7; q += WAVE) st[4 * N3 + q] = S.dqdt[q];
+  if (lane == 0) {
+    st[4 * N3 + N7] = S.t;
+    R.jb[slot] = jb;
+    R.cidx[slot] = c;
+    R.dt0[slot] = dt0;
+  }
+  if (D.grad) {
+    const auto J_ = jj<NB>(S);
+    double* dst = jobJ + (int64_t)slot * 2 * N6 * N7;
+    for (int col = lane; col < N7; col += WAVE)
+      for (int r = 0; r < N6; ++r) {
+        dst[r * N7 + col] = J_[(unsigned)(r * N7 + col)];
+        dst[N6 * N7 + r * N7 + col] = J_[(unsigned)(Smem<NB>::JOFF + r * N7 + col)];
+      }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  if (lane == 0) ring_store(&R.tail, tail + 1);
+  wsync();
+}
+
+// Wave 0: the step loop of run_system (MODE_TRANSITS, whole run) with the
+// transit searches handed to wave 1 (detect_transits' tests and counting).
+// The kernel sets `done` after it returns.
+template <int NB, int HI>
+__device__ void coop_integrate(Smem<NB>& S, const SysDesc& D, const SysBufs& B, TransitRing<HI>& R, double* jobJ,
+                               int lane, unsigned long long* err) {
+  load_system<NB>(S, D, B, lane);
+  int64_t* ints = B.ints + D.off_int;
+  const int64_t istep0 = ints[NB];
+  const bool anyKick = D.any_kick != 0;
+  const bool grad = D.grad != 0;
+  if (istep0 <= D.nsteps) {
+    if (istep0 == 0) {
+      if (lane == 0)
+        for (int i = 0; i < NB; ++i)
+          if (i != D.ti) S.gsave[i] = gsky<NB>(S, i, D.ti);
+      wsync();
+    }
+#pragma unroll 1
+    for (int64_t is = istep0; is < D.nsteps; ++is) {
+      const bool last = (is + 1 == D.nsteps);
+      ahl21_step<NB>(S, D.h, grad, last, anyKick, lane);
+      if (lane == 0) S.t = D.t0 + ((double)(is + 1) * D.h);
+      wsync();
+      const int ti = D.ti;
+      for (int i = 0; i < NB; ++i) {
+        if (i == ti) continue;
+        const double gi = gsky<NB>(S, i, ti);
+        const double ri =
+            __builtin_sqrt((S.x[3 * i] * S.x[3 * i] + S.x[3 * i + 1] * S.x[3 * i + 1]) + S.x[3 * i + 2] * S.x[3 * i + 2]);
+        if (gi > 0.0 && S.gsave[i] < 0.0 && -S.x[3 * i + 2] > 0.25 * ri && ri < 1e12) {
+          const int64_t cnt = S.count[i] + 1;
+          wsync();
+          if (lane == 0) S.count[i] = cnt;
+          wsync();
+          if (cnt >= 1 && cnt <= D.ntt) {
+            const double dt0 = ((-gi) * D.hstep) / (gi - S.gsave[i]);
+            push_transit<NB, HI>(S, D, R, jobJ, i, dt0, cnt - 1, lane, err);
+          }
+        }
+        wsync();
+        if (lane == 0) S.gsave[i] = gi;
+        wsync();
+      }
+    }
+    store_system<NB>(S, D, B, lane);
+    if (lane == 0) ints[NB] = D.nsteps + 1;
+  }
+}
+
+// Wave 1: run the pushed transit searches until wave 0 is done and the ring
+// is empty.  Its layout holds the system's masses, pair mask and level
+// schedule; state and J come from each job.
+template <int NB, int HI>
+__device__ void coop_search(Smem<NB>& S, const SysDesc& D, const SysBufs& B, TransitRing<HI>& R, double* jobJ,
+                            int lane, unsigned long long* err) {
+  constexpr int N3 = 3 * NB, N6 = 6 * NB, N7 = 7 * NB;
+  load_system<NB>(S, D, B, lane, false);
+  double* ws = B.ws + D.off_ws;
+  int head = 0;
+#pragma unroll 1
+  while (true) {
+    int tail = ring_load(&R.tail);
+    long long spins = 0;
+    bool stop = false;
+    while (tail == head) {
+      if (ring_load(&R.done)) {
+        tail = ring_load(&R.tail);
+        break;
+      }
+      if (++spins > COOP_SPIN_JOB) {
+        if (lane == 0) atomicOr(err, 2ull);
+        stop = true;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(8);
+      tail = ring_load(&R.tail);
+    }
+    if (stop || tail == head) break;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const int slot = head % COOP_RQ;
+    const double* st = R.st[slot];
+    for (int q = lane; q < N3; q += WAVE) {
+      S.px[q] = st[q];
+      S.pv[q] = st[N3 + q];
+      S.pxe[q] = st[2 * N3 + q];
+      S.pve[q] = st[3 * N3 + q];
+    }
+    for (int q = lane; q < N7; q += WAVE) S.pdqdt[q] = st[4 * N3 + q];
+    const int jb = R.jb[slot];
+    const int64_t c = R.cidx[slot];
+    const double dt0 = R.dt0[slot];
+    if (lane == 0) {
+      S.pt = st[4 * N3 + N7];
+      S.count[jb] = c + 1;
+    }
+    double* src = jobJ + (int64_t)slot * 2 * N6 * N7;
+    if constexpr (Smem<NB>::JG) {
+      if (lane == 0) S.jg = src;  // the snapshot is this search's Jacobian store
+    } else if (D.grad) {
+      for (int col = lane; col < N7; col += WAVE)
+        for (int r = 0; r < N6; ++r) {
+          S.JJ[r * N7 + col] = src[r * N7 + col];
+          S.JJ[Smem<NB>::JOFF + r * N7 + col] = src[N6 * N7 + r * N7 + col];
+        }
+    }
+    wsync();
+    findtransit<NB>(S, D, B, D.ti, jb, dt0, lane, ws);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    if (lane == 0) ring_store(&R.head, head + 1);
+    head += 1;
+    wsync();
+  }
+}
+
+template <int N, int HI>
+__device__ __forceinline__ void dispatch_coop(unsigned char* raw, int wave, const SysDesc& D, const SysBufs& B,
+                                              TransitRing<HI>& R, double* jobJ, int lane, unsigned long long* err) {
+  if constexpr (N <= HI) {
+    if (D.n == N) {
+      Smem<N>& S = *reinterpret_cast<Smem<N>*>(raw);
+      if (wave == 0) coop_integrate<N, HI>(S, D, B, R, jobJ, lane, err);
+      else coop_search<N, HI>(S, D, B, R, jobJ, lane, err);
+      return;
+    }
+    dispatch_coop<N + 1, HI>(raw, wave, D, B, R, jobJ, lane, err);
+  }
+}
+
+template <int N, int HI>
+__device__ __forceinline__ void dispatch_dtdel(unsigned char* raw, const SysDesc& D, const SysBufs& B, int lane) {
+  if constexpr (N <= HI) {
+    if (D.n == N) {
+      dtdelements<N>(*reinterpret_cast<Smem<N>*>(raw), D, B, lane);
+      return;
+    }
+    dispatch_dtdel<N + 1, HI>(raw, D, B, lane);
+  }
+}
+
+// Persistent cooperative kernel: each two-wave workgroup takes systems
+// (MODE_TRANSITS, whole runs) from the queue of group 0 of its list.
+// units[12] (no class-9 system has 12 bodies) collects wait failures.
+template <int HI>
+__global__ __launch_bounds__(2 * WAVE) void nbg_run_coop(GroupList gl, SysBufs B, CoopBufs C,
+                                                         unsigned long long* __restrict__ units) {
+  __shared__ __attribute__((aligned(16))) unsigned char raw[2][lds_bytes_upto<HI>()];
+  __shared__ TransitRing<HI> R;
+  __shared__ int sys;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
+  const int lane = threadIdx.x % WAVE;
+  double* jobJ = C.jobs + (int64_t)blockIdx.x * C.stride;
+#pragma unroll 1
+  while (true) {
+    if (threadIdx.x == 0) {
+      sys = atomicAdd(gl.queue[0], 1);
+      R.head = 0;
+      R.tail = 0;
+      R.done = 0;
+    }
+    __syncthreads();
+    const int idx = __builtin_amdgcn_readfirstlane(sys);
+    if (idx >= gl.nsys[0]) break;
+    const SysDesc D = gl.descs[0][idx];
+    const int64_t* ints = B.ints + D.off_int;
+    const int64_t before = ints[D.n];
+    __syncthreads();
+    dispatch_coop<2, HI>(raw[wave], wave, D, B, R, jobJ, lane, units + 12);
+    if (wave == 0) {  // unconditionally, so that wave 1 always leaves its loop
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      if (lane == 0) ring_store(&R.done, 1);
+    }
+    __syncthreads();  // every search is written
+    if (wave == 0 && D.grad) dispatch_dtdel<2, HI>(raw[0], D, B, lane);
+    __syncthreads();
+    if (threadIdx.x == 0 && before <= D.nsteps && ints[D.n] > D.nsteps)
+      atomicAdd(&units[D.n], (unsigned long long)(D.nsteps + (D.grad ? ints[D.n + 1] : 0)));
+    __syncthreads();
+  }
+}
+
+// Persistent kernel of launch class HI: every workgroup (one wave) takes
+// systems from the queues of its group list in order -- its own class first,
+// then classes of fewer bodies, each ordered longest first -- and runs each
+// to completion (or for `chunk` steps).  LDS is sized for HI bodies; Smem<N>
+// grows with N, so every smaller layout fits in it.  `units` (13 entries,
+// indexed by body count) collects the work units this launch completed:
+// main steps plus one per recorded transit when gradients are on.
+template <int HI>
+__global__ __launch_bounds__(WAVE) void nbg_run(GroupList gl, SysBufs B, int64_t chunk,
+                                                unsigned long long* __restrict__ units) {
+  __shared__ __attribute__((aligned(16))) unsigned char raw[lds_bytes_upto<HI>()];
+  const int lane = threadIdx.x;
+  int g = 0;
+  while (g < gl.ng) {
+    int idx = 0;
+    if (lane == 0) idx = atomicAdd(gl.queue[g], 1);
+    // readfirstlane (lane 0: every lane is active here) makes the system
+    // index, its descriptor and every branch on it wave-uniform (scalar
+    // registers, scalar branches).  With a lane shuffle instead, the compiler
+    // treats them as divergent and builds exec-masked regions around the
+    // per-system code; one such region's exit once dropped a register copy
+    // carried to the next system (it ran with an empty exec mask).
+    idx = __builtin_amdgcn_readfirstlane(idx);
+    if (idx >= gl.nsys[g]) {
+      ++g;
+      continue;
+    }
+    const SysDesc D = gl.descs[g][idx];
+    const int64_t* ints = B.ints + D.off_int;
+    const int64_t before = ints[D.n];  // step cursor (lane 0 reads and writes it)
+    dispatch_n<2, HI>(raw, D, B, chunk, lane);
+    wsync();
+    if (lane == 0 && before <= D.nsteps && ints[D.n] > D.nsteps) {  // finished during this visit
+      atomicAdd(&units[D.n], (unsigned long long)(D.nsteps + (D.grad ? ints[D.n + 1] : 0)));
+      if (B.done_list) B.done_list[atomicAdd(B.done_count, 1)] = D.tag;
+    }
+  }
+}
+
+}  // namespace nbg

@@ -75,6 +75,16 @@ struct SysBufs {
   int32_t* done_list;  // during a launch are appended here
 };
 
+// Cooperative workgroups (two waves per system, batch launches only): the
+// HBM slots that carry a transit search's Jacobian snapshot from the wave
+// that integrates to the wave that searches.  Each workgroup owns `stride`
+// doubles from jobs + blockIdx.x * stride: COOP_RQ slots of 2 * 6n * 7n.
+constexpr int COOP_RQ = 4;
+struct CoopBufs {
+  double* jobs;
+  int64_t stride;
+};
+
 // state slots
 NBG_TD int64_t state_len(int n) { return 13 * (int64_t)n + 7 * n + n + 4; }
 NBG_TD int64_t off_x(int) { return 0; }

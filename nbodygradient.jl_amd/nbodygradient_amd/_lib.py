@@ -61,6 +61,7 @@ def lib():
     L.nbg_batch_destroy.argtypes = [VP]
     L.nbg_set_launch.argtypes = [VP, C.c_int32, C.c_int32]
     L.nbg_set_grid_cap.argtypes = [VP, C.c_int32]
+    L.nbg_set_coop.argtypes = [VP, C.c_int32, C.c_int32]
     L.nbg_server_create.argtypes = [VP, C.c_int64, C.c_int64, C.POINTER(VP)]
     L.nbg_server_destroy.argtypes = [VP]
     L.nbg_server_submit.argtypes = [VP, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int64, C.c_int32,
@@ -137,6 +138,11 @@ class Context:
 
     def set_grid_cap(self, grid_cap=0):
         self.check(lib().nbg_set_grid_cap(self._h, int(grid_cap)), "nbg_set_grid_cap")
+
+    def set_coop(self, mode=1, max_systems=128):
+        """Two-wave workgroups for the longest transit-search systems of batch
+        runs: 0 off, 1 automatic (default), 2 the longest max_systems."""
+        self.check(lib().nbg_set_coop(self._h, int(mode), int(max_systems)), "nbg_set_coop")
 
     def close(self):
         if getattr(self, "_h", None):
