@@ -34,9 +34,9 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per system-step of nbg_run<9> (every n <= 9 in one launch), from
 # the rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the same counters'
 # reading of a copy of known size (profiles/nbg_run_c2_x0.05_counters.md:
-# 696.1 GB read + 901.6 GB written for the 4,310,895 system-steps of the
-# C2 x0.05 batch in one dispatch)
-HBM_BYTES_PER_UNIT = (696.1e9 + 901.6e9) / 4310895
+# 586.7 GB read + 797.5 GB written for the 4,174,120 system-steps nbg_run<9>
+# carries of the C2 x0.05 batch, in one dispatch)
+HBM_BYTES_PER_UNIT = (586.7e9 + 797.5e9) / 4174120
 METRIC = "system-steps/s w/ Jacobian, batched vs plain GPU; p50/p99 request latency"
 
 

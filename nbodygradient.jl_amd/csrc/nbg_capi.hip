@@ -357,8 +357,10 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
     load[si] += L.gcost[g] / grid;
     hipStream_t st = ctx->streams[si];
     const bool timed = group_ms != nullptr;
+    int64_t work = 0;  // systems this launch can take (none left when all went to cooperative workgroups)
+    for (int e = 0; e < gl.ng; ++e) work += gl.nsys[e];
     if (timed) NBG_HIP(hipEventRecord(ctx->gev[2 * g], st));
-    NBG_HIP(nbg_launch_group(L.gn[g], gl, B, chunk, grid, S.units.p + 13 * g, st));
+    if (work > 0) NBG_HIP(nbg_launch_group(L.gn[g], gl, B, chunk, grid, S.units.p + 13 * g, st));
     if (timed) NBG_HIP(hipEventRecord(ctx->gev[2 * g + 1], st));
   }
   for (int s = 0; s < ns; ++s) {
