@@ -181,7 +181,7 @@ def main():
     launches = {}
     for cls, L in stats["launches"].items():
         flops = sum(W(n) * u for n, u in L["units"].items())
-        name = "nbg_run_coop<9>" if cls == 1 else "nbg_run<%d>" % cls  # class 1: two-wave workgroups
+        name = "nbg_run_coop<10>" if cls == 1 else "nbg_run<%d>" % cls  # class 1: two-wave workgroups
         launches[str(cls)] = {"kernel": name, "ms": L["ms"], "grid": L["grid"],
                               "units": sum(L["units"].values()),
                               "units_by_n": {str(n): u for n, u in sorted(L["units"].items())}, "flops": flops,

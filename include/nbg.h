@@ -157,10 +157,11 @@ int nbg_set_grid_cap(nbg_ctx* ctx, int32_t grid_cap);
  * a system of at most 9 bodies with a transit search can run on a workgroup
  * of two waves, the second running its transit searches (findtransit!,
  * timing.jl:30-110) from snapshots while the first integrates; results are
- * bitwise those of the one-wave kernel.  mode 0: off; 1 (default): the
- * systems whose estimated time exceeds the batch's work spread over every
- * wave slot of the GPU (the ones that would set the batch's end); 2: the
- * longest ones regardless.  At most max_systems per batch (default 128). */
+ * bitwise those of the one-wave kernel.  mode 0: off; 1 (default): in
+ * batch runs, the systems whose estimated time exceeds the batch's work
+ * spread over every wave slot of the GPU (the ones that would set the
+ * batch's end); 2: the longest ones regardless, in batch runs and serving
+ * rounds.  At most max_systems per launch (default 128); n <= 10. */
 int nbg_set_coop(nbg_ctx* ctx, int32_t mode, int32_t max_systems);
 
 /* Diagnostic only (no counterpart in the reference): a plain device copy of

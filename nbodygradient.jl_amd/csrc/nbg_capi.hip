@@ -26,9 +26,10 @@ extern "C" hipError_t nbg_launch_group(int cls, nbg::GroupList gl, nbg::SysBufs 
 extern "C" size_t nbg_lds_bytes(int nb);
 extern "C" int nbg_class_of(int n);
 extern "C" hipError_t nbg_launch_copy(const void* src, void* dst, size_t bytes, int cu_count, hipStream_t stream);
-extern "C" hipError_t nbg_launch_coop(int cls, nbg::GroupList gl, nbg::SysBufs bufs, nbg::CoopBufs coop, int grid,
+extern "C" hipError_t nbg_launch_coop(nbg::GroupList gl, nbg::SysBufs bufs, nbg::CoopBufs coop, int64_t chunk, int grid,
                                       unsigned long long* units, hipStream_t stream);
-extern "C" int64_t nbg_coop_stride(int cls);
+extern "C" int64_t nbg_coop_stride();
+extern "C" int nbg_coop_max_bodies();
 
 namespace {
 
@@ -223,8 +224,8 @@ int store_alloc(nbg_ctx* ctx, Store& S, const Layout& L, bool keep_initial) {
   NBG_HIP(S.ints.ensure(L.n_int));
   NBG_HIP(S.pair.ensure(L.n_pair));
   NBG_HIP(S.desc.ensure(L.desc.size()));
-  NBG_HIP(S.queue.ensure(L.gn.size() + 1));
-  NBG_HIP(S.units.ensure(13 * (L.gn.size() + 1)));
+  NBG_HIP(S.queue.ensure(2 * L.gn.size() + 1));
+  NBG_HIP(S.units.ensure(13 * L.gn.size() + 14));
   if (keep_initial) {
     NBG_HIP(S.state0.ensure(L.n_state));
     NBG_HIP(S.jac0.ensure(L.n_jac));
@@ -253,8 +254,8 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
     ctx->err = "too many launch classes";
     return -1;
   }
-  NBG_HIP(hipMemset(S.queue.p, 0, sizeof(int) * (G + 1)));
-  NBG_HIP(hipMemset(S.units.p, 0, sizeof(unsigned long long) * 13 * (G + 1)));
+  NBG_HIP(hipMemset(S.queue.p, 0, sizeof(int) * (2 * G + 1)));
+  NBG_HIP(hipMemset(S.units.p, 0, sizeof(unsigned long long) * (13 * G + 14)));
   const nbg::SysBufs B = bufs ? *bufs : bufs_of(S);
   const int ns = (int)ctx->streams.size();
   NBG_HIP(hipEventRecord(ctx->ev0, 0));
@@ -292,46 +293,56 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
     used += grids[g] * frac[g];
   }
   if (group_grid) *group_grid = grids;
-  // Cooperative systems (batch runs only): the longest transit-search systems
-  // of class 9, from the front of its cost-ordered group, get a workgroup of
-  // two waves (the second runs their transit searches).  Auto mode takes the
-  // systems whose estimated time exceeds the whole batch's work spread over
+  // Cooperative systems: the longest transit-search systems of the classes
+  // the two-wave kernel serves (n <= nbg_coop_max_bodies()), from the front of
+  // each cost-ordered group, get a workgroup of two waves (the second runs
+  // their transit searches).  Auto mode, for whole runs (batches), takes the
+  // systems whose estimated time exceeds the whole launch's work spread over
   // every wave slot of the GPU -- the ones that would otherwise set the end.
-  int gk = -1, K = 0;
-  if (!bufs && chunk == 0 && ctx->coop_mode > 0 && ctx->grid_cap == 0 && ctx->max_groups_per_cu == 0 &&
-      ctx->streams.size() > 1) {
-    for (size_t g = 0; g < G; ++g)
-      if (L.gn[g] == 9) gk = (int)g;
-    if (gk >= 0) {
-      double total = 0.0;
-      for (size_t g = 0; g < G; ++g) total += L.gcost[g];
-      const double floor_cost = total / (4.0 * ctx->cu_count);
-      const int cap = std::min({ctx->coop_max, ctx->cu_count / 2, L.gcnt[gk]});
-      while (K < cap) {
-        const int i = L.order[L.goff[gk] + K];
+  // Serving rounds (chunked) use it only when forced (mode 2): measured on
+  // the C4 trace, two-wave rounds raised p99 latency by 12 % (each round's
+  // end waits for the searches still in flight).
+  std::vector<int> kg(G, 0);
+  int K = 0;
+  if ((ctx->coop_mode == 2 || (ctx->coop_mode == 1 && chunk == 0)) && ctx->grid_cap == 0 &&
+      ctx->max_groups_per_cu == 0 && ctx->streams.size() > 1) {
+    double total = 0.0;
+    for (size_t g = 0; g < G; ++g) total += L.gcost[g];
+    const double floor_cost = total / (4.0 * ctx->cu_count);
+    const int cap = std::min(ctx->coop_max, ctx->cu_count / 2);
+    for (size_t g = 0; g < G; ++g) {
+      if (L.gn[g] > nbg_coop_max_bodies()) continue;
+      while (K < cap && kg[g] < L.gcnt[g]) {
+        const int i = L.order[L.goff[g] + kg[g]];
         if (L.desc[i].mode != nbg::MODE_TRANSITS) break;
         if (ctx->coop_mode == 1 && sys_cost(L, i) <= floor_cost) break;
+        ++kg[g];
         ++K;
       }
     }
   }
   const bool timed_coop = K > 0 && group_ms != nullptr;
   if (K > 0) {
-    NBG_HIP(S.coop.ensure((int64_t)K * nbg_coop_stride(9)));
+    NBG_HIP(S.coop.ensure((int64_t)K * nbg_coop_stride()));
     nbg::GroupList cg{};
-    cg.ng = 1;
-    cg.nsys[0] = K;
-    cg.descs[0] = S.desc.p + L.goff[gk];
-    cg.queue[0] = S.queue.p + G;
-    const nbg::CoopBufs cb{S.coop.p, nbg_coop_stride(9)};
+    for (size_t g = 0; g < G; ++g) {
+      if (kg[g] == 0) continue;
+      cg.nsys[cg.ng] = kg[g];
+      cg.descs[cg.ng] = S.desc.p + L.goff[g];
+      cg.queue[cg.ng] = S.queue.p + G + 1 + g;
+      cg.ng += 1;
+      grids[g] = std::max(1, grids[g] - 2 * kg[g]);  // each holds two wave slots of that launch
+    }
+    if (group_grid) *group_grid = grids;
+    const nbg::CoopBufs cb{S.coop.p, nbg_coop_stride()};
     hipStream_t st = ctx->streams[0];
     if (timed_coop) NBG_HIP(hipEventRecord(ctx->gev[2 * 15], st));
-    NBG_HIP(nbg_launch_coop(9, cg, B, cb, K, S.units.p + 13 * G, st));
+    NBG_HIP(nbg_launch_coop(cg, B, cb, chunk, K, S.units.p + 13 * G, st));
     if (timed_coop) NBG_HIP(hipEventRecord(ctx->gev[2 * 15 + 1], st));
-    // each cooperative workgroup holds two wave slots of the class-9 launch
-    grids[gk] = std::max(1, grids[gk] - 2 * K);
-    if (group_grid) *group_grid = grids;
-    std::this_thread::sleep_for(std::chrono::microseconds(2000));
+    // a batch: let the two-wave workgroups (the largest LDS blocks) be placed
+    // before the one-wave launches fill the CUs (a serving round is short and
+    // leaves most of the GPU free)
+    if (chunk == 0) std::this_thread::sleep_for(std::chrono::microseconds(2000));
   }
   // Launch order: the largest class first, alone for a moment, so that its
   // workgroups (the largest LDS blocks, the hardest to place) are resident
@@ -342,10 +353,10 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
   std::iota(gord.begin(), gord.end(), 0);
   std::vector<double> load(ns, 0.0);
   for (size_t g : gord) {
-    if (g == 1) std::this_thread::sleep_for(std::chrono::microseconds(2000));
+    if (g == 1 && chunk == 0) std::this_thread::sleep_for(std::chrono::microseconds(2000));
     nbg::GroupList gl{};
     for (size_t h = g; h < G; ++h) {  // own class, then the smaller ones
-      const int skip = ((int)h == gk) ? K : 0;  // the cooperative systems
+      const int skip = kg[h];  // the cooperative systems
       gl.nsys[gl.ng] = L.gcnt[h] - skip;
       gl.descs[gl.ng] = S.desc.p + L.goff[h] + skip;
       gl.queue[gl.ng] = S.queue.p + h;
@@ -384,7 +395,7 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
   }
   if (K > 0) {
     unsigned long long werr = 0;
-    NBG_HIP(hipMemcpy(&werr, S.units.p + 13 * G + 12, sizeof(werr), hipMemcpyDeviceToHost));
+    NBG_HIP(hipMemcpy(&werr, S.units.p + 13 * G + 13, sizeof(werr), hipMemcpyDeviceToHost));
     if (werr) {
       ctx->err = "cooperative workgroup: a wait between its two waves timed out";
       return -2;
@@ -1077,8 +1088,8 @@ int nbg_server_step(nbg_server* sv, int32_t* n_finished, double* kernel_ms) {
   group_layout(L);
   const size_t na = sv->active.size();
   NBG_HIP(sv->S.desc.ensure(na));
-  NBG_HIP(sv->S.queue.ensure(L.gn.size() + 1));
-  NBG_HIP(sv->S.units.ensure(13 * (L.gn.size() + 1)));
+  NBG_HIP(sv->S.queue.ensure(2 * L.gn.size() + 1));
+  NBG_HIP(sv->S.units.ensure(13 * L.gn.size() + 14));
   NBG_HIP(sv->done.ensure(na + 1));
   std::vector<nbg::SysDesc> sorted(na);
   for (size_t k = 0; k < na; ++k) sorted[k] = L.desc[L.order[k]];

@@ -1428,18 +1428,20 @@ __device__ void push_transit(Smem<NB>& S, const SysDesc& D, TransitRing<HI>& R, 
   wsync();
 }
 
-// Wave 0: the step loop of run_system (MODE_TRANSITS, whole run) with the
-// transit searches handed to wave 1 (detect_transits' tests and counting).
-// The kernel sets `done` after it returns.
+// Wave 0: the step loop of run_system (MODE_TRANSITS; the whole run, or at
+// most `chunk` steps of it) with the transit searches handed to wave 1
+// (detect_transits' tests and counting).  The kernel sets `done` after it
+// returns, so the searches of the visit are all finished when it ends.
 template <int NB, int HI>
 __device__ void coop_integrate(Smem<NB>& S, const SysDesc& D, const SysBufs& B, TransitRing<HI>& R, double* jobJ,
-                               int lane, unsigned long long* err) {
+                               int64_t chunk, int lane, unsigned long long* err) {
   load_system<NB>(S, D, B, lane);
   int64_t* ints = B.ints + D.off_int;
   const int64_t istep0 = ints[NB];
   const bool anyKick = D.any_kick != 0;
   const bool grad = D.grad != 0;
   if (istep0 <= D.nsteps) {
+    const int64_t iend = (chunk > 0 && istep0 + chunk < D.nsteps) ? istep0 + chunk : D.nsteps;
     if (istep0 == 0) {
       if (lane == 0)
         for (int i = 0; i < NB; ++i)
@@ -1447,7 +1449,7 @@ __device__ void coop_integrate(Smem<NB>& S, const SysDesc& D, const SysBufs& B, 
       wsync();
     }
 #pragma unroll 1
-    for (int64_t is = istep0; is < D.nsteps; ++is) {
+    for (int64_t is = istep0; is < iend; ++is) {
       const bool last = (is + 1 == D.nsteps);
       ahl21_step<NB>(S, D.h, grad, last, anyKick, lane);
       if (lane == 0) S.t = D.t0 + ((double)(is + 1) * D.h);
@@ -1474,7 +1476,7 @@ __device__ void coop_integrate(Smem<NB>& S, const SysDesc& D, const SysBufs& B, 
       }
     }
     store_system<NB>(S, D, B, lane);
-    if (lane == 0) ints[NB] = D.nsteps + 1;
+    if (lane == 0) ints[NB] = (iend == D.nsteps) ? D.nsteps + 1 : iend;
   }
 }
 
@@ -1545,15 +1547,16 @@ __device__ void coop_search(Smem<NB>& S, const SysDesc& D, const SysBufs& B, Tra
 
 template <int N, int HI>
 __device__ __forceinline__ void dispatch_coop(unsigned char* raw, int wave, const SysDesc& D, const SysBufs& B,
-                                              TransitRing<HI>& R, double* jobJ, int lane, unsigned long long* err) {
+                                              TransitRing<HI>& R, double* jobJ, int64_t chunk, int lane,
+                                              unsigned long long* err) {
   if constexpr (N <= HI) {
     if (D.n == N) {
       Smem<N>& S = *reinterpret_cast<Smem<N>*>(raw);
-      if (wave == 0) coop_integrate<N, HI>(S, D, B, R, jobJ, lane, err);
+      if (wave == 0) coop_integrate<N, HI>(S, D, B, R, jobJ, chunk, lane, err);
       else coop_search<N, HI>(S, D, B, R, jobJ, lane, err);
       return;
     }
-    dispatch_coop<N + 1, HI>(raw, wave, D, B, R, jobJ, lane, err);
+    dispatch_coop<N + 1, HI>(raw, wave, D, B, R, jobJ, chunk, lane, err);
   }
 }
 
@@ -1569,10 +1572,12 @@ __device__ __forceinline__ void dispatch_dtdel(unsigned char* raw, const SysDesc
 }
 
 // Persistent cooperative kernel: each two-wave workgroup takes systems
-// (MODE_TRANSITS, whole runs) from the queue of group 0 of its list.
-// units[12] (no class-9 system has 12 bodies) collects wait failures.
+// (MODE_TRANSITS, at most HI bodies) from the queues of its group list in
+// order and runs each to completion (or for `chunk` steps), as nbg_run does.
+// units[0..12] counts work units by body count; units[13] collects wait
+// failures.
 template <int HI>
-__global__ __launch_bounds__(2 * WAVE) void nbg_run_coop(GroupList gl, SysBufs B, CoopBufs C,
+__global__ __launch_bounds__(2 * WAVE) void nbg_run_coop(GroupList gl, SysBufs B, CoopBufs C, int64_t chunk,
                                                          unsigned long long* __restrict__ units) {
   __shared__ __attribute__((aligned(16))) unsigned char raw[2][lds_bytes_upto<HI>()];
   __shared__ TransitRing<HI> R;
@@ -1580,31 +1585,39 @@ __global__ __launch_bounds__(2 * WAVE) void nbg_run_coop(GroupList gl, SysBufs B
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
   const int lane = threadIdx.x % WAVE;
   double* jobJ = C.jobs + (int64_t)blockIdx.x * C.stride;
+  int g = 0;
 #pragma unroll 1
-  while (true) {
+  while (g < gl.ng) {
     if (threadIdx.x == 0) {
-      sys = atomicAdd(gl.queue[0], 1);
+      sys = atomicAdd(gl.queue[g], 1);
       R.head = 0;
       R.tail = 0;
       R.done = 0;
     }
     __syncthreads();
     const int idx = __builtin_amdgcn_readfirstlane(sys);
-    if (idx >= gl.nsys[0]) break;
-    const SysDesc D = gl.descs[0][idx];
+    if (idx >= gl.nsys[g]) {
+      ++g;
+      __syncthreads();  // thread 0 writes `sys` again
+      continue;
+    }
+    const SysDesc D = gl.descs[g][idx];
     const int64_t* ints = B.ints + D.off_int;
     const int64_t before = ints[D.n];
     __syncthreads();
-    dispatch_coop<2, HI>(raw[wave], wave, D, B, R, jobJ, lane, units + 12);
+    dispatch_coop<2, HI>(raw[wave], wave, D, B, R, jobJ, chunk, lane, units + 13);
     if (wave == 0) {  // unconditionally, so that wave 1 always leaves its loop
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       if (lane == 0) ring_store(&R.done, 1);
     }
     __syncthreads();  // every search is written
-    if (wave == 0 && D.grad) dispatch_dtdel<2, HI>(raw[0], D, B, lane);
+    const bool finished = before <= D.nsteps && ints[D.n] > D.nsteps;  // during this visit
+    if (wave == 0 && D.grad && finished) dispatch_dtdel<2, HI>(raw[0], D, B, lane);
     __syncthreads();
-    if (threadIdx.x == 0 && before <= D.nsteps && ints[D.n] > D.nsteps)
+    if (threadIdx.x == 0 && finished) {
       atomicAdd(&units[D.n], (unsigned long long)(D.nsteps + (D.grad ? ints[D.n + 1] : 0)));
+      if (B.done_list) B.done_list[atomicAdd(B.done_count, 1)] = D.tag;
+    }
     __syncthreads();
   }
 }

@@ -1,6 +1,6 @@
 """Two-wave (cooperative) workgroups: a system whose transit searches run on a
 second wave, from snapshots, gives the same bits as the one-wave kernel and
-as the CPU oracle -- alone, for every body count 2..9, and mixed with
+as the CPU oracle -- alone, for every body count 2..10, and mixed with
 one-wave systems in the same batch."""
 import numpy as np
 import pytest
@@ -42,8 +42,8 @@ def _same(r1, r2):
 
 
 def test_coop_every_body_count_matches_oracle():
-    print("gpu: coop n=2..9 vs oracle", flush=True)
-    specs = _specs(range(2, 10), seed=41, steps=500)
+    print("gpu: coop n=2..10 vs oracle", flush=True)
+    specs = _specs(range(2, 11), seed=41, steps=500)
     states = W.initial_states(specs, processes=1)
     st, res = _run(specs, states, 2, len(specs))
     assert st["launches"][1]["grid"] == len(specs)  # every system ran on a two-wave workgroup
@@ -81,3 +81,39 @@ def test_coop_long_runs_with_many_transits():
     for r1, r2 in zip(off, on):
         assert r1["count"].sum() > 50
         _same(r1, r2)
+
+
+def test_coop_serving_rounds_match_the_oracle():
+    """Forced two-wave workgroups in chunked serving rounds: each round ends
+    with the searches of its visit finished, and a request resumes (with its
+    step cursor, transit counts and gsave) on any workgroup in the next."""
+    print("gpu: coop serving rounds vs oracle", flush=True)
+    from nbodygradient_amd.server import Server
+    specs = _specs([2 + q % 9 for q in range(12)], seed=44, steps=300)
+    states = W.initial_states(specs, processes=1)
+    ctx = nb.default_context()
+    ctx.set_coop(2, 128)
+    try:
+        sv = Server(max_bytes=1 << 30, chunk=37)
+        ids = [sv.submit(s, st) for s, st in zip(specs, states)]
+        out = {}
+        for _ in range(10000):
+            sv.step()
+            for rid in sv.finished():
+                out[rid] = sv.fetch(rid)
+            st = sv.status()
+            if st["pending"] == 0 and st["active"] == 0:
+                break
+        sv.close()
+    finally:
+        ctx.set_coop(1, 128)
+    assert sorted(out) == sorted(ids)
+    for rid, s, (x, v, jinit) in zip(ids, specs, states):
+        os_ = O.OState(x, v, s.elements[:, 0], t=s.t0, jac_init=jinit)
+        ott, odq, odel, oc = O.integrate_transits(os_, s.h, s.tmax, s.ntt(), grad=True)
+        r = out[rid]
+        assert np.array_equal(r["count"], oc)
+        assert np.array_equal(r["tt"], ott)
+        assert np.array_equal(r["dtdq0"], odq)
+        assert np.array_equal(r["dtdelements"], odel)
+        assert np.array_equal(r["x"], os_.x)
