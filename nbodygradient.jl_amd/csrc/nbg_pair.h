@@ -322,6 +322,23 @@ NBG_HD SeriesOut gfuncs(double gamma, double beta, double sqb) {
 // on the lanes that do not, so no lane's inputs push the others off the
 // libm fast paths.
 NBG_HD void sc_pair(bool ell, double xx, double* sx, double* cx) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // Every lane's argument is on the polynomial paths of both branches (sincos
+  // without reduction, sinh's series, exp without reduction: what the general
+  // calls below would evaluate for it): one straight block, so the four
+  // polynomial chains can overlap.
+  {
+    const double ye = ell ? xx : 0.125, yh = ell ? 0.125 : xx;
+    if (__all(nbg::fabsd(ye * TWO_OVER_PI) < 0.5 && nbg::fabsd(ye) >= 0x1p-27 && nbg::fabsd(yh * LOG2E) < 0.5)) {
+      const double s1 = nbg::ksin(ye, 0.0), c1 = nbg::kcos(ye, 0.0);
+      const double s2 = nbg::sinh_poly(yh);
+      const double c2 = nbg::exp_poly(-yh) + s2;
+      *sx = ell ? s1 : s2;
+      *cx = ell ? c1 : c2;
+      return;
+    }
+  }
+#endif
   double s = 0.0, c = 0.0;
   if (NBG_ANY(ell)) {
     double s1, c1;
