@@ -163,6 +163,12 @@ int nbg_set_grid_cap(nbg_ctx* ctx, int32_t grid_cap);
  * batch's end); 2: the longest ones regardless, in batch runs and serving
  * rounds.  At most max_systems per launch (default 128); n <= 10. */
 int nbg_set_coop(nbg_ctx* ctx, int32_t mode, int32_t max_systems);
+/* Diagnostic only (no counterpart in the reference): bound every wait between
+ * the two waves of a cooperative workgroup to `polls` polls (0: the defaults,
+ * ~1 s for a free search slot and ~2 min for the next search).  A wait that
+ * runs out makes the launch return an error instead of hanging; tests use a
+ * tiny bound to check that path. */
+int nbg_set_coop_wait(nbg_ctx* ctx, int64_t polls);
 
 /* Diagnostic only (no counterpart in the reference): a plain device copy of
  * `bytes` (a multiple of 16) through a grid-stride kernel, run `reps` times;

@@ -170,6 +170,7 @@ struct nbg_ctx {
   int grid_cap = 0;
   int coop_mode = 1;   // 0 off, 1 systems longer than the batch's throughput bound, 2 the longest ones
   int coop_max = 128;  // at most this many cooperative systems per batch
+  int64_t coop_polls = 0;  // > 0: bound on every inter-wave wait (diagnostic); 0: the defaults
   int nstreams = 4;
   std::vector<hipStream_t> streams;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -334,7 +335,11 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
       grids[g] = std::max(1, grids[g] - 2 * kg[g]);  // each holds two wave slots of that launch
     }
     if (group_grid) *group_grid = grids;
-    const nbg::CoopBufs cb{S.coop.p, nbg_coop_stride()};
+    // wait bounds in polls of ~500 cycles: ~1 s for a free ring slot (one
+    // search frees it), ~2 min for the next job (a whole run may have none)
+    const int64_t slot_polls = ctx->coop_polls > 0 ? ctx->coop_polls : (int64_t)1 << 22;
+    const int64_t job_polls = ctx->coop_polls > 0 ? ctx->coop_polls : (int64_t)1 << 29;
+    const nbg::CoopBufs cb{S.coop.p, nbg_coop_stride(), slot_polls, job_polls};
     hipStream_t st = ctx->streams[0];
     if (timed_coop) NBG_HIP(hipEventRecord(ctx->gev[2 * 15], st));
     NBG_HIP(nbg_launch_coop(cg, B, cb, chunk, K, S.units.p + 13 * G, st));
@@ -596,6 +601,14 @@ int nbg_set_coop(nbg_ctx* ctx, int32_t mode, int32_t max_systems) {
   NBG_ARG(max_systems >= 0, "max_systems must be >= 0");
   ctx->coop_mode = mode;
   ctx->coop_max = max_systems;
+  return 0;
+}
+
+int nbg_set_coop_wait(nbg_ctx* ctx, int64_t polls) {
+  if (!ctx) return -1;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(polls >= 0, "polls must be >= 0");
+  ctx->coop_polls = polls;
   return 0;
 }
 

@@ -1363,12 +1363,12 @@ template <int N> constexpr size_t lds_bytes_upto() {
 // a workgroup-scope release before the counter store and read after an
 // acquire.  Every wait has a partner that always makes progress: wave 1
 // consumes jobs unconditionally, and wave 0 sets `done` after its last step.
-// Each wait is also bounded (COOP_SPIN_* polls of ~500 cycles): should one
-// ever run out, the wave records the failure in `err` (the host turns it
-// into an error) and goes on, so that every workgroup still finishes.
+// Each wait is also bounded (CoopBufs::spin_* polls of ~500 cycles; the host
+// passes ~1 s for a free slot, which frees within one search, and ~2 min for
+// a job, a whole integration without transits): should one ever run out, the
+// wave records the failure in `err` (the host turns it into an error) and
+// goes on, so that every workgroup still finishes.
 // ---------------------------------------------------------------------------
-constexpr long long COOP_SPIN_SLOT = 1LL << 22;  // ~1 s: a slot frees within one search
-constexpr long long COOP_SPIN_JOB = 1LL << 29;   // ~2 min: a whole integration without transits
 template <int HI> struct TransitRing {
   int head, tail, done, pad;
   int jb[COOP_RQ];
@@ -1387,12 +1387,12 @@ __device__ __forceinline__ void ring_store(int* p, int v) {
 // Wave 0: hand one transit search to wave 1.
 template <int NB, int HI>
 __device__ void push_transit(Smem<NB>& S, const SysDesc& D, TransitRing<HI>& R, double* jobJ, int jb, double dt0,
-                             int64_t c, int lane, unsigned long long* err) {
+                             int64_t c, int lane, unsigned long long* err, int64_t spin_slot) {
   constexpr int N3 = 3 * NB, N6 = 6 * NB, N7 = 7 * NB;
   const int tail = R.tail;  // only this wave writes it
   long long spins = 0;
   while (tail - ring_load(&R.head) >= COOP_RQ) {
-    if (++spins > COOP_SPIN_SLOT) {
+    if (++spins > spin_slot) {
       if (lane == 0) atomicOr(err, 1ull);
       return;
     }
@@ -1434,7 +1434,7 @@ __device__ void push_transit(Smem<NB>& S, const SysDesc& D, TransitRing<HI>& R, 
 // returns, so the searches of the visit are all finished when it ends.
 template <int NB, int HI>
 __device__ void coop_integrate(Smem<NB>& S, const SysDesc& D, const SysBufs& B, TransitRing<HI>& R, double* jobJ,
-                               int64_t chunk, int lane, unsigned long long* err) {
+                               int64_t chunk, int lane, unsigned long long* err, int64_t spin_slot) {
   load_system<NB>(S, D, B, lane);
   int64_t* ints = B.ints + D.off_int;
   const int64_t istep0 = ints[NB];
@@ -1467,7 +1467,7 @@ __device__ void coop_integrate(Smem<NB>& S, const SysDesc& D, const SysBufs& B, 
           wsync();
           if (cnt >= 1 && cnt <= D.ntt) {
             const double dt0 = ((-gi) * D.hstep) / (gi - S.gsave[i]);
-            push_transit<NB, HI>(S, D, R, jobJ, i, dt0, cnt - 1, lane, err);
+            push_transit<NB, HI>(S, D, R, jobJ, i, dt0, cnt - 1, lane, err, spin_slot);
           }
         }
         wsync();
@@ -1485,7 +1485,7 @@ __device__ void coop_integrate(Smem<NB>& S, const SysDesc& D, const SysBufs& B, 
 // schedule; state and J come from each job.
 template <int NB, int HI>
 __device__ void coop_search(Smem<NB>& S, const SysDesc& D, const SysBufs& B, TransitRing<HI>& R, double* jobJ,
-                            int lane, unsigned long long* err) {
+                            int lane, unsigned long long* err, int64_t spin_job) {
   constexpr int N3 = 3 * NB, N6 = 6 * NB, N7 = 7 * NB;
   load_system<NB>(S, D, B, lane, false);
   double* ws = B.ws + D.off_ws;
@@ -1500,7 +1500,7 @@ __device__ void coop_search(Smem<NB>& S, const SysDesc& D, const SysBufs& B, Tra
         tail = ring_load(&R.tail);
         break;
       }
-      if (++spins > COOP_SPIN_JOB) {
+      if (++spins > spin_job) {
         if (lane == 0) atomicOr(err, 2ull);
         stop = true;
         break;
@@ -1548,15 +1548,15 @@ __device__ void coop_search(Smem<NB>& S, const SysDesc& D, const SysBufs& B, Tra
 template <int N, int HI>
 __device__ __forceinline__ void dispatch_coop(unsigned char* raw, int wave, const SysDesc& D, const SysBufs& B,
                                               TransitRing<HI>& R, double* jobJ, int64_t chunk, int lane,
-                                              unsigned long long* err) {
+                                              unsigned long long* err, const CoopBufs& C) {
   if constexpr (N <= HI) {
     if (D.n == N) {
       Smem<N>& S = *reinterpret_cast<Smem<N>*>(raw);
-      if (wave == 0) coop_integrate<N, HI>(S, D, B, R, jobJ, chunk, lane, err);
-      else coop_search<N, HI>(S, D, B, R, jobJ, lane, err);
+      if (wave == 0) coop_integrate<N, HI>(S, D, B, R, jobJ, chunk, lane, err, C.spin_slot);
+      else coop_search<N, HI>(S, D, B, R, jobJ, lane, err, C.spin_job);
       return;
     }
-    dispatch_coop<N + 1, HI>(raw, wave, D, B, R, jobJ, chunk, lane, err);
+    dispatch_coop<N + 1, HI>(raw, wave, D, B, R, jobJ, chunk, lane, err, C);
   }
 }
 
@@ -1605,7 +1605,7 @@ __global__ __launch_bounds__(2 * WAVE) void nbg_run_coop(GroupList gl, SysBufs B
     const int64_t* ints = B.ints + D.off_int;
     const int64_t before = ints[D.n];
     __syncthreads();
-    dispatch_coop<2, HI>(raw[wave], wave, D, B, R, jobJ, chunk, lane, units + 13);
+    dispatch_coop<2, HI>(raw[wave], wave, D, B, R, jobJ, chunk, lane, units + 13, C);
     if (wave == 0) {  // unconditionally, so that wave 1 always leaves its loop
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       if (lane == 0) ring_store(&R.done, 1);

@@ -83,6 +83,8 @@ constexpr int COOP_RQ = 4;
 struct CoopBufs {
   double* jobs;
   int64_t stride;
+  int64_t spin_slot;  // polls before a wait for a free ring slot gives up
+  int64_t spin_job;   // polls before a wait for a job (or the end) gives up
 };
 
 // state slots

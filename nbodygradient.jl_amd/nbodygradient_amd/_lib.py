@@ -62,6 +62,7 @@ def lib():
     L.nbg_set_launch.argtypes = [VP, C.c_int32, C.c_int32]
     L.nbg_set_grid_cap.argtypes = [VP, C.c_int32]
     L.nbg_set_coop.argtypes = [VP, C.c_int32, C.c_int32]
+    L.nbg_set_coop_wait.argtypes = [VP, C.c_int64]
     L.nbg_server_create.argtypes = [VP, C.c_int64, C.c_int64, C.POINTER(VP)]
     L.nbg_server_destroy.argtypes = [VP]
     L.nbg_server_submit.argtypes = [VP, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_int32, C.c_int64, C.c_int32,
@@ -143,6 +144,11 @@ class Context:
         """Two-wave workgroups for the longest transit-search systems of batch
         runs: 0 off, 1 automatic (default), 2 the longest max_systems."""
         self.check(lib().nbg_set_coop(self._h, int(mode), int(max_systems)), "nbg_set_coop")
+
+    def set_coop_wait(self, polls=0):
+        """Diagnostic: bound each wait between the two waves of a cooperative
+        workgroup (0: defaults)."""
+        self.check(lib().nbg_set_coop_wait(self._h, int(polls)), "nbg_set_coop_wait")
 
     def close(self):
         if getattr(self, "_h", None):

@@ -117,3 +117,30 @@ def test_coop_serving_rounds_match_the_oracle():
         assert np.array_equal(r["dtdq0"], odq)
         assert np.array_equal(r["dtdelements"], odel)
         assert np.array_equal(r["x"], os_.x)
+
+
+def test_coop_wait_bound_gives_an_error_not_a_hang():
+    """With every inter-wave wait bounded to a single poll, the searching wave
+    gives up before the first transit and the integrating wave then finds the
+    ring full: the kernel still ends, and the run reports the failure.  The
+    next run, with the default bounds, is the usual one."""
+    print("gpu: coop wait bound", flush=True)
+    specs = _specs([9, 8], seed=45, steps=600)
+    states = W.initial_states(specs, processes=1)
+    ctx = nb.default_context()
+    ctx.set_coop(2, 2)
+    ctx.set_coop_wait(1)
+    try:
+        b = Batch(specs, states)
+        with pytest.raises(nb.NbgError) as ei:
+            b.run()
+        assert "timed out" in str(ei.value)
+        b.close()
+    finally:
+        ctx.set_coop_wait(0)
+        ctx.set_coop(1, 128)
+    _, off = _run(specs, states, 0, 0)
+    _, on = _run(specs, states, 2, 2)
+    for r1, r2 in zip(off, on):
+        assert r1["count"].sum() > 4  # enough transits to fill the ring
+        _same(r1, r2)
