@@ -72,6 +72,22 @@ int nbg_integrate_transits(nbg_ctx* ctx, int32_t n, double h, double tmax, int32
                            const double* jac_init, int64_t ntt, double* tt, double* dtdq0, double* dtdelements,
                            int64_t* count);
 
+/* (intr::Integrator)(s, o::CartesianOutput) -- src/outputs/Outputs.jl:26-49:
+ * nstep steps of h * check_step(*t, tmax) (tmax = intr.tmax sets only the
+ * direction), recording the state before each step k = 0..nstep-1:
+ *   out_t : nstep          out_x, out_v : 3 x n x nstep
+ *   out_xerror, out_verror : 3 x n x nstep (both or neither; may be NULL)
+ *   out_jac, out_jac_error : 7n x 7n x nstep (each may be NULL; need grad)
+ * Afterwards the state holds the result of nstep steps and *t = t0 + h*nstep.
+ * The reference always steps with the Jacobian; grad == 0 gives the same
+ * x and v (its test_integrator.jl:184-185) without it.  Writing the records
+ * to a JLD2 file (o.file) stays with the caller. */
+int nbg_integrate_cartesian(nbg_ctx* ctx, int32_t n, double h, double tmax, int64_t nstep, int32_t grad,
+                            double* t, double* x, double* v, double* xerror, double* verror, const double* m,
+                            const uint8_t* pair, double* jac_step, double* jac_error, double* dqdt, double* out_t,
+                            double* out_x, double* out_v, double* out_xerror, double* out_verror, double* out_jac,
+                            double* out_jac_error);
+
 /* ---- Batched form: many independent systems in one persistent launch ----
  * Systems are packed back to back: system s uses n[s] bodies and ntt[s]
  * transit slots; x/v/m/pair/jac_init/tt/dtdq0/dtdelements/count hold the

@@ -64,16 +64,17 @@ struct Store {
   DevVec<int> queue;
   DevVec<unsigned long long> units;  // work units completed, [launch][n]
   DevVec<double> coop;               // Jacobian snapshot slots of the cooperative workgroups
+  DevVec<double> traj;               // trajectory records (CartesianOutput runs)
   void release() {
     state.release(); state0.release(); jac.release(); jac0.release(); ws.release();
     tt.release(); dq.release(); del.release(); ints.release(); pair.release(); desc.release(); queue.release();
-    units.release(); coop.release();
+    units.release(); coop.release(); traj.release();
   }
 };
 
 struct Layout {
   std::vector<nbg::SysDesc> desc;  // system order
-  int64_t n_state = 0, n_jac = 0, n_ws = 0, n_tt = 0, n_dq = 0, n_int = 0, n_pair = 0;
+  int64_t n_state = 0, n_jac = 0, n_ws = 0, n_tt = 0, n_dq = 0, n_int = 0, n_pair = 0, n_traj = 0;
   // launch groups: systems sorted by (launch class, cost desc); gn holds the
   // class (the largest body count it serves), gcost an estimate of its time
   std::vector<int> order;
@@ -105,7 +106,7 @@ double sys_cost(const Layout& L, int i) {
 }
 
 void finalize_layout(Layout& L) {
-  int64_t s = 0, j = 0, w = 0, t = 0, q = 0, in = 0, pr = 0;
+  int64_t s = 0, j = 0, w = 0, t = 0, q = 0, in = 0, pr = 0, tr = 0;
   for (auto& d : L.desc) {
     const int n = d.n;
     d.off_state = s; s += nbg::state_len(n);
@@ -115,8 +116,9 @@ void finalize_layout(Layout& L) {
     d.off_dq = q; d.off_del = q; q += (int64_t)d.ntbv * n * d.ntt * 7 * n;
     d.off_int = in; in += nbg::int_len(n);
     d.off_pair = pr; pr += (int64_t)n * n;
+    d.off_traj = tr; tr += d.mode == nbg::MODE_CARTESIAN ? nbg::traj_len(n, d.nsteps, d.traj) : 0;
   }
-  L.n_state = s; L.n_jac = j; L.n_ws = w; L.n_tt = t; L.n_dq = q; L.n_int = in; L.n_pair = pr;
+  L.n_state = s; L.n_jac = j; L.n_ws = w; L.n_tt = t; L.n_dq = q; L.n_int = in; L.n_pair = pr; L.n_traj = tr;
   for (size_t k = 0; k < L.desc.size(); ++k) L.desc[k].tag = (int32_t)k;
   group_layout(L);
 }
@@ -224,6 +226,7 @@ int store_alloc(nbg_ctx* ctx, Store& S, const Layout& L, bool keep_initial) {
   NBG_HIP(S.del.ensure(L.n_dq));
   NBG_HIP(S.ints.ensure(L.n_int));
   NBG_HIP(S.pair.ensure(L.n_pair));
+  if (L.n_traj > 0) NBG_HIP(S.traj.ensure(L.n_traj));
   NBG_HIP(S.desc.ensure(L.desc.size()));
   NBG_HIP(S.queue.ensure(2 * L.gn.size() + 1));
   NBG_HIP(S.units.ensure(13 * L.gn.size() + 14));
@@ -241,6 +244,7 @@ nbg::SysBufs bufs_of(Store& S) {
   nbg::SysBufs B;
   B.state = S.state.p; B.jac = S.jac.p; B.ws = S.ws.p; B.tt = S.tt.p; B.dq = S.dq.p; B.del = S.del.p;
   B.ints = S.ints.p; B.pair = S.pair.p;
+  B.traj = S.traj.p;
   B.done_count = nullptr; B.done_list = nullptr;
   return B;
 }
@@ -466,10 +470,15 @@ int any_kick(int n, const uint8_t* pair) {
   return 0;
 }
 
+// Host destinations of a MODE_CARTESIAN run's records (null: not recorded).
+struct TrajOut {
+  double *t, *x, *v, *xe, *ve, *jac, *jace;
+};
+
 // Run one system through the single-system store.
 int run_single(nbg_ctx* ctx, nbg::SysDesc d, double* x, double* v, double* xe, double* ve, const double* m,
                const uint8_t* pair, double* J, double* JE, double* dqdt, const double* jinit, double* t,
-               double* tt, double* dtdq0, double* dtdel, int64_t* count) {
+               double* tt, double* dtdq0, double* dtdel, int64_t* count, const TrajOut* traj = nullptr) {
   Layout L;
   L.desc.push_back(d);
   finalize_layout(L);
@@ -510,6 +519,26 @@ int run_single(nbg_ctx* ctx, nbg::SysDesc d, double* x, double* v, double* xe, d
       for (int q = 0; q < s7; ++q) dqdt[q] = R.state[nbg::off_dqdt(n) + q];
   } else if (dqdt && D.mode == nbg::MODE_TRANSITS) {
     for (int q = 0; q < s7; ++q) dqdt[q] = R.state[nbg::off_dqdt(n) + q];
+  }
+  if (D.mode == nbg::MODE_CARTESIAN && traj) {
+    const int64_t ns = D.nsteps;
+    const int f = D.traj;
+    auto get = [&](double* dst, int64_t off, int64_t len) -> int {
+      if (dst) NBG_HIP(hipMemcpy(dst, S.traj.p + D.off_traj + off, (size_t)len * sizeof(double), hipMemcpyDeviceToHost));
+      return 0;
+    };
+    const int64_t n3 = 3 * (int64_t)n * ns, nj = 49 * (int64_t)n * n * ns;
+    if (int rc = get(traj->t, 0, ns)) return rc;
+    if (int rc = get(traj->x, nbg::traj_off_x(n, ns), n3)) return rc;
+    if (int rc = get(traj->v, nbg::traj_off_v(n, ns), n3)) return rc;
+    if (f & nbg::TRAJ_ERR) {
+      if (int rc = get(traj->xe, nbg::traj_off_xe(n, ns), n3)) return rc;
+      if (int rc = get(traj->ve, nbg::traj_off_ve(n, ns), n3)) return rc;
+    }
+    if (f & nbg::TRAJ_JAC)
+      if (int rc = get(traj->jac, nbg::traj_off_jac(n, ns, f), nj)) return rc;
+    if (f & nbg::TRAJ_JACERR)
+      if (int rc = get(traj->jace, nbg::traj_off_jacerr(n, ns, f), nj)) return rc;
   }
   if (D.mode == nbg::MODE_TRANSITS) {
     if (t) *t = R.state[nbg::off_t(n)];
@@ -710,6 +739,37 @@ int nbg_integrate_transits(nbg_ctx* ctx, int32_t n, double h, double tmax, int32
   for (int q = 0; q < n; ++q) count[q] = 0;
   return run_single(ctx, d, x, v, xerror, verror, m, pair, jac_step, jac_error, dqdt, jac_init, t, tt, dtdq0,
                     dtdelements, count);
+}
+
+int nbg_integrate_cartesian(nbg_ctx* ctx, int32_t n, double h, double tmax, int64_t nstep, int32_t grad,
+                            double* t, double* x, double* v, double* xerror, double* verror, const double* m,
+                            const uint8_t* pair, double* jac_step, double* jac_error, double* dqdt, double* out_t,
+                            double* out_x, double* out_v, double* out_xerror, double* out_verror, double* out_jac,
+                            double* out_jac_error) {
+  if (int rc = check_common(ctx, n, h, x, v, m)) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(t && out_t && out_x && out_v, "t, out_t, out_x and out_v are required");
+  NBG_ARG(nstep >= 0, "nstep must be >= 0");
+  NBG_ARG((out_xerror == nullptr) == (out_verror == nullptr), "out_xerror and out_verror go together");
+  NBG_ARG(grad || (!out_jac && !out_jac_error), "Jacobian records need grad != 0");
+  NBG_ARG(!grad || (jac_step && jac_error && dqdt), "grad integration needs jac_step, jac_error and dqdt");
+  NBG_ARG(!grad || mass_rows_ok(n, jac_step, jac_error), "jac_step mass rows must be unit rows, jac_error's zero");
+  // Outputs.jl:27-31: the direction from check_step(t0, intr.tmax), o.nstep steps
+  const double t0 = *t;
+  const double hs = h * check_step(t0, tmax);
+  nbg::SysDesc d{};
+  d.n = n; d.mode = nbg::MODE_CARTESIAN; d.grad = grad ? 1 : 0; d.ti = 0; d.ntbv = 1;
+  d.any_kick = any_kick(n, pair);
+  d.traj = (out_xerror ? nbg::TRAJ_ERR : 0) | (out_jac ? nbg::TRAJ_JAC : 0) | (out_jac_error ? nbg::TRAJ_JACERR : 0);
+  d.nsteps = nstep; d.ntt = 0; d.h = hs; d.hstep = h; d.hlast = 0.0; d.t0 = t0;
+  if (nstep > 0) {
+    TrajOut o{out_t, out_x, out_v, out_xerror, out_verror, out_jac, out_jac_error};
+    int rc = run_single(ctx, d, x, v, xerror, verror, m, pair, jac_step, jac_error, dqdt, nullptr, nullptr, nullptr,
+                        nullptr, nullptr, nullptr, &o);
+    if (rc) return rc;
+  }
+  *t = t0 + (hs * (double)nstep);  // s.t[1] = t0 + (h * i) after step i (Outputs.jl:42)
+  return 0;
 }
 
 // ---------------------------------------------------------------- batches
@@ -1112,6 +1172,7 @@ int nbg_server_step(nbg_server* sv, int32_t* n_finished, double* kernel_ms) {
   B.state = B.jac = B.ws = B.tt = B.dq = B.del = reinterpret_cast<double*>(base);
   B.ints = reinterpret_cast<int64_t*>(base);
   B.pair = base;
+  B.traj = nullptr;
   B.done_count = sv->done.p;
   B.done_list = sv->done.p + 1;
   double ms = 0.0;

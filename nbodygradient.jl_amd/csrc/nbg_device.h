@@ -1292,6 +1292,46 @@ template <int NB> __device__ void store_system(Smem<NB>& S, const SysDesc& D, co
   wsync();
 }
 
+// The state before step k of a MODE_CARTESIAN run into its trajectory record
+// (o.states[k+1] = deepcopy(s), Outputs.jl:37-40): t, x, v, and by flag
+// xerror/verror, jac_step, jac_error (the mass rows, which no step changes:
+// unit rows of jac_step, zero rows of jac_error).  Out of line, so that the
+// step loop's register allocation is the same as without it.
+template <int NB>
+__device__ __attribute__((noinline)) void write_snapshot(const Smem<NB>& S, const SysDesc& D, const SysBufs& B,
+                                                         int64_t k, int lane) {
+  constexpr int N3 = 3 * NB, N7 = 7 * NB;
+  const int64_t ns = D.nsteps;
+  const int f = D.traj;
+  double* base = B.traj + D.off_traj;
+  wsync();
+  if (lane == 0) base[k] = (k == 0) ? D.t0 : D.t0 + (D.h * (double)k);  // s.t[1] = t0 + (h * i)
+  for (int q = lane; q < N3; q += WAVE) {
+    base[traj_off_x(NB, ns) + N3 * k + q] = S.x[q];
+    base[traj_off_v(NB, ns) + N3 * k + q] = S.v[q];
+    if (f & TRAJ_ERR) {
+      base[traj_off_xe(NB, ns) + N3 * k + q] = S.xe[q];
+      base[traj_off_ve(NB, ns) + N3 * k + q] = S.ve[q];
+    }
+  }
+  if (f & (TRAJ_JAC | TRAJ_JACERR)) {
+    const auto J_ = jj<NB>(S);
+    const int64_t NN = (int64_t)N7 * N7;
+    for (int r = lane; r < N7; r += WAVE) {
+      const int b = r / 7, kk = r - 7 * b;
+      const unsigned row = (unsigned)((6 * b + (kk < 6 ? kk : 0)) * N7);  // mass rows read nothing
+      if (f & TRAJ_JAC) {
+        double* o = base + traj_off_jac(NB, ns, f) + NN * k + r;
+        for (int c = 0; c < N7; ++c) o[(int64_t)N7 * c] = kk == 6 ? (c == r ? 1.0 : 0.0) : J_[row + c];
+      }
+      if (f & TRAJ_JACERR) {
+        double* o = base + traj_off_jacerr(NB, ns, f) + NN * k + r;
+        for (int c = 0; c < N7; ++c) o[(int64_t)N7 * c] = kk == 6 ? 0.0 : J_[Smem<NB>::JOFF + row + c];
+      }
+    }
+  }
+}
+
 // Run (or resume) one system for at most `chunk` steps.  Returns true when
 // the system has finished.
 template <int NB>
@@ -1313,6 +1353,7 @@ __device__ void run_system(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int6
   }
 #pragma unroll 1
   for (int64_t is = istep0; is < iend; ++is) {
+    if (D.mode == MODE_CARTESIAN) write_snapshot<NB>(S, D, B, is, lane);
     const bool last = (is + 1 == D.nsteps);
     const bool needQ = last && (D.mode == MODE_TRANSITS || D.hlast == 0.0);
     ahl21_step<NB>(S, D.h, grad, needQ, anyKick, lane);

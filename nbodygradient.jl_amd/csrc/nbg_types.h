@@ -15,6 +15,7 @@
 //                   at off_del)
 //  ints   (int64) : count[n] istep ntransits_done newton_iters_total
 //  pair   (uint8) : n x n
+//  traj   (double): MODE_CARTESIAN only, the state before each step (traj_* below)
 #pragma once
 
 #include <stdint.h>
@@ -31,6 +32,7 @@ namespace nbg {
 enum SysMode : int32_t {
   MODE_STEPS = 0,     // (intr)(s, N) / (intr)(s, time): nsteps steps of h, then an optional step of hlast
   MODE_TRANSITS = 1,  // (intr)(s, tt): step loop with transit search
+  MODE_CARTESIAN = 2, // (intr)(s, o::CartesianOutput): nsteps steps of h, the state recorded before each
 };
 
 struct SysDesc {
@@ -41,14 +43,14 @@ struct SysDesc {
   int32_t ntbv;      // 1 TransitTiming, 3 TransitParameters
   int32_t any_kick;  // some pair[i,j] (i<j) is true
   int32_t tag;       // reported in SysBufs.done_list when the system finishes
-  int32_t pad;
+  int32_t traj;      // MODE_CARTESIAN: TRAJ_* flags of the optional records
   int64_t nsteps;    // full steps in the run
   int64_t ntt;
   double h;          // signed step used by the loop
   double hstep;      // intr.h (unsigned) for the first transit-time guess
   double hlast;      // MODE_STEPS: extra final step (0 = none)
   double t0;         // time at step 0 of the run (t = t0 + istep*h)
-  int64_t off_state, off_jac, off_ws, off_tt, off_dq, off_del, off_int, off_pair;
+  int64_t off_state, off_jac, off_ws, off_tt, off_dq, off_del, off_int, off_pair, off_traj;
 };
 
 // The systems one launch may take: its own launch class first, then the
@@ -71,6 +73,7 @@ struct SysBufs {
   double* del;
   int64_t* ints;
   uint8_t* pair;
+  double* traj;        // trajectory records (null when no system records one)
   int* done_count;     // optional (null): tags of the systems that finished
   int32_t* done_list;  // during a launch are appended here
 };
@@ -103,5 +106,22 @@ NBG_TD int64_t jac_len(int n) { return 3 * 49 * (int64_t)n * n; }
 constexpr int JGLOBAL_MIN = 7;
 NBG_TD int64_t ws_len(int n) { return (n >= JGLOBAL_MIN ? 4 : 2) * 42 * (int64_t)n * n; }
 NBG_TD int64_t int_len(int n) { return (int64_t)n + 3; }
+
+// Trajectory record of a MODE_CARTESIAN run of ns steps: the state before
+// step k (k = 0..ns-1) as arrays over k, each column-major with k last:
+// t[ns], x[3n x ns], v[3n x ns], then, by flag, xerror and verror
+// (TRAJ_ERR), jac_step[7n x 7n x ns] (TRAJ_JAC), jac_error (TRAJ_JACERR).
+constexpr int TRAJ_ERR = 1, TRAJ_JAC = 2, TRAJ_JACERR = 4;
+NBG_TD int64_t traj_off_x(int, int64_t ns) { return ns; }
+NBG_TD int64_t traj_off_v(int n, int64_t ns) { return ns + 3 * (int64_t)n * ns; }
+NBG_TD int64_t traj_off_xe(int n, int64_t ns) { return ns + 6 * (int64_t)n * ns; }
+NBG_TD int64_t traj_off_ve(int n, int64_t ns) { return ns + 9 * (int64_t)n * ns; }
+NBG_TD int64_t traj_off_jac(int n, int64_t ns, int f) { return ns + ((f & TRAJ_ERR) ? 12 : 6) * (int64_t)n * ns; }
+NBG_TD int64_t traj_off_jacerr(int n, int64_t ns, int f) {
+  return traj_off_jac(n, ns, f) + ((f & TRAJ_JAC) ? 49 * (int64_t)n * n * ns : 0);
+}
+NBG_TD int64_t traj_len(int n, int64_t ns, int f) {
+  return traj_off_jacerr(n, ns, f) + ((f & TRAJ_JACERR) ? 49 * (int64_t)n * n * ns : 0);
+}
 
 }  // namespace nbg

@@ -1,6 +1,7 @@
-"""State, Integrator, TransitTiming and TransitParameters: the reference's
-user-facing API (src/integrator/Integrator.jl, src/transits/Transits.jl) with
-the integration running in libnbg_hip.so on the GPU.
+"""State, Integrator, TransitTiming, TransitParameters and CartesianOutput: the
+reference's user-facing API (src/integrator/Integrator.jl, src/transits/Transits.jl,
+src/outputs/Outputs.jl) with the integration running in libnbg_hip.so on the
+GPU.
 
     s = State(ic)
     tt = TransitTiming(tmax, ic)            # ti = 0 (the star) by default
@@ -147,6 +148,56 @@ class TransitParameters(_TransitOutput):
         self._tt, self._dq, self._del = self.ttbv, self.dtbvdq0, self.dtbvdelements
 
 
+class CartesianOutput:
+    """Positions, velocities and Jacobian before every integrator step
+    (src/outputs/Outputs.jl:7-21): CartesianOutput(nbody, nstep).  The
+    records are arrays over the step index k (last axis): t[nstep],
+    x/v/xerror/verror[3, n, nstep], jac_step/jac_error[7n, 7n, nstep]
+    (jacobian=False leaves both out).  `states` gives one snapshot per step,
+    as the reference's Vector{State}.  With file=True the records are saved
+    to `filename` (numpy .npz; the reference writes JLD2)."""
+
+    def __init__(self, nbody, nstep, T=float, filename="data.npz", file=False, jacobian=True):
+        if T is not float:
+            raise TypeError("only Float64 states are integrated on the GPU")
+        n, ns = int(nbody), int(nstep)
+        if n < 1 or ns < 0:
+            raise ValueError("nbody >= 1 and nstep >= 0")
+        self.n = n
+        self.nstep = ns
+        self.filename = filename
+        self.file = bool(file)
+        self.t = np.zeros(ns)
+        self.x = np.zeros((3, n, ns), order="F")
+        self.v = np.zeros((3, n, ns), order="F")
+        self.xerror = np.zeros((3, n, ns), order="F")
+        self.verror = np.zeros((3, n, ns), order="F")
+        self.jacobian = bool(jacobian)
+        if self.jacobian:
+            self.jac_step = np.zeros((7 * n, 7 * n, ns), order="F")
+            self.jac_error = np.zeros((7 * n, 7 * n, ns), order="F")
+        else:
+            self.jac_step = self.jac_error = None
+
+    def state(self, k):
+        """Snapshot before step k (0-based): views into the records."""
+        from types import SimpleNamespace
+        J = self.jac_step[:, :, k] if self.jacobian else None
+        JE = self.jac_error[:, :, k] if self.jacobian else None
+        return SimpleNamespace(t=self.t[k], x=self.x[:, :, k], v=self.v[:, :, k], xerror=self.xerror[:, :, k],
+                               verror=self.verror[:, :, k], jac_step=J, jac_error=JE)
+
+    @property
+    def states(self):
+        return [self.state(k) for k in range(self.nstep)]
+
+    def save(self):
+        arrays = dict(t=self.t, x=self.x, v=self.v, xerror=self.xerror, verror=self.verror)
+        if self.jacobian:
+            arrays.update(jac_step=self.jac_step, jac_error=self.jac_error)
+        np.savez(self.filename, **arrays)
+
+
 class Integrator:
     """Integrator(scheme, h, t0, tmax) (Integrator.jl:17-31); callable as
     intr(s), intr(s, time), intr(s, N), intr(s, tt) with grad=True/False."""
@@ -174,6 +225,8 @@ class Integrator:
             arg, d = None, arg
         if arg is None:
             return self._to_time(s, s.t[0] + self.tmax, grad, ctx)
+        if isinstance(arg, CartesianOutput):
+            return self._cartesian(s, arg, grad, ctx)
         if isinstance(arg, _TransitOutput):
             self._transits(s, arg, grad, ctx)
             return Derivatives(float, s.n) if return_arrays else None
@@ -213,3 +266,21 @@ class Integrator:
             dptr(tt._tt), dptr(tt._dq), dptr(tt._del), i64ptr(tt.count))
         c.check(rc, "nbg_integrate_transits")
         s.t[0] = t.value
+
+    # (intr)(s, o::CartesianOutput): Outputs.jl:26-49 (the reference always
+    # steps with the Jacobian; grad=False records x and v only)
+    def _cartesian(self, s, o, grad, ctx):
+        if o.n != s.n:
+            raise ValueError("CartesianOutput was made for %d bodies, the state has %d" % (o.n, s.n))
+        c = _ctx(ctx)
+        t = C.c_double(s.t[0])
+        jac = o.jacobian and grad
+        rc = _lib.lib().nbg_integrate_cartesian(
+            c.handle, s.n, C.c_double(self.h), C.c_double(self.tmax), C.c_int64(o.nstep), int(grad), C.byref(t),
+            dptr(s.x), dptr(s.v), dptr(s.xerror), dptr(s.verror), dptr(s.m), u8ptr(s.pair_u8()),
+            dptr(s.jac_step), dptr(s.jac_error), dptr(s.dqdt), dptr(o.t), dptr(o.x), dptr(o.v), dptr(o.xerror),
+            dptr(o.verror), dptr(o.jac_step) if jac else None, dptr(o.jac_error) if jac else None)
+        c.check(rc, "nbg_integrate_cartesian")
+        s.t[0] = t.value
+        if o.file:
+            o.save()

@@ -1427,6 +1427,22 @@ template <class T> void integrate_to(State<T>& s, T hstep, T time, bool grad) {
   s.t = time;
 }
 
+// (intr)(s, o::CartesianOutput): Outputs.jl:26-49.  o.states[i] =
+// deepcopy(s) before step i (here the record callback, k = i-1), then the
+// step (always with the Jacobian in the reference; grad = false gives its
+// x and v), then s.t[1] = t0 + (h * i).  tmax only sets the direction.
+template <class T, class Rec> void cartesian_output(State<T>& s, T hstep, T tmax, long long nstep, bool grad,
+                                                    Rec&& record) {
+  const T t0 = s.t;
+  const T h = hstep * check_step(t0, tmax);
+  Derivatives<T> d(s.n);
+  for (long long i = 1; i <= nstep; ++i) {
+    record(i - 1, s);
+    if (grad) ahl21_grad(s, d, h); else ahl21_nograd(s, h);
+    s.t = t0 + (h * T((double)i));
+  }
+}
+
 // (intr)(s, N; grad): Integrator.jl:211-234 (the integrator's h is restored)
 template <class T> void integrate_steps(State<T>& s, T hstep, long long N, bool grad) {
   T s2 = T(0);

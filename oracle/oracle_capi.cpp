@@ -339,6 +339,31 @@ int orc_integrate(int mode, int n, double h, long long nsteps, double time, int 
   *t = (double)s.t;
   return 0;
 }
+// CartesianOutput run; records as arrays over the step index (last axis):
+// t[ns], x/v/xe/ve[3n x ns], jac/jace[7n x 7n x ns] (null pointers: skipped).
+int orc_cartesian(int n, double h, double tmax, long long nstep, int grad, double* t, double* x, double* v, double* xe,
+                  double* ve, const double* m, const uint8_t* pair, double* jac, double* jace, double* dqdt,
+                  double* ot, double* ox, double* ov, double* oxe, double* ove, double* ojac, double* ojace) {
+  State<double> s(n);
+  load_state(s, n, x, v, xe, ve, m, pair, jac, jace, dqdt, *t);
+  const size_t n3 = 3 * (size_t)n, nn = 49 * (size_t)n * n;
+  cartesian_output(s, h, tmax, nstep, grad != 0, [&](long long k, const State<double>& c) {
+    ot[k] = c.t;
+    for (size_t q = 0; q < n3; ++q) {
+      ox[n3 * k + q] = c.x[q];
+      ov[n3 * k + q] = c.v[q];
+      if (oxe) oxe[n3 * k + q] = c.xerror[q];
+      if (ove) ove[n3 * k + q] = c.verror[q];
+    }
+    for (size_t q = 0; q < nn; ++q) {
+      if (ojac) ojac[nn * k + q] = c.jac_step[q];
+      if (ojace) ojace[nn * k + q] = c.jac_error[q];
+    }
+  });
+  store_state(s, x, v, xe, ve, jac, jace, dqdt);
+  *t = s.t;
+  return 0;
+}
 // Binary128 no-grad integration of N steps (finite-difference pinning).
 void orc_integrate_steps_q_fd(int n, double h, long long nsteps, const double* x, const double* v, const double* m,
                               const uint8_t* pair, double dlnq, double* jac_num) {

@@ -225,6 +225,24 @@ def integrate_steps(s, h, N, grad=True):
     return s
 
 
+def cartesian_output(s, h, tmax, nstep, grad=True, jacobian=True):
+    """(intr)(s, o::CartesianOutput) (Outputs.jl:26-49) on the oracle: the
+    state before each step as arrays over the step index (last axis)."""
+    n = s.n
+    rec = dict(t=np.zeros(nstep), x=np.zeros((3, n, nstep), order="F"), v=np.zeros((3, n, nstep), order="F"),
+               xerror=np.zeros((3, n, nstep), order="F"), verror=np.zeros((3, n, nstep), order="F"))
+    if jacobian:
+        rec["jac_step"] = np.zeros((7 * n, 7 * n, nstep), order="F")
+        rec["jac_error"] = np.zeros((7 * n, 7 * n, nstep), order="F")
+    t = C.c_double(s.t)
+    lib().orc_cartesian(s.n, C.c_double(h), C.c_double(tmax), C.c_longlong(nstep), int(grad), C.byref(t),
+                        dp(s.x), dp(s.v), dp(s.xerror), dp(s.verror), dp(s.m), up(s.pair), dp(s.jac_step),
+                        dp(s.jac_error), dp(s.dqdt), dp(rec["t"]), dp(rec["x"]), dp(rec["v"]), dp(rec["xerror"]),
+                        dp(rec["verror"]), dp(rec.get("jac_step")), dp(rec.get("jac_error")))
+    s.t = t.value
+    return rec
+
+
 def integrate_to(s, h, time, grad=True):
     t = C.c_double(s.t)
     lib().orc_integrate(1, s.n, C.c_double(h), C.c_longlong(0), C.c_double(time), int(grad), C.byref(t),
