@@ -102,7 +102,8 @@ typedef struct {
   int64_t newton_iters;    /* findtransit! Newton iterations */
   double kernel_ms;        /* device time of the last run (HIP events, all launches) */
   /* Per launch, indexed by its class c (the largest body count it serves:
-   * n = 2..6 share class 6, every larger n has its own): device time (HIP
+   * n = 2..9 share class 9, every larger n has its own; index 1 is the
+   * two-wave launch of nbg_set_coop): device time (HIP
    * events on the stream it ran on) and the work units it completed per body
    * count n, launch_units[c][n] (main steps + one per recorded transit when
    * grad != 0).  A launch runs its own class first, then takes systems of
@@ -116,6 +117,22 @@ int nbg_batch_create(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double*
                      const double* t0, const int32_t* ti, const int64_t* ntt, int32_t grad, int32_t ntbv,
                      const double* x, const double* v, const double* m, const uint8_t* pair,
                      const double* jac_init, nbg_batch** out);
+
+/* nbg_batch_create with the initial conditions computed on the GPU: per system
+ * the elements (n x 7, column-major rows m, P, t0, ecosw, esinw, I, Omega, as
+ * ElementsIC holds them) and, optionally, the hierarchy matrix (n x n; NULL =
+ * fully nested [n, 1, ..., 1]) become x, v and jac_init -- init_nbody
+ * (src/ics/init_nbody.jl:13-162) with kepler_init (kepler_init.jl:66-210) and
+ * ekepler (kepler.jl:1-41), which the reference runs on the host in State(ic)
+ * (Integrator.jl:73-103).  Agreement with a host computation is to rounding
+ * (the device math library's atan2, sin and cos), not bit for bit. */
+int nbg_batch_create_elements(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* tmax,
+                              const double* t0, const int32_t* ti, const int64_t* ntt, int32_t grad, int32_t ntbv,
+                              const double* elements, const double* hierarchy, const uint8_t* pair,
+                              nbg_batch** out);
+/* The batch's initial x, v (3 x n per system) and jac_init (7n x 7n per
+ * system), concatenated in system order; any pointer may be NULL. */
+int nbg_batch_initial(nbg_batch* b, double* x, double* v, double* jac_init);
 /* Restore every system to its initial state (device-side copy). */
 int nbg_batch_reset(nbg_batch* b);
 /* Integrate all systems to completion: one persistent launch per class of

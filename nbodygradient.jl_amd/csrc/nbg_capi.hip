@@ -29,6 +29,9 @@ extern "C" hipError_t nbg_launch_copy(const void* src, void* dst, size_t bytes, 
 extern "C" hipError_t nbg_launch_coop(nbg::GroupList gl, nbg::SysBufs bufs, nbg::CoopBufs coop, int64_t chunk, int grid,
                                       unsigned long long* units, hipStream_t stream);
 extern "C" int64_t nbg_coop_stride();
+// Initial conditions kernel (nbg_ic.hip)
+extern "C" hipError_t nbg_launch_ic(const void* descs, int nsys, const double* in, double* state, double* jac,
+                                    hipStream_t stream);
 extern "C" int nbg_coop_max_bodies();
 
 namespace {
@@ -773,16 +776,16 @@ int nbg_integrate_cartesian(nbg_ctx* ctx, int32_t n, double h, double tmax, int6
 }
 
 // ---------------------------------------------------------------- batches
-int nbg_batch_create(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* tmax,
-                     const double* t0, const int32_t* ti, const int64_t* ntt, int32_t grad, int32_t ntbv,
-                     const double* x, const double* v, const double* m, const uint8_t* pair,
-                     const double* jac_init, nbg_batch** out) {
-  NBG_ARG(ctx != nullptr && out != nullptr, "null argument");
-  std::lock_guard<std::mutex> lk(ctx->mu);
+// The batch's records, uploaded (state0 / jac0) but not yet copied to the
+// working buffers (nbg_batch_reset does that).  With jac_init null, jac_init
+// starts as the identity.
+static int batch_build(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* tmax,
+                       const double* t0, const int32_t* ti, const int64_t* ntt, int32_t grad, int32_t ntbv,
+                       const double* x, const double* v, const double* m, const uint8_t* pair,
+                       const double* jac_init, nbg_batch** out) {
   NBG_ARG(nsys > 0, "nsys must be positive");
   NBG_ARG(n && h && tmax && t0 && ntt && x && v && m, "missing per-system arrays");
   NBG_ARG(ntbv == 1 || ntbv == 3, "ntbv must be 1 or 3");
-  NBG_ARG(!grad || jac_init, "grad batch needs jac_init");
   auto* b = new nbg_batch();
   b->ctx = ctx;
   b->L.desc.resize(nsys);
@@ -837,7 +840,108 @@ int nbg_batch_create(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double*
   NBG_HIP(hipMemcpy(b->S.jac0.p, hj.data(), hj.size() * sizeof(double), hipMemcpyHostToDevice));
   NBG_HIP(hipMemcpy(b->S.pair.p, hp.data(), hp.size(), hipMemcpyHostToDevice));
   *out = b;
-  // ints start at zero; the reset below also copies the initial state
+  return 0;
+}
+
+int nbg_batch_create(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* tmax,
+                     const double* t0, const int32_t* ti, const int64_t* ntt, int32_t grad, int32_t ntbv,
+                     const double* x, const double* v, const double* m, const uint8_t* pair,
+                     const double* jac_init, nbg_batch** out) {
+  NBG_ARG(ctx != nullptr && out != nullptr, "null argument");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(!grad || jac_init, "grad batch needs jac_init");
+  if (int rc = batch_build(ctx, nsys, n, h, tmax, t0, ti, ntt, grad, ntbv, x, v, m, pair, jac_init, out)) return rc;
+  // ints start at zero; the reset also copies the initial state
+  return nbg_batch_reset(*out);
+}
+
+// Fully nested hierarchy [n, 1, ..., 1] (setup_hierarchy.jl:9-145): row r
+// pairs the inner r + 1 bodies (-1) with body r + 1 (+1); the last row is
+// the centre of mass of all.
+static void nested_hierarchy(int n, double* H) {
+  for (int q = 0; q < n * n; ++q) H[q] = 0.0;
+  for (int r = 0; r < n - 1; ++r) {
+    for (int c = 0; c <= r; ++c) H[r + n * c] = -1.0;
+    H[r + n * (r + 1)] = 1.0;
+  }
+  for (int c = 0; c < n; ++c) H[(n - 1) + n * c] = -1.0;
+}
+
+int nbg_batch_create_elements(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* tmax,
+                              const double* t0, const int32_t* ti, const int64_t* ntt, int32_t grad, int32_t ntbv,
+                              const double* elements, const double* hierarchy, const uint8_t* pair,
+                              nbg_batch** out) {
+  NBG_ARG(ctx != nullptr && out != nullptr, "null argument");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(nsys > 0 && n && t0 && elements, "missing per-system arrays");
+  int64_t nb = 0;
+  for (int s = 0; s < nsys; ++s) {
+    NBG_ARG(n[s] >= NB_MIN && n[s] <= NB_MAX, "n must be in [2, 12]");
+    nb += n[s];
+  }
+  // masses from the elements' first column; x, v and jac_init come from the GPU
+  std::vector<double> m(nb), zero(3 * nb, 0.0);
+  std::vector<nbg::IcDesc> ic(nsys);
+  std::vector<double> in;
+  int64_t ob = 0, oe = 0, oh = 0;
+  for (int s = 0; s < nsys; ++s) {
+    const int k = n[s];
+    for (int q = 0; q < k; ++q) m[ob + q] = elements[oe + q];
+    for (int q = 0; q < 7 * k; ++q) {
+      const double e = elements[oe + q];
+      NBG_ARG(std::isfinite(e), "elements must be finite");
+      in.push_back(e);
+    }
+    const size_t at = in.size();
+    in.resize(at + (size_t)k * k);
+    if (hierarchy) std::copy(hierarchy + oh, hierarchy + oh + (int64_t)k * k, in.begin() + at);
+    else nested_hierarchy(k, in.data() + at);
+    {
+      // the orbits' element rows (init_nbody.jl:58-75) must exist
+      const double* H = in.data() + at;
+      int bb = 0;
+      for (int i = 0; i < k - 1; ++i) {
+        if (H[i] == 0.0) bb += 1;
+        NBG_ARG(i + 1 + bb >= 0 && i + 1 + bb < k, "hierarchy does not match the body count");
+        if (bb > 0) bb -= 2;
+        else if (bb < 0) bb = 0;
+      }
+    }
+    ic[s].n = k;
+    ic[s].t0 = t0[s];
+    ic[s].off_el = (int64_t)(at - 7 * (size_t)k);
+    ob += k;
+    oe += 7 * (int64_t)k;
+    oh += (int64_t)k * k;
+  }
+  if (int rc = batch_build(ctx, nsys, n, h, tmax, t0, ti, ntt, grad, ntbv, zero.data(), zero.data(), m.data(), pair,
+                           nullptr, out))
+    return rc;
+  nbg_batch* b = *out;
+  for (int s = 0; s < nsys; ++s) {
+    const nbg::SysDesc& d = b->L.desc[s];
+    ic[s].off_state = d.off_state;
+    ic[s].off_jinit = d.off_jac + 2 * 49LL * d.n * d.n;
+  }
+  DevVec<double> din;
+  DevVec<nbg::IcDesc> dds;
+  auto fail = [&](hipError_t e) {
+    ctx->err = std::string("initial conditions on the GPU failed: ") + hipGetErrorString(e);
+    din.release();
+    dds.release();
+    nbg_batch_destroy(b);
+    *out = nullptr;
+    return -2;
+  };
+  hipError_t e = din.ensure(in.size());
+  if (e == hipSuccess) e = dds.ensure(ic.size());
+  if (e == hipSuccess) e = hipMemcpy(din.p, in.data(), in.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dds.p, ic.data(), ic.size() * sizeof(nbg::IcDesc), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = nbg_launch_ic(dds.p, nsys, din.p, b->S.state0.p, b->S.jac0.p, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) return fail(e);
+  din.release();
+  dds.release();
   return nbg_batch_reset(b);
 }
 
@@ -919,6 +1023,28 @@ int nbg_batch_download(nbg_batch* b, double* tt, double* dtdq0, double* dtdeleme
       }
       ob += d.n;
     }
+  }
+  return 0;
+}
+
+int nbg_batch_initial(nbg_batch* b, double* x, double* v, double* jac_init) {
+  if (!b) return -1;
+  nbg_ctx* ctx = b->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  const Layout& L = b->L;
+  std::vector<double> hs((size_t)L.n_state);
+  NBG_HIP(hipMemcpy(hs.data(), b->S.state0.p, hs.size() * sizeof(double), hipMemcpyDeviceToHost));
+  int64_t ob = 0, oj = 0;
+  for (const auto& d : L.desc) {
+    for (int q = 0; q < 3 * d.n; ++q) {
+      if (x) x[3 * ob + q] = hs[d.off_state + nbg::off_x(d.n) + q];
+      if (v) v[3 * ob + q] = hs[d.off_state + nbg::off_v(d.n) + q];
+    }
+    const int64_t nn = 49LL * d.n * d.n;
+    if (jac_init)
+      NBG_HIP(hipMemcpy(jac_init + oj, b->S.jac0.p + d.off_jac + 2 * nn, nn * sizeof(double), hipMemcpyDeviceToHost));
+    ob += d.n;
+    oj += nn;
   }
   return 0;
 }

@@ -90,6 +90,16 @@ struct CoopBufs {
   int64_t spin_job;   // polls before a wait for a job (or the end) gives up
 };
 
+// Initial conditions on the GPU (one system): inputs at in + off_el are the
+// elements (n x 7, column-major: m, P, t0, ecosw, esinw, I, Omega) followed by
+// the hierarchy matrix (n x n); x and v go to the state record at off_state,
+// jac_init (7n x 7n, column-major) to the jac buffer at off_jinit.
+struct IcDesc {
+  int32_t n, pad;
+  double t0;
+  int64_t off_el, off_state, off_jinit;
+};
+
 // state slots
 NBG_TD int64_t state_len(int n) { return 13 * (int64_t)n + 7 * n + n + 4; }
 NBG_TD int64_t off_x(int) { return 0; }

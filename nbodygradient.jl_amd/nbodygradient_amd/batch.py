@@ -13,24 +13,23 @@ class Batch:
     """A set of systems resident on the GPU.
 
     specs: list of SystemSpec (elements, t0, h, tmax); states: optional list
-    of (x, v, jac_init) as returned by init_nbody (computed when omitted).
+    of (x, v, jac_init) as returned by init_nbody (computed on the host when
+    omitted, or on the GPU with device_ic=True: fully nested hierarchies).
     Each system runs (intr)(s, tt; grad) with TransitTiming (ntbv = 1) or
     TransitParameters (ntbv = 3), transits measured against body ti."""
 
-    def __init__(self, specs, states=None, grad=True, ntbv=1, ti=0, pair=None, ctx=None):
+    def __init__(self, specs, states=None, grad=True, ntbv=1, ti=0, pair=None, ctx=None, device_ic=False):
         self.ctx = ctx if ctx is not None else _lib.default_context()
         self.specs = list(specs)
+        if device_ic:
+            if states is not None:
+                raise ValueError("device_ic computes the states; pass one or the other")
+            self._create_elements(grad, ntbv, ti, pair)
+            return
         if states is None:
             states = initial_states(self.specs)
         ns = len(self.specs)
-        self.n = np.array([s.n for s in self.specs], dtype=np.int32)
-        self.h = np.array([s.h for s in self.specs], dtype=np.float64)
-        self.tmax = np.array([s.tmax for s in self.specs], dtype=np.float64)
-        self.t0 = np.array([s.t0 for s in self.specs], dtype=np.float64)
-        self.ti = np.full(ns, ti, dtype=np.int32)
-        self.ntt = np.array([s.ntt() for s in self.specs], dtype=np.int64)
-        self.grad = bool(grad)
-        self.ntbv = int(ntbv)
+        self._arrays(grad, ntbv, ti)
         x = np.concatenate([np.asarray(st[0], order="F").ravel(order="F") for st in states])
         v = np.concatenate([np.asarray(st[1], order="F").ravel(order="F") for st in states])
         m = np.concatenate([s.elements[:, 0] for s in self.specs])
@@ -45,6 +44,54 @@ class Batch:
                                          dptr(x), dptr(v), dptr(m), u8ptr(pr), dptr(jinit), C.byref(h))
         self.ctx.check(rc, "nbg_batch_create")
         self._h = h
+        self._offsets()
+
+    def _arrays(self, grad, ntbv, ti):
+        ns = len(self.specs)
+        self.n = np.array([s.n for s in self.specs], dtype=np.int32)
+        self.h = np.array([s.h for s in self.specs], dtype=np.float64)
+        self.tmax = np.array([s.tmax for s in self.specs], dtype=np.float64)
+        self.t0 = np.array([s.t0 for s in self.specs], dtype=np.float64)
+        self.ti = np.full(ns, ti, dtype=np.int32)
+        self.ntt = np.array([s.ntt() for s in self.specs], dtype=np.int64)
+        self.grad = bool(grad)
+        self.ntbv = int(ntbv)
+
+    def _create_elements(self, grad, ntbv, ti, pair):
+        """nbg_batch_create_elements: init_nbody on the GPU."""
+        self._arrays(grad, ntbv, ti)
+        el = np.concatenate([np.asfortranarray(s.elements[:, :7], dtype=np.float64).ravel(order="F")
+                             for s in self.specs])
+        pr = None
+        if pair is not None:
+            pr = np.concatenate([np.asarray(p, dtype=np.uint8).ravel(order="F") for p in pair])
+        self._el, self._pair = el, pr
+        h = C.c_void_p()
+        rc = _lib.lib().nbg_batch_create_elements(self.ctx.handle, len(self.specs), i32ptr(self.n), dptr(self.h),
+                                                  dptr(self.tmax), dptr(self.t0), i32ptr(self.ti), i64ptr(self.ntt),
+                                                  int(grad), self.ntbv, dptr(el), None, u8ptr(pr), C.byref(h))
+        self.ctx.check(rc, "nbg_batch_create_elements")
+        self._h = h
+        self._offsets()
+
+    def initial(self):
+        """The batch's initial (x, v, jac_init) per system."""
+        nbod = self.n.astype(np.int64)
+        x = np.zeros(3 * int(nbod.sum()))
+        v = np.zeros(3 * int(nbod.sum()))
+        J = np.zeros(int((49 * nbod ** 2).sum()))
+        self.ctx.check(_lib.lib().nbg_batch_initial(self._h, dptr(x), dptr(v), dptr(J)), "nbg_batch_initial")
+        out, ob, oj = [], 0, 0
+        for n in nbod:
+            n = int(n)
+            out.append((x[3 * ob:3 * (ob + n)].reshape((3, n), order="F"),
+                        v[3 * ob:3 * (ob + n)].reshape((3, n), order="F"),
+                        J[oj:oj + 49 * n * n].reshape((7 * n, 7 * n), order="F")))
+            ob += n
+            oj += 49 * n * n
+        return out
+
+    def _offsets(self):
         nb = self.ntbv
         self._off_tt = np.concatenate([[0], np.cumsum(nb * self.n.astype(np.int64) * self.ntt)])
         self._off_dq = np.concatenate([[0], np.cumsum(nb * self.n.astype(np.int64) ** 2 * 7 * self.ntt)])
