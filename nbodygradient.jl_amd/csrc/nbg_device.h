@@ -750,9 +750,22 @@ __device__ __attribute__((always_inline)) inline void sweep(Smem<NB>& S, int dir
         const int q = q0 + lane;
         const int i = pl[2 * (c0 + q)], j = pl[2 * (c0 + q) + 1];
         PairSlot& sl = S.u.slot[q];
-        for (int k = 0; k < 3; ++k) {
-          sl.x0[k] = S.x[3 * i + k] - S.x[3 * j + k];
-          sl.v0[k] = S.v[3 * i + k] - S.v[3 * j + k];
+        // every LDS operand is read into registers before anything is
+        // written back: the compiler cannot tell the slot from the state
+        // arrays, so interleaved reads and writes would each wait for the
+        // previous write (a dozen LDS round trips in a row per level)
+        {
+          double xi[3], xj[3], vi[3], vj[3];
+          for (int k = 0; k < 3; ++k) {
+            xi[k] = S.x[3 * i + k];
+            xj[k] = S.x[3 * j + k];
+            vi[k] = S.v[3 * i + k];
+            vj[k] = S.v[3 * j + k];
+          }
+          for (int k = 0; k < 3; ++k) {
+            sl.x0[k] = xi[k] - xj[k];
+            sl.v0[k] = vi[k] - vj[k];
+          }
         }
         const double m_i = S.m[i], m_j = S.m[j];
         const double gm = NBG_GNEWT * (m_i + m_j);
@@ -761,13 +774,35 @@ __device__ __attribute__((always_inline)) inline void sweep(Smem<NB>& S, int dir
           const double mijinv = 1.0 / (m_i + m_j);
           const double mi = m_i * mijinv;
           const double mj = m_j * mijinv;
+          double d[6], xi[3], xj[3], vi[3], vj[3], exi[3], exj[3], evi[3], evj[3];
+          for (int k = 0; k < 6; ++k) d[k] = sl.delxv[k];
           for (int k = 0; k < 3; ++k) {
-            kahan(S.x[3 * i + k], S.xe[3 * i + k], mj * sl.delxv[k]);
-            kahan(S.x[3 * j + k], S.xe[3 * j + k], (-mi) * sl.delxv[k]);
+            xi[k] = S.x[3 * i + k];
+            exi[k] = S.xe[3 * i + k];
+            xj[k] = S.x[3 * j + k];
+            exj[k] = S.xe[3 * j + k];
+            vi[k] = S.v[3 * i + k];
+            evi[k] = S.ve[3 * i + k];
+            vj[k] = S.v[3 * j + k];
+            evj[k] = S.ve[3 * j + k];
           }
           for (int k = 0; k < 3; ++k) {
-            kahan(S.v[3 * i + k], S.ve[3 * i + k], mj * sl.delxv[3 + k]);
-            kahan(S.v[3 * j + k], S.ve[3 * j + k], (-mi) * sl.delxv[3 + k]);
+            kahan(xi[k], exi[k], mj * d[k]);
+            kahan(xj[k], exj[k], (-mi) * d[k]);
+          }
+          for (int k = 0; k < 3; ++k) {
+            kahan(vi[k], evi[k], mj * d[3 + k]);
+            kahan(vj[k], evj[k], (-mi) * d[3 + k]);
+          }
+          for (int k = 0; k < 3; ++k) {
+            S.x[3 * i + k] = xi[k];
+            S.xe[3 * i + k] = exi[k];
+            S.x[3 * j + k] = xj[k];
+            S.xe[3 * j + k] = exj[k];
+            S.v[3 * i + k] = vi[k];
+            S.ve[3 * i + k] = evi[k];
+            S.v[3 * j + k] = vj[k];
+            S.ve[3 * j + k] = evj[k];
           }
           sl.mi = mi; sl.mj = mj; sl.mijinv = mijinv; sl.m_i = m_i; sl.m_j = m_j;
         }
@@ -887,9 +922,13 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
       zero_add_rows<NB>(S, c, true, !anyKick);
       if (anyKick) vel_product_add<NB>(S, S.u.f.dadq, c, P);
     }
-  // ---- drift of the state (drift_grad! / drift!)
-  if (lane == 0)
-    for (int q = 0; q < N3; ++q) kahan(S.x[q], S.xe[q], h2 * S.v[q]);
+  // ---- drift of the state (drift_grad! / drift!): entries are independent
+  for (int q = lane; q < N3; q += WAVE) {
+    double xq = S.x[q], eq = S.xe[q];
+    kahan(xq, eq, h2 * S.v[q]);
+    S.x[q] = xq;
+    S.xe[q] = eq;
+  }
   wsync();
   if (needQ) {
     for (int q = lane; q < N3; q += WAVE) {
@@ -938,8 +977,12 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
   // ---- drift
   if (needJ)
     for (int c = lane; c < N7; c += WAVE) drift_cols<NB>(S, c, h2);
-  if (lane == 0)
-    for (int q = 0; q < N3; ++q) kahan(S.x[q], S.xe[q], h2 * S.v[q]);
+  for (int q = lane; q < N3; q += WAVE) {
+    double xq = S.x[q], eq = S.xe[q];
+    kahan(xq, eq, h2 * S.v[q]);
+    S.x[q] = xq;
+    S.xe[q] = eq;
+  }
   wsync();
   if (needQ) {
     for (int q = lane; q < N3; q += WAVE) {
