@@ -181,13 +181,23 @@ static __device__ __attribute__((noinline)) void pair_jacobian(PairSlot* sl, boo
   for (int q = 0; q < 48; ++q) kep[q] = 0.0;
   for (int q = 0; q < 6; ++q) jm[q] = 0.0;
   const KParams P = sl->P;
+  const double mi = sl->mi, mj = sl->mj, mijinv = sl->mijinv, m_i = sl->m_i, m_j = sl->m_j;
+  double dl[6];
+  for (int q = 0; q < 6; ++q) dl[q] = sl->delxv[q];
   compute_jacobian_gamma(P, x0, v0, kep, jm, drift_first);
+  // jac_ij's mass columns 6 and 13 (the k == 6, 13 cases of jij_coef),
+  // formed from registers: reading them back from the slot after the
+  // stores below would cost an LDS round trip each
+  double mc[24];
+  for (int rk = 0; rk < 6; ++rk) {
+    mc[rk] = jm[rk] * m_j;
+    mc[6 + rk] = (((-mj) * dl[rk]) * mijinv) - ((NBG_GNEWT * mi) * kep[rk + 36]);
+    mc[12 + rk] = ((mi * dl[rk]) * mijinv) + ((NBG_GNEWT * mj) * kep[rk + 36]);
+    mc[18 + rk] = (-jm[rk]) * m_i;
+  }
   for (int q = 0; q < 48; ++q) sl->kep[q] = kep[q];
   for (int q = 0; q < 6; ++q) sl->jm[q] = jm[q];
-  for (int q = 0; q < 12; ++q) {
-    sl->mc[q] = jij_coef(*sl, q < 6, q < 6 ? q : q - 6, 6);
-    sl->mc[12 + q] = jij_coef(*sl, q < 6, q < 6 ? q : q - 6, 13);
-  }
+  for (int q = 0; q < 24; ++q) sl->mc[q] = mc[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -456,7 +466,7 @@ template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick,
     for (int c = 0; c < NB; ++c) {
       if (c == b) continue;
       const int i = c < b ? c : b, j = c < b ? b : c;
-      if (S.pair[i + NB * j]) continue;
+      if (anyKick && S.pair[i + NB * j]) continue;
       const double* o = pq + pidx(i, j) * PQ;
       const double fac2 = o[0];
       const double rmy = S.x[3 * i + ry] - S.x[3 * j + ry];
@@ -532,7 +542,7 @@ template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick,
     for (int c = 0; c < NB; ++c) {
       if (c == b) continue;
       const int i = c < b ? c : b, j = c < b ? b : c;
-      if (S.pair[i + NB * j]) continue;
+      if (anyKick && S.pair[i + NB * j]) continue;
       const double* o = pq + pidx(i, j) * PQ;
       double rij[3], Di[3], Dj[3];
       for (int k = 0; k < 3; ++k) {
