@@ -125,6 +125,9 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (read MIB, write MIB) through an elementwise "
                          "kernel: the reference reading for the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--autotune", type=float, default=0.05, metavar="FRAC",
+                    help="before the timed steps, pick the launch configuration with the autotuner on a copy of "
+                         "the batch with step counts x FRAC (0: off; only with --coop auto)")
     ap.add_argument("--coop", default="auto", metavar="auto|off|K",
                     help="two-wave workgroups for the longest systems: automatic selection (default), off, or the "
                          "K longest systems")
@@ -158,6 +161,14 @@ def main():
     pb.close()
 
     batch = Batch(specs, states, grad=True)
+    tune = None
+    if args.autotune > 0 and args.coop == "auto":
+        t = time.perf_counter()
+        tune = batch.autotune(args.autotune)
+        c = tune["candidates"][tune["best"]]
+        log("autotune (x%g copy, %.1f s): %s -> coop mode %d max %d, groups/CU %d" %
+            (args.autotune, time.perf_counter() - t, ", ".join("%.0f" % q["ms"] for q in tune["candidates"]),
+             c["coop_mode"], c["coop_max"], c["groups_per_cu"]))
     for w in range(args.warmup):
         batch.reset()
         batch.run()
@@ -214,7 +225,8 @@ def main():
                                "Jacobian + transit times + dt/delements" %
                                (len(specs), "" if args.tmax_scale == 1.0 else " x%g" % args.tmax_scale),
                    "systems": len(specs), "system_steps": sys_steps, "transits": stats["transits"],
-                   "newton_iters": stats["newton_iters"], "parallelism": "1 GPU, one persistent launch per body-count class (n <= 9 in one)"},
+                   "newton_iters": stats["newton_iters"], "parallelism": "1 GPU, one persistent launch per body-count class (n <= 9 in one)",
+                   "autotune": tune},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": HBM_BYTES_PER_UNIT * launches[dom]["units"],
@@ -229,6 +241,8 @@ def main():
     }
     if args.serve > 0:
         from nbodygradient_amd.server import Server, serve
+        ctx.set_coop(1, 128)  # serving rounds keep the defaults, whatever the batch's autotuner chose
+        ctx.set_launch(0)
         trace = Wk.c4(count=args.serve)
         arrivals = np.array([t for t, _ in trace])
         sspecs = [sp for _, sp in trace]

@@ -133,6 +133,20 @@ int nbg_batch_create_elements(nbg_ctx* ctx, int32_t nsys, const int32_t* n, cons
 /* The batch's initial x, v (3 x n per system) and jac_init (7n x 7n per
  * system), concatenated in system order; any pointer may be NULL. */
 int nbg_batch_initial(nbg_batch* b, double* x, double* v, double* jac_init);
+
+/* Autotuner (no counterpart in the reference): runs a copy of batch b with
+ * every system's step count scaled by frac (0 < frac <= 1) under each
+ * candidate launch configuration -- the workgroup form (two-wave workgroups
+ * for the longest systems: off, automatic, or the 16/32/64 longest) and the
+ * one-wave workgroups per CU -- and keeps the fastest in b's context (the
+ * values nbg_set_coop and nbg_set_launch set).  cand_ms[c] receives candidate
+ * c's device time for c < max_cand (may be NULL), *best the chosen index.
+ * The copy needs as much device memory as b.  The batch itself is not run. */
+int nbg_batch_autotune(nbg_batch* b, double frac, int32_t* best, double* cand_ms, int32_t max_cand);
+/* Candidate `index` of the autotuner (its nbg_set_coop mode and maximum,
+ * and its workgroups per CU, 0 = as LDS allows); returns the number of
+ * candidates. */
+int nbg_batch_autotune_candidate(int32_t index, int32_t* coop_mode, int32_t* coop_max, int32_t* groups_per_cu);
 /* Restore every system to its initial state (device-side copy). */
 int nbg_batch_reset(nbg_batch* b);
 /* Integrate all systems to completion: one persistent launch per class of

@@ -997,6 +997,81 @@ int nbg_batch_run(nbg_batch* b, nbg_batch_stats* stats) {
   return 0;
 }
 
+// Autotuner: each candidate launch configuration runs a copy of the batch
+// with every step count scaled by `frac`; the fastest one is kept in the
+// context.  The copy has the batch's systems, body counts and transit slots,
+// so its launch classes, LPT order and two-wave selection scale with it.
+namespace {
+struct TuneCand {
+  int coop_mode, coop_max, groups_per_cu;
+};
+const TuneCand kTune[] = {
+    {1, 128, 0},  // two-wave workgroups for the systems beyond the batch's throughput bound
+    {0, 0, 0},    // one wave per system throughout
+    {2, 16, 0},   // two-wave workgroups for the 16, 32, 64 longest systems
+    {2, 32, 0},
+    {2, 64, 0},
+    {1, 128, 3},  // three one-wave workgroups per CU instead of four
+};
+}  // namespace
+
+int nbg_batch_autotune(nbg_batch* b, double frac, int32_t* best, double* cand_ms, int32_t max_cand) {
+  if (!b) return -1;
+  nbg_ctx* ctx = b->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(frac > 0.0 && frac <= 1.0, "frac must be in (0, 1]");
+  Layout Ls = b->L;  // offsets depend on n and ntt only: the copy keeps them
+  for (auto& d : Ls.desc) d.nsteps = std::max<int64_t>(1, (int64_t)std::llround((double)d.nsteps * frac));
+  Ls.cost.clear();
+  group_layout(Ls);
+  Store T;
+  auto done = [&](int rc) {
+    T.release();
+    return rc;
+  };
+  if (int rc = store_alloc(ctx, T, Ls, false)) return done(rc);
+  const int nc = (int)(sizeof(kTune) / sizeof(kTune[0]));
+  const int keep_mode = ctx->coop_mode, keep_max = ctx->coop_max, keep_g = ctx->max_groups_per_cu;
+  int bi = 0;
+  double bms = 0.0;
+  for (int c = 0; c < nc; ++c) {
+    hipError_t e = hipMemcpy(T.state.p, b->S.state0.p, Ls.n_state * sizeof(double), hipMemcpyDeviceToDevice);
+    if (e == hipSuccess) e = hipMemcpy(T.jac.p, b->S.jac0.p, Ls.n_jac * sizeof(double), hipMemcpyDeviceToDevice);
+    if (e == hipSuccess) e = hipMemcpy(T.pair.p, b->S.pair.p, std::max<int64_t>(Ls.n_pair, 1), hipMemcpyDeviceToDevice);
+    if (e == hipSuccess) e = hipMemset(T.ints.p, 0, Ls.n_int * sizeof(int64_t));
+    if (e != hipSuccess) {
+      ctx->err = std::string("autotune: ") + hipGetErrorString(e);
+      ctx->coop_mode = keep_mode; ctx->coop_max = keep_max; ctx->max_groups_per_cu = keep_g;
+      return done(-2);
+    }
+    ctx->coop_mode = kTune[c].coop_mode;
+    ctx->coop_max = kTune[c].coop_max;
+    ctx->max_groups_per_cu = kTune[c].groups_per_cu;
+    double ms = 0.0;
+    const int rc = run_groups(ctx, T, Ls, 0, &ms);
+    if (rc) {
+      ctx->coop_mode = keep_mode; ctx->coop_max = keep_max; ctx->max_groups_per_cu = keep_g;
+      return done(rc);
+    }
+    if (cand_ms && c < max_cand) cand_ms[c] = ms;
+    if (c == 0 || ms < bms) { bms = ms; bi = c; }
+  }
+  ctx->coop_mode = kTune[bi].coop_mode;
+  ctx->coop_max = kTune[bi].coop_max;
+  ctx->max_groups_per_cu = kTune[bi].groups_per_cu;
+  if (best) *best = bi;
+  return done(0);
+}
+
+int nbg_batch_autotune_candidate(int32_t index, int32_t* coop_mode, int32_t* coop_max, int32_t* groups_per_cu) {
+  const int nc = (int)(sizeof(kTune) / sizeof(kTune[0]));
+  if (index < 0 || index >= nc) return nc;
+  if (coop_mode) *coop_mode = kTune[index].coop_mode;
+  if (coop_max) *coop_max = kTune[index].coop_max;
+  if (groups_per_cu) *groups_per_cu = kTune[index].groups_per_cu;
+  return nc;
+}
+
 int nbg_batch_download(nbg_batch* b, double* tt, double* dtdq0, double* dtdelements, int64_t* count, double* x,
                        double* v) {
   if (!b) return -1;

@@ -60,6 +60,8 @@ def lib():
     L.nbg_batch_create_elements.argtypes = [VP, C.c_int32, I32, D, D, D, I32, I64, C.c_int32, C.c_int32, D, D, U8,
                                             C.POINTER(VP)]
     L.nbg_batch_initial.argtypes = [VP, D, D, D]
+    L.nbg_batch_autotune.argtypes = [VP, C.c_double, I32, D, C.c_int32]
+    L.nbg_batch_autotune_candidate.argtypes = [C.c_int32, I32, I32, I32]
     L.nbg_batch_reset.argtypes = [VP]
     L.nbg_batch_run.argtypes = [VP, C.POINTER(BatchStats)]
     L.nbg_batch_download.argtypes = [VP, D, D, D, I64, D, D]

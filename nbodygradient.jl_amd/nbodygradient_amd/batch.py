@@ -91,6 +91,25 @@ class Batch:
             oj += 49 * n * n
         return out
 
+    def autotune(self, frac=0.05):
+        """nbg_batch_autotune: time a copy of this batch with every step count
+        scaled by frac under each candidate launch configuration (two-wave
+        workgroups off / automatic / for the 16, 32, 64 longest systems;
+        four or three one-wave workgroups per CU) and keep the fastest in the
+        context.  Returns the candidates with their times and the choice."""
+        L = _lib.lib()
+        nc = L.nbg_batch_autotune_candidate(-1, None, None, None)
+        ms = np.zeros(nc)
+        best = C.c_int32(-1)
+        self.ctx.check(L.nbg_batch_autotune(self._h, C.c_double(frac), C.byref(best), dptr(ms), nc),
+                       "nbg_batch_autotune")
+        cands = []
+        for k in range(nc):
+            a, b_, g = C.c_int32(), C.c_int32(), C.c_int32()
+            L.nbg_batch_autotune_candidate(k, C.byref(a), C.byref(b_), C.byref(g))
+            cands.append({"coop_mode": a.value, "coop_max": b_.value, "groups_per_cu": g.value, "ms": float(ms[k])})
+        return {"frac": frac, "best": best.value, "candidates": cands}
+
     def _offsets(self):
         nb = self.ntbv
         self._off_tt = np.concatenate([[0], np.cumsum(nb * self.n.astype(np.int64) * self.ntt)])
