@@ -150,8 +150,9 @@ int nbg_batch_autotune_candidate(int32_t index, int32_t* coop_mode, int32_t* coo
 /* Restore every system to its initial state (device-side copy). */
 int nbg_batch_reset(nbg_batch* b);
 /* Integrate all systems to completion: one persistent launch per class of
- * body counts (2..6, then 7, 8, ... alone), systems longest first within a
- * launch, launches overlapped on streams. */
+ * body counts (2..9 together, then 10, 11, 12 alone), systems longest first
+ * within a launch, launches overlapped on streams; the longest systems may
+ * run on two-wave workgroups in a launch of their own (nbg_set_coop). */
 int nbg_batch_run(nbg_batch* b, nbg_batch_stats* stats);
 /* Copy results to host (any pointer may be NULL). */
 int nbg_batch_download(nbg_batch* b, double* tt, double* dtdq0, double* dtdelements, int64_t* count, double* x,
