@@ -63,3 +63,23 @@ def test_server_footprint_matches_the_record_layout():
         b = 8 * ((21 * n + 4) + 3 * 49 * n * n + ws + ntbv * n * ntt + 2 * ntbv * n * ntt * 7 * n + (n + 3))
         b += (n * n + 7) // 8 * 8
         assert L.nbg_server_footprint(n, ntbv, ntt) == (b + 255) // 256 * 256
+
+
+def test_autotune_candidates_start_with_the_default():
+    """The autotuner's candidate table (host code, no GPU needed): its first
+    entry is the context's default (automatic two-wave selection of at most
+    128 systems, workgroups per CU as LDS allows), and it also tries one wave
+    per system throughout."""
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libnbg_hip.so not built")
+    L = _lib.lib()
+    nc = L.nbg_batch_autotune_candidate(-1, None, None, None)
+    assert nc >= 2
+    cands = []
+    for k in range(nc):
+        a, b, g = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        assert L.nbg_batch_autotune_candidate(k, ctypes.byref(a), ctypes.byref(b), ctypes.byref(g)) == nc
+        cands.append((a.value, b.value, g.value))
+    assert cands[0] == (1, 128, 0)
+    assert any(c[0] == 0 for c in cands)
+    assert len(set(cands)) == nc
