@@ -14,10 +14,15 @@ Reported beside it:
                systems, one system per thread on all host cores;
   roofline     algorithmic FP64 flops W(n) = 2016 n^2 (n-1) + 126 n^3 per
                system-step (the structure-aware Jacobian-chain work), over the
-               batch's device time, against the FP64 peak (78.6 TFLOP/s).
+               batch's device time, against the FP64 peak (78.6 TFLOP/s);
+  serving      the C4 request trace through the serving loop (p50/p99);
+  fit_batch    the C3 fit-style batch (16384 perturbed 5-body systems, tmax
+               scaled), initial conditions computed on the GPU from the
+               elements, beside the host's init_nbody for the same systems.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--systems N]
-                       [--tmax-scale S] [--no-plain] [--no-cpu]
+                       [--tmax-scale S] [--no-plain] [--no-cpu] [--serve R]
+                       [--fit SYSTEMS] [--autotune FRAC] [--coop auto|off|K]
 """
 import argparse
 import json
@@ -68,6 +73,38 @@ def plain_latency(specs, states, per_n=2, steps_cap=1500):
         lat[n] = tot_t / max(tot_steps, 1)
         log("plain n=%d: %.1f us per system-step" % (n, 1e6 * lat[n]))
     return lat
+
+
+def fit_batch(count, tmax_scale):
+    """The C3 fit-style evaluation: `count` Gaussian perturbations of one
+    5-body system, every transit time with its dt/delements, the initial
+    conditions (x, v, jac_init) computed on the GPU from the elements."""
+    from nbodygradient_amd import workloads as Wk
+    from nbodygradient_amd.batch import Batch
+    specs = Wk.c3(count)
+    for s in specs:
+        s.tmax *= tmax_scale
+    t = time.perf_counter()
+    b = Batch(specs, device_ic=True)
+    t_ic = time.perf_counter() - t
+    st = b.run()
+    t_run = st["kernel_ms"] / 1e3
+    res = b.download(derivatives=False)
+    b.close()
+    t = time.perf_counter()
+    Wk.initial_states(specs[:1024])
+    t_host = (time.perf_counter() - t) * len(specs) / 1024
+    out = {"workload": "C3: %d perturbations of the 5-body base system (P = 5, 8.1, 13.2, 21.6, 35.0 d), h = 0.1 d, "
+                       "tmax 3000 d x%g, Jacobian + transit times + dt/delements" % (len(specs), tmax_scale),
+           "system_steps": st["system_steps"], "transits": st["transits"], "device_s": t_run,
+           "system_steps_per_s": st["system_steps"] / t_run, "transits_per_s": st["transits"] / t_run,
+           "ic_on_gpu_s": t_ic, "ic_host_s_estimate": t_host,
+           "ic_host_basis": "init_nbody in host processes for 1024 systems, scaled to the batch",
+           "max_count": int(max(int(r["count"].max()) for r in res))}
+    log("fit batch: %d systems, %d system-steps in %.2f s (%.3g/s), %d transits; IC on the GPU %.2f s "
+        "(host estimate %.1f s)" % (len(specs), st["system_steps"], t_run, out["system_steps_per_s"], st["transits"],
+                                    t_ic, t_host))
+    return out
 
 
 def cpu_baseline(specs, states, sample, threads, cap_steps=2000):
@@ -125,6 +162,9 @@ def main():
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (read MIB, write MIB) through an elementwise "
                          "kernel: the reference reading for the FETCH_SIZE / WRITE_SIZE counters")
+    ap.add_argument("--fit", type=int, default=16384, metavar="SYSTEMS",
+                    help="C3 fit-style batch leg with initial conditions on the GPU (0: skip)")
+    ap.add_argument("--fit-tmax-scale", type=float, default=0.1)
     ap.add_argument("--autotune", type=float, default=0.05, metavar="FRAC",
                     help="before the timed steps, pick the launch configuration with the autotuner on a copy of "
                          "the batch with step counts x FRAC (0: off; only with --coop auto)")
@@ -265,6 +305,8 @@ def main():
         log("serving: %d requests, p50 %.3f s, p99 %.3f s, %.1f requests/s sustained, %.3g main steps/s" %
             (len(sspecs), out["serving"]["p50_s"], out["serving"]["p99_s"], out["serving"]["sustained_requests_per_s"],
              out["serving"]["main_steps_per_s"]))
+    if args.fit > 0:
+        out["fit_batch"] = fit_batch(args.fit, args.fit_tmax_scale)
     if not args.no_plain:
         lat = plain_latency(specs, states)
         t_plain = 0.0
