@@ -148,6 +148,27 @@ class TransitParameters(_TransitOutput):
         self._tt, self._dq, self._del = self.ttbv, self.dtbvdq0, self.dtbvdelements
 
 
+class TransitSnapshot:
+    """Impact parameter squared and sky velocity of every body relative to
+    body 1 at a list of times, with their derivatives (Transits.jl:112-125):
+    vsky, bsky2 [n, nt]; dbvdq0, dbvdelements [2, n, nt, 7, n] (index 0 the
+    sky velocity, 1 the impact parameter squared).  The reference's
+    constructor builds an undefined TransitSeries (Transits.jl:124); its
+    integrator method (Transits.jl:186-211) is what this follows."""
+
+    def __init__(self, times, ic):
+        if not isinstance(ic, ElementsIC):
+            raise TypeError("TransitSnapshot needs an ElementsIC (Transits.jl:121)")
+        self.times = np.asarray(times, dtype=np.float64).copy()
+        self.nt = len(self.times)
+        n = ic.nbody
+        self.n = n
+        self.vsky = np.zeros((n, self.nt), order="F")
+        self.bsky2 = np.zeros((n, self.nt), order="F")
+        self.dbvdq0 = np.zeros((2, n, self.nt, 7, n), order="F")
+        self.dbvdelements = np.zeros((2, n, self.nt, 7, n), order="F")
+
+
 class CartesianOutput:
     """Positions, velocities and Jacobian before every integrator step
     (src/outputs/Outputs.jl:7-21): CartesianOutput(nbody, nstep).  The
@@ -227,6 +248,8 @@ class Integrator:
             return self._to_time(s, s.t[0] + self.tmax, grad, ctx)
         if isinstance(arg, CartesianOutput):
             return self._cartesian(s, arg, grad, ctx)
+        if isinstance(arg, TransitSnapshot):
+            return self._snapshot(s, arg, grad, ctx)
         if isinstance(arg, _TransitOutput):
             self._transits(s, arg, grad, ctx)
             return Derivatives(float, s.n) if return_arrays else None
@@ -284,3 +307,49 @@ class Integrator:
         s.t[0] = t.value
         if o.file:
             o.save()
+
+    # (intr)(s, ts::TransitSnapshot; grad): Transits.jl:186-211.  Each time is
+    # reached with (intr)(s, time) on the GPU; b^2, v_sky and their
+    # derivatives follow calc_bv / calc_dbvdq! (snapshot.jl:7-41) with the
+    # reference's operation order, element by element, and the element
+    # gradients calc_dbvdelements! (snapshot.jl:44-54), which the reference
+    # runs with or without grad.
+    def _snapshot(self, s, ts, grad, ctx):
+        n = s.n
+        if ts.n != n:
+            raise ValueError("TransitSnapshot was made for %d bodies, the state has %d" % (ts.n, n))
+        ts.vsky[:] = 0.0
+        ts.bsky2[:] = 0.0
+        ts.dbvdq0[:] = 0.0
+        for j, tj in enumerate(ts.times):
+            self._to_time(s, float(tj), grad, ctx)
+            x, v, q = s.x, s.v, s.dqdt
+            for b in range(1, n):
+                dvx, dvy = v[0, b] - v[0, 0], v[1, b] - v[1, 0]
+                dx, dy = x[0, b] - x[0, 0], x[1, b] - x[1, 0]
+                vs = math.sqrt(dvx * dvx + dvy * dvy)
+                ts.vsky[b, j] = vs
+                ts.bsky2[b, j] = dx * dx + dy * dy
+                if not grad:
+                    continue
+                gsky = dx * dvx + dy * dvy
+                gdot = ((dx * (q[7 * b + 3] - q[3]) + dy * (q[7 * b + 4] - q[4])) + dvx * (q[7 * b] - q[0])) + \
+                    dvy * (q[7 * b + 1] - q[1])
+                dvdt = (dvx * (q[7 * b + 3] - q[3]) + dvy * (q[7 * b + 4] - q[4])) / vs
+                dbdt = 2.0 * gsky
+                J = s.jac_step
+                Jx, Jy = J[7 * b] - J[0], J[7 * b + 1] - J[1]
+                Jvx, Jvy = J[7 * b + 3] - J[3], J[7 * b + 4] - J[4]
+                dtdq = -(((Jx * dvx + Jy * dvy) + Jvx * dx) + Jvy * dy) / gdot
+                d0 = (Jvx * dvx + Jvy * dvy) / vs + dvdt * dtdq
+                d1 = 2.0 * (Jx * dx + Jy * dy) + dbdt * dtdq
+                ts.dbvdq0[0, b, j] = d0.reshape((7, n), order="F")
+                ts.dbvdq0[1, b, j] = d1.reshape((7, n), order="F")
+        # dbvdelements[a, i, j, l, k] = sum over p, then q, of
+        # dbvdq0[a, i, j, q, p] * jac_init[7p + q, 7k + l], accumulated in that order
+        Jr = s.jac_init.reshape((7, n, 7, n), order="F")  # Jr[q, p, l, k]
+        acc = np.zeros((2, n, ts.nt, 7, n))
+        for p in range(n):
+            for qq in range(7):
+                acc += ts.dbvdq0[:, :, :, qq, p][:, :, :, None, None] * Jr[qq, p][None, None, None, :, :]
+        ts.dbvdelements[:] = acc

@@ -1443,6 +1443,69 @@ template <class T, class Rec> void cartesian_output(State<T>& s, T hstep, T tmax
   }
 }
 
+// (intr)(s, ts::TransitSnapshot; grad): Transits.jl:186-211 with calc_bv,
+// calc_dbvdq! and calc_dbvdelements! (snapshot.jl:7-54).  The reference's
+// constructor is broken (Transits.jl:124 builds an undefined TransitSeries);
+// the integrator method is followed as written: integrate to each time in
+// turn, then b^2 and v_sky of every body i >= 2 relative to body 1 and, with
+// grad, their derivatives as calc_dbvdq! forms them (including its
+// transit-time terms dvdt*dtdq and dbdt*dtdq).  Outputs (column-major):
+// vsky, bsky2 n x nt; dbvdq0, dbvdel 2 x n x nt x 7 x n.  calc_dbvdelements!
+// runs unconditionally, as in the reference.
+template <class T>
+void transit_snapshot(State<T>& s, T hstep, const T* times, int nt, bool grad, T* vsky, T* bsky2, T* dbvdq0,
+                      T* dbvdel) {
+  const int n = s.n, n7 = 7 * n;
+  const size_t nq = (size_t)2 * n * nt * 7 * n;
+  std::fill(dbvdq0, dbvdq0 + nq, T(0));
+  auto DQ = [&](int ibv, int i, int j, int k, int p) -> T& {
+    return dbvdq0[ibv + 2 * (i + (size_t)n * (j + (size_t)nt * (k + 7 * (size_t)p)))];
+  };
+  for (int j = 0; j < nt; ++j) {
+    integrate_to(s, hstep, times[j], grad);
+    const int a = 0;
+    for (int b = 1; b < n; ++b) {
+      const T dvx = s.V(0, b) - s.V(0, a), dvy = s.V(1, b) - s.V(1, a);
+      const T dx = s.X(0, b) - s.X(0, a), dy = s.X(1, b) - s.X(1, a);
+      const T vs = M<T>::sqrt(dvx * dvx + dvy * dvy);
+      const T bs = dx * dx + dy * dy;
+      vsky[b + (size_t)n * j] = vs;
+      bsky2[b + (size_t)n * j] = bs;
+      if (!grad) continue;
+      const T* q = s.dqdt.data();
+      const T gsky = dx * dvx + dy * dvy;
+      const T gdot = ((dx * (q[7 * b + 3] - q[7 * a + 3]) + dy * (q[7 * b + 4] - q[7 * a + 4])) +
+                      dvx * (q[7 * b] - q[7 * a])) +
+                     dvy * (q[7 * b + 1] - q[7 * a + 1]);
+      const T dvdt = (dvx * (q[7 * b + 3] - q[7 * a + 3]) + dvy * (q[7 * b + 4] - q[7 * a + 4])) / vs;
+      const T dbdt = T(2) * gsky;
+      for (int p = 0; p < n; ++p)
+        for (int k = 0; k < 7; ++k) {
+          const size_t col = (size_t)n7 * (7 * p + k);
+          auto J = [&](int r) { return s.jac_step[r + col]; };
+          const T dtdq = -((((J(7 * b) - J(7 * a)) * dvx + (J(7 * b + 1) - J(7 * a + 1)) * dvy) +
+                            (J(7 * b + 3) - J(7 * a + 3)) * dx) +
+                           (J(7 * b + 4) - J(7 * a + 4)) * dy) /
+                         gdot;
+          DQ(0, b, j, k, p) = ((J(7 * b + 3) - J(7 * a + 3)) * dvx + (J(7 * b + 4) - J(7 * a + 4)) * dvy) / vs +
+                              dvdt * dtdq;
+          DQ(1, b, j, k, p) = T(2) * ((J(7 * b) - J(7 * a)) * dx + (J(7 * b + 1) - J(7 * a + 1)) * dy) + dbdt * dtdq;
+        }
+    }
+  }
+  for (int ibv = 0; ibv < 2; ++ibv)
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < nt; ++j)
+        for (int k = 0; k < n; ++k)
+          for (int l = 0; l < 7; ++l) {
+            T acc = T(0);
+            for (int p = 0; p < n; ++p)
+              for (int qq = 0; qq < 7; ++qq)
+                acc += DQ(ibv, i, j, qq, p) * s.jac_init[(7 * p + qq) + (size_t)n7 * (7 * k + l)];
+            dbvdel[ibv + 2 * (i + (size_t)n * (j + (size_t)nt * (l + 7 * (size_t)k)))] = acc;
+          }
+}
+
 // (intr)(s, N; grad): Integrator.jl:211-234 (the integrator's h is restored)
 template <class T> void integrate_steps(State<T>& s, T hstep, long long N, bool grad) {
   T s2 = T(0);

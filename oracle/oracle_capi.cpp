@@ -364,6 +364,18 @@ int orc_cartesian(int n, double h, double tmax, long long nstep, int grad, doubl
   *t = s.t;
   return 0;
 }
+// TransitSnapshot run (see transit_snapshot).
+int orc_snapshot(int n, double h, int nt, const double* times, int grad, double* t, double* x, double* v, double* xe,
+                 double* ve, const double* m, const uint8_t* pair, double* jac, double* jace, double* dqdt,
+                 const double* jac_init, double* vsky, double* bsky2, double* dbvdq0, double* dbvdel) {
+  State<double> s(n);
+  load_state(s, n, x, v, xe, ve, m, pair, jac, jace, dqdt, *t);
+  for (size_t q = 0; q < (size_t)49 * n * n; ++q) s.jac_init[q] = jac_init[q];
+  transit_snapshot(s, h, times, nt, grad != 0, vsky, bsky2, dbvdq0, dbvdel);
+  store_state(s, x, v, xe, ve, jac, jace, dqdt);
+  *t = s.t;
+  return 0;
+}
 // Binary128 no-grad integration of N steps (finite-difference pinning).
 void orc_integrate_steps_q_fd(int n, double h, long long nsteps, const double* x, const double* v, const double* m,
                               const uint8_t* pair, double dlnq, double* jac_num) {

@@ -243,6 +243,24 @@ def cartesian_output(s, h, tmax, nstep, grad=True, jacobian=True):
     return rec
 
 
+def transit_snapshot(s, h, times, grad=True):
+    """(intr)(s, ts::TransitSnapshot) (Transits.jl:186-211) on the oracle:
+    vsky, bsky2 (n x nt), dbvdq0 and dbvdelements (2 x n x nt x 7 x n)."""
+    n = s.n
+    times = np.ascontiguousarray(times, dtype=np.float64)
+    nt = len(times)
+    vsky = np.zeros((n, nt), order="F")
+    bsky2 = np.zeros((n, nt), order="F")
+    dq = np.zeros((2, n, nt, 7, n), order="F")
+    de = np.zeros((2, n, nt, 7, n), order="F")
+    t = C.c_double(s.t)
+    lib().orc_snapshot(n, C.c_double(h), nt, times.ctypes.data_as(D), int(grad), C.byref(t), dp(s.x), dp(s.v),
+                       dp(s.xerror), dp(s.verror), dp(s.m), up(s.pair), dp(s.jac_step), dp(s.jac_error), dp(s.dqdt),
+                       dp(s.jac_init), dp(vsky), dp(bsky2), dp(dq), dp(de))
+    s.t = t.value
+    return vsky, bsky2, dq, de
+
+
 def integrate_to(s, h, time, grad=True):
     t = C.c_double(s.t)
     lib().orc_integrate(1, s.n, C.c_double(h), C.c_longlong(0), C.c_double(time), int(grad), C.byref(t),
