@@ -88,6 +88,22 @@ int nbg_integrate_cartesian(nbg_ctx* ctx, int32_t n, double h, double tmax, int6
                             double* out_x, double* out_v, double* out_xerror, double* out_verror, double* out_jac,
                             double* out_jac_error);
 
+/* (intr::Integrator)(s, ts::TransitSnapshot; grad) -- Transits.jl:186-211
+ * with snapshot.jl:7-54 (the reference's constructor, Transits.jl:124, is
+ * broken; its integrator method is what this follows): integrate from *t to
+ * each of the nt times in turn as nbg_integrate_to does, and there record
+ * v_sky and b^2 of every body relative to body 1 (vsky, bsky2: n x nt) and,
+ * with grad, their derivatives as calc_dbvdq! forms them (dbvdq0:
+ * 2 x n x nt x 7 x n; index 0 v_sky, 1 b^2) and the element gradients
+ * calc_dbvdelements! forms from them with jac_init (dbvdelements, same
+ * shape; zero without grad).  Body 1's entries are zero.  Afterwards the
+ * state is at the last time, *t = times[nt-1]. */
+int nbg_integrate_snapshot(nbg_ctx* ctx, int32_t n, double h, int64_t nt, const double* times, int32_t grad,
+                           double* t, double* x, double* v, double* xerror, double* verror, const double* m,
+                           const uint8_t* pair, double* jac_step, double* jac_error, double* dqdt,
+                           const double* jac_init, double* vsky, double* bsky2, double* dbvdq0,
+                           double* dbvdelements);
+
 /* ---- Batched form: many independent systems in one persistent launch ----
  * Systems are packed back to back: system s uses n[s] bodies and ntt[s]
  * transit slots; x/v/m/pair/jac_init/tt/dtdq0/dtdelements/count hold the

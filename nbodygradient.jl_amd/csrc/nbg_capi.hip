@@ -119,7 +119,10 @@ void finalize_layout(Layout& L) {
     d.off_dq = q; d.off_del = q; q += (int64_t)d.ntbv * n * d.ntt * 7 * n;
     d.off_int = in; in += nbg::int_len(n);
     d.off_pair = pr; pr += (int64_t)n * n;
-    d.off_traj = tr; tr += d.mode == nbg::MODE_CARTESIAN ? nbg::traj_len(n, d.nsteps, d.traj) : 0;
+    d.off_traj = tr;
+    tr += d.mode == nbg::MODE_CARTESIAN ? nbg::traj_len(n, d.nsteps, d.traj)
+          : d.mode == nbg::MODE_SNAPSHOT ? nbg::snap_len(n, d.nsteps)
+                                         : 0;
   }
   L.n_state = s; L.n_jac = j; L.n_ws = w; L.n_tt = t; L.n_dq = q; L.n_int = in; L.n_pair = pr; L.n_traj = tr;
   for (size_t k = 0; k < L.desc.size(); ++k) L.desc[k].tag = (int32_t)k;
@@ -476,6 +479,9 @@ int any_kick(int n, const uint8_t* pair) {
 // Host destinations of a MODE_CARTESIAN run's records (null: not recorded).
 struct TrajOut {
   double *t, *x, *v, *xe, *ve, *jac, *jace;
+  // MODE_SNAPSHOT: the times (input) and the b / v_sky outputs
+  const double* times;
+  double *vsky, *bsky2, *dbvdq0, *dbvdel;
 };
 
 // Run one system through the single-system store.
@@ -495,6 +501,12 @@ int run_single(nbg_ctx* ctx, nbg::SysDesc d, double* x, double* v, double* xe, d
   NBG_HIP(hipMemcpy(S.ints.p, R.ints.data(), R.ints.size() * sizeof(int64_t), hipMemcpyHostToDevice));
   NBG_HIP(hipMemcpy(S.pair.p, R.pair.data(), R.pair.size(), hipMemcpyHostToDevice));
   const size_t ntt_len = (size_t)L.n_tt, ndq = (size_t)L.n_dq;
+  if (D.mode == nbg::MODE_SNAPSHOT) {
+    // the times in, every output zero (body 1's entries, and every
+    // derivative without grad, stay so)
+    NBG_HIP(hipMemset(S.traj.p + D.off_traj, 0, nbg::snap_len(D.n, D.nsteps) * sizeof(double)));
+    NBG_HIP(hipMemcpy(S.traj.p + D.off_traj, traj->times, D.nsteps * sizeof(double), hipMemcpyHostToDevice));
+  }
   if (D.mode == nbg::MODE_TRANSITS) {
     // outputs start from the caller's arrays (entries past count stay as given)
     NBG_HIP(hipMemcpy(S.tt.p, tt, ntt_len * sizeof(double), hipMemcpyHostToDevice));
@@ -522,6 +534,19 @@ int run_single(nbg_ctx* ctx, nbg::SysDesc d, double* x, double* v, double* xe, d
       for (int q = 0; q < s7; ++q) dqdt[q] = R.state[nbg::off_dqdt(n) + q];
   } else if (dqdt && D.mode == nbg::MODE_TRANSITS) {
     for (int q = 0; q < s7; ++q) dqdt[q] = R.state[nbg::off_dqdt(n) + q];
+  }
+  if (D.mode == nbg::MODE_SNAPSHOT) {
+    const int64_t nt = D.nsteps;
+    const double* base = S.traj.p + D.off_traj;
+    NBG_HIP(hipMemcpy(traj->vsky, base + nbg::snap_off_vsky(n, nt), (size_t)n * nt * sizeof(double),
+                      hipMemcpyDeviceToHost));
+    NBG_HIP(hipMemcpy(traj->bsky2, base + nbg::snap_off_bsky2(n, nt), (size_t)n * nt * sizeof(double),
+                      hipMemcpyDeviceToHost));
+    const size_t nq = (size_t)14 * n * n * nt;
+    if (traj->dbvdq0)
+      NBG_HIP(hipMemcpy(traj->dbvdq0, base + nbg::snap_off_dq(n, nt), nq * sizeof(double), hipMemcpyDeviceToHost));
+    if (traj->dbvdel)
+      NBG_HIP(hipMemcpy(traj->dbvdel, base + nbg::snap_off_del(n, nt), nq * sizeof(double), hipMemcpyDeviceToHost));
   }
   if (D.mode == nbg::MODE_CARTESIAN && traj) {
     const int64_t ns = D.nsteps;
@@ -766,7 +791,8 @@ int nbg_integrate_cartesian(nbg_ctx* ctx, int32_t n, double h, double tmax, int6
   d.traj = (out_xerror ? nbg::TRAJ_ERR : 0) | (out_jac ? nbg::TRAJ_JAC : 0) | (out_jac_error ? nbg::TRAJ_JACERR : 0);
   d.nsteps = nstep; d.ntt = 0; d.h = hs; d.hstep = h; d.hlast = 0.0; d.t0 = t0;
   if (nstep > 0) {
-    TrajOut o{out_t, out_x, out_v, out_xerror, out_verror, out_jac, out_jac_error};
+    TrajOut o{out_t, out_x, out_v, out_xerror, out_verror, out_jac, out_jac_error, nullptr, nullptr, nullptr, nullptr,
+              nullptr};
     int rc = run_single(ctx, d, x, v, xerror, verror, m, pair, jac_step, jac_error, dqdt, nullptr, nullptr, nullptr,
                         nullptr, nullptr, nullptr, &o);
     if (rc) return rc;
@@ -776,6 +802,31 @@ int nbg_integrate_cartesian(nbg_ctx* ctx, int32_t n, double h, double tmax, int6
 }
 
 // ---------------------------------------------------------------- batches
+int nbg_integrate_snapshot(nbg_ctx* ctx, int32_t n, double h, int64_t nt, const double* times, int32_t grad,
+                           double* t, double* x, double* v, double* xerror, double* verror, const double* m,
+                           const uint8_t* pair, double* jac_step, double* jac_error, double* dqdt,
+                           const double* jac_init, double* vsky, double* bsky2, double* dbvdq0,
+                           double* dbvdelements) {
+  if (int rc = check_common(ctx, n, h, x, v, m)) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(t && vsky && bsky2 && jac_init, "t, vsky, bsky2 and jac_init are required");
+  NBG_ARG(nt >= 0 && (nt == 0 || times), "nt must be >= 0, with the times");
+  NBG_ARG(!grad || (jac_step && jac_error && dqdt), "grad run needs jac_step, jac_error and dqdt");
+  NBG_ARG(!grad || mass_rows_ok(n, jac_step, jac_error), "jac_step mass rows must be unit rows, jac_error's zero");
+  for (int64_t j = 0; j < nt; ++j) NBG_ARG(std::isfinite(times[j]), "times must be finite");
+  if (nt == 0) return 0;
+  nbg::SysDesc d{};
+  d.n = n; d.mode = nbg::MODE_SNAPSHOT; d.grad = grad ? 1 : 0; d.ti = 0; d.ntbv = 1;
+  d.any_kick = any_kick(n, pair);
+  d.nsteps = nt; d.ntt = 0; d.h = h; d.hstep = h; d.hlast = 0.0; d.t0 = *t;
+  TrajOut o{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, times, vsky, bsky2, dbvdq0, dbvdelements};
+  int rc = run_single(ctx, d, x, v, xerror, verror, m, pair, jac_step, jac_error, dqdt, jac_init, t, nullptr, nullptr,
+                      nullptr, nullptr, &o);  // t: the state's time at the start
+  if (rc) return rc;
+  *t = times[nt - 1];
+  return 0;
+}
+
 // The batch's records, uploaded (state0 / jac0) but not yet copied to the
 // working buffers (nbg_batch_reset does that).  With jac_init null, jac_init
 // starts as the identity.

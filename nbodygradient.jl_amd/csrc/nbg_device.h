@@ -1385,6 +1385,97 @@ __device__ __attribute__((noinline)) void write_snapshot(const Smem<NB>& S, cons
   }
 }
 
+// check_step (Integrator.jl:249-259): the direction of an integration from
+// t0 towards tmax.
+__device__ inline double check_step_dev(double t0, double tmax) {
+  if (__builtin_fabs(tmax) > __builtin_fabs(t0)) return nbg::sign(tmax);
+  if (nbg::sign(tmax) != nbg::sign(t0)) return nbg::sign(tmax);
+  return -1.0 * nbg::sign(tmax);
+}
+
+// (intr)(s, ts::TransitSnapshot; grad) (Transits.jl:186-211): integrate to
+// each time in turn as (intr)(s, time) does (Integrator.jl:159-197: full
+// steps of h, then one step onto the time), then b^2 and v_sky of every body
+// relative to body 1 (calc_bv) and, with grad, their derivatives as
+// calc_dbvdq! forms them (snapshot.jl:17-41, its transit-time terms
+// included), one lane per Jacobian column; at the end the element gradients
+// of calc_dbvdelements! (snapshot.jl:44-54), which the reference runs with
+// or without grad.  Out of line, so the step loop's register allocation
+// stays as it is.
+template <int NB>
+__device__ __attribute__((noinline)) void run_snapshot(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int lane) {
+  constexpr int N7 = 7 * NB;
+  const int n = NB;
+  const int64_t nt = D.nsteps;
+  double* base = B.traj + D.off_traj;
+  double* vsky = base + snap_off_vsky(n, nt);
+  double* bsky2 = base + snap_off_bsky2(n, nt);
+  double* dq = base + snap_off_dq(n, nt);
+  double* del = base + snap_off_del(n, nt);
+  const bool grad = D.grad != 0, anyKick = D.any_kick != 0;
+  const double hstep = D.hstep;
+#pragma unroll 1
+  for (int64_t j = 0; j < nt; ++j) {
+    const double time = base[j];
+    const double t0 = S.t;
+    const int64_t ns = (int64_t)__builtin_fabs(__builtin_rint((time - t0) / hstep));  // abs(round(Int64, ...))
+    const double hs = hstep * check_step_dev(t0, time);
+    const double tmax = t0 + (hs * (double)ns);
+    const bool land = (tmax != time);
+#pragma unroll 1
+    for (int64_t k = 0; k < ns; ++k) ahl21_step<NB>(S, hs, grad, grad && k + 1 == ns && !land, anyKick, lane);
+    if (land) ahl21_step<NB>(S, time - tmax, grad, grad, anyKick, lane);
+    if (lane == 0) S.t = time;
+    wsync();
+    const auto J_ = jj<NB>(S);
+    for (int b = 1; b < n; ++b) {
+      const double dvx = S.v[3 * b] - S.v[0], dvy = S.v[3 * b + 1] - S.v[1];
+      const double dx = S.x[3 * b] - S.x[0], dy = S.x[3 * b + 1] - S.x[1];
+      const double vs = __builtin_sqrt(dvx * dvx + dvy * dvy);
+      if (lane == 0) {
+        vsky[b + n * j] = vs;
+        bsky2[b + n * j] = dx * dx + dy * dy;
+      }
+      if (!grad) continue;
+      const double* q = S.dqdt;
+      const double gs = dx * dvx + dy * dvy;
+      const double gd = ((dx * (q[7 * b + 3] - q[3]) + dy * (q[7 * b + 4] - q[4])) + dvx * (q[7 * b] - q[0])) +
+                        dvy * (q[7 * b + 1] - q[1]);
+      const double dvdt = (dvx * (q[7 * b + 3] - q[3]) + dvy * (q[7 * b + 4] - q[4])) / vs;
+      const double dbdt = 2.0 * gs;
+      // rows 7b + k of jac_step (k < 6) are row 6b + k of the compressed layout
+      for (int c = lane; c < N7; c += WAVE) {
+        auto Jr = [&](int body, int k) { return J_[(unsigned)((6 * body + k) * N7 + c)]; };
+        const double Jx = Jr(b, 0) - Jr(0, 0), Jy = Jr(b, 1) - Jr(0, 1);
+        const double Jvx = Jr(b, 3) - Jr(0, 3), Jvy = Jr(b, 4) - Jr(0, 4);
+        const double dtdq = -(((Jx * dvx + Jy * dvy) + Jvx * dx) + Jvy * dy) / gd;
+        const int64_t at = 2 * ((int64_t)b + n * (j + nt * c));
+        dq[at] = (Jvx * dvx + Jvy * dvy) / vs + dvdt * dtdq;
+        dq[at + 1] = 2.0 * (Jx * dx + Jy * dy) + dbdt * dtdq;
+      }
+    }
+  }
+  store_system<NB>(S, D, B, lane);
+  if (lane == 0) B.ints[D.off_int + NB] = D.nsteps + 1;
+  // calc_dbvdelements!: the sum over p, then q, of dbvdq0[ibv, i, j, q, p] x
+  // jac_init[7p + q, 7k + l], one lane per output column 7k + l; jac_init
+  // transposed into the Jacobian's store (no longer needed) first
+  const auto jt = jj<NB>(S);
+  const double* jinit = B.jac + D.off_jac + 2 * (int64_t)N7 * N7;
+  for (int col = lane; col < N7; col += WAVE)
+    for (int cp = 0; cp < N7; ++cp) jt[col + N7 * cp] = jinit[cp + (int64_t)N7 * col];
+  wsync();
+  for (int ibv = 0; ibv < 2; ++ibv)
+    for (int i = 0; i < n; ++i)
+      for (int64_t j = 0; j < nt; ++j)
+        for (int col = lane; col < N7; col += WAVE) {
+          double acc = 0.0;
+          for (int cp = 0; cp < N7; ++cp) acc = acc + dq[ibv + 2 * ((int64_t)i + n * (j + nt * cp))] * jt[col + N7 * cp];
+          del[ibv + 2 * ((int64_t)i + n * (j + nt * col))] = acc;
+        }
+  wsync();
+}
+
 // Run (or resume) one system for at most `chunk` steps.  Returns true when
 // the system has finished.
 template <int NB>
@@ -1393,6 +1484,10 @@ __device__ void run_system(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int6
   int64_t* ints = B.ints + D.off_int;
   const int64_t istep0 = ints[NB];
   if (istep0 > D.nsteps) return;  // finished earlier
+  if (D.mode == MODE_SNAPSHOT) {
+    run_snapshot<NB>(S, D, B, lane);
+    return;
+  }
   const int64_t iend = (chunk > 0 && istep0 + chunk < D.nsteps) ? istep0 + chunk : D.nsteps;
   const bool anyKick = D.any_kick != 0;
   const bool grad = D.grad != 0;

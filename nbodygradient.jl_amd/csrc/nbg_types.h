@@ -33,6 +33,7 @@ enum SysMode : int32_t {
   MODE_STEPS = 0,     // (intr)(s, N) / (intr)(s, time): nsteps steps of h, then an optional step of hlast
   MODE_TRANSITS = 1,  // (intr)(s, tt): step loop with transit search
   MODE_CARTESIAN = 2, // (intr)(s, o::CartesianOutput): nsteps steps of h, the state recorded before each
+  MODE_SNAPSHOT = 3,  // (intr)(s, ts::TransitSnapshot): integrate to each of nsteps times, b and v_sky there
 };
 
 struct SysDesc {
@@ -133,5 +134,14 @@ NBG_TD int64_t traj_off_jacerr(int n, int64_t ns, int f) {
 NBG_TD int64_t traj_len(int n, int64_t ns, int f) {
   return traj_off_jacerr(n, ns, f) + ((f & TRAJ_JACERR) ? 49 * (int64_t)n * n * ns : 0);
 }
+
+// Record of a MODE_SNAPSHOT run over nt times (in the traj buffer): the
+// times (input), then vsky and bsky2 [n x nt], dbvdq0 and dbvdelements
+// [2 x n x nt x 7 x n], column-major.
+NBG_TD int64_t snap_off_vsky(int, int64_t nt) { return nt; }
+NBG_TD int64_t snap_off_bsky2(int n, int64_t nt) { return nt + (int64_t)n * nt; }
+NBG_TD int64_t snap_off_dq(int n, int64_t nt) { return nt + 2 * (int64_t)n * nt; }
+NBG_TD int64_t snap_off_del(int n, int64_t nt) { return snap_off_dq(n, nt) + 14 * (int64_t)n * n * nt; }
+NBG_TD int64_t snap_len(int n, int64_t nt) { return snap_off_del(n, nt) + 14 * (int64_t)n * n * nt; }
 
 }  // namespace nbg

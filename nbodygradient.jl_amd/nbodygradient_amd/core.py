@@ -308,48 +308,18 @@ class Integrator:
         if o.file:
             o.save()
 
-    # (intr)(s, ts::TransitSnapshot; grad): Transits.jl:186-211.  Each time is
-    # reached with (intr)(s, time) on the GPU; b^2, v_sky and their
-    # derivatives follow calc_bv / calc_dbvdq! (snapshot.jl:7-41) with the
-    # reference's operation order, element by element, and the element
-    # gradients calc_dbvdelements! (snapshot.jl:44-54), which the reference
-    # runs with or without grad.
+    # (intr)(s, ts::TransitSnapshot; grad): Transits.jl:186-211, one call:
+    # the GPU integrates to each time and forms b^2, v_sky, their derivatives
+    # and the element gradients there
     def _snapshot(self, s, ts, grad, ctx):
-        n = s.n
-        if ts.n != n:
-            raise ValueError("TransitSnapshot was made for %d bodies, the state has %d" % (ts.n, n))
-        ts.vsky[:] = 0.0
-        ts.bsky2[:] = 0.0
-        ts.dbvdq0[:] = 0.0
-        for j, tj in enumerate(ts.times):
-            self._to_time(s, float(tj), grad, ctx)
-            x, v, q = s.x, s.v, s.dqdt
-            for b in range(1, n):
-                dvx, dvy = v[0, b] - v[0, 0], v[1, b] - v[1, 0]
-                dx, dy = x[0, b] - x[0, 0], x[1, b] - x[1, 0]
-                vs = math.sqrt(dvx * dvx + dvy * dvy)
-                ts.vsky[b, j] = vs
-                ts.bsky2[b, j] = dx * dx + dy * dy
-                if not grad:
-                    continue
-                gsky = dx * dvx + dy * dvy
-                gdot = ((dx * (q[7 * b + 3] - q[3]) + dy * (q[7 * b + 4] - q[4])) + dvx * (q[7 * b] - q[0])) + \
-                    dvy * (q[7 * b + 1] - q[1])
-                dvdt = (dvx * (q[7 * b + 3] - q[3]) + dvy * (q[7 * b + 4] - q[4])) / vs
-                dbdt = 2.0 * gsky
-                J = s.jac_step
-                Jx, Jy = J[7 * b] - J[0], J[7 * b + 1] - J[1]
-                Jvx, Jvy = J[7 * b + 3] - J[3], J[7 * b + 4] - J[4]
-                dtdq = -(((Jx * dvx + Jy * dvy) + Jvx * dx) + Jvy * dy) / gdot
-                d0 = (Jvx * dvx + Jvy * dvy) / vs + dvdt * dtdq
-                d1 = 2.0 * (Jx * dx + Jy * dy) + dbdt * dtdq
-                ts.dbvdq0[0, b, j] = d0.reshape((7, n), order="F")
-                ts.dbvdq0[1, b, j] = d1.reshape((7, n), order="F")
-        # dbvdelements[a, i, j, l, k] = sum over p, then q, of
-        # dbvdq0[a, i, j, q, p] * jac_init[7p + q, 7k + l], accumulated in that order
-        Jr = s.jac_init.reshape((7, n, 7, n), order="F")  # Jr[q, p, l, k]
-        acc = np.zeros((2, n, ts.nt, 7, n))
-        for p in range(n):
-            for qq in range(7):
-                acc += ts.dbvdq0[:, :, :, qq, p][:, :, :, None, None] * Jr[qq, p][None, None, None, :, :]
-        ts.dbvdelements[:] = acc
+        if ts.n != s.n:
+            raise ValueError("TransitSnapshot was made for %d bodies, the state has %d" % (ts.n, s.n))
+        c = _ctx(ctx)
+        t = C.c_double(s.t[0])
+        rc = _lib.lib().nbg_integrate_snapshot(
+            c.handle, s.n, C.c_double(self.h), C.c_int64(ts.nt), dptr(ts.times), int(grad), C.byref(t),
+            dptr(s.x), dptr(s.v), dptr(s.xerror), dptr(s.verror), dptr(s.m), u8ptr(s.pair_u8()),
+            dptr(s.jac_step), dptr(s.jac_error), dptr(s.dqdt), dptr(s.jac_init), dptr(ts.vsky), dptr(ts.bsky2),
+            dptr(ts.dbvdq0), dptr(ts.dbvdelements))
+        c.check(rc, "nbg_integrate_snapshot")
+        s.t[0] = t.value
