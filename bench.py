@@ -39,9 +39,9 @@ FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per system-step of nbg_run<9> (every n <= 9 in one launch), from
 # the rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the same counters'
 # reading of a copy of known size (profiles/nbg_run_c2_x0.05_counters.md:
-# 697.1 GB read + 917.8 GB written for the 4,174,120 system-steps nbg_run<9>
+# 700.9 GB read + 914.1 GB written for the 4,174,120 system-steps nbg_run<9>
 # carries of the C2 x0.05 batch, in one dispatch)
-HBM_BYTES_PER_UNIT = (697.1e9 + 917.8e9) / 4174120
+HBM_BYTES_PER_UNIT = (700.9e9 + 914.1e9) / 4174120
 METRIC = "system-steps/s w/ Jacobian, batched vs plain GPU; p50/p99 request latency"
 
 
