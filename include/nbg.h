@@ -175,6 +175,19 @@ int nbg_batch_download(nbg_batch* b, double* tt, double* dtdq0, double* dtdeleme
                        double* v);
 void nbg_batch_destroy(nbg_batch* b);
 
+/* TransitSnapshot for many systems in one persistent launch (the batched
+ * form of nbg_integrate_snapshot; no counterpart in the reference).  System
+ * s has n[s] bodies, step h[s], starts at t0[s] from x, v, m, jac_init (its
+ * slices of the concatenated arrays, as for nbg_batch_create; jac_step = I,
+ * errors zero) and is integrated to its nt[s] >= 1 times (concatenated in
+ * `times`).  Outputs concatenated in system order: vsky, bsky2 (n x nt per
+ * system), dbvdq0, dbvdelements (2 x n x nt x 7 x n per system; may be NULL
+ * without grad). */
+int nbg_batch_snapshot(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* t0,
+                       const int64_t* nt, const double* times, int32_t grad, const double* x, const double* v,
+                       const double* m, const uint8_t* pair, const double* jac_init, double* vsky, double* bsky2,
+                       double* dbvdq0, double* dbvdelements);
+
 /* One-shot convenience: create, run, download, destroy. */
 int nbg_batch_transits(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* tmax,
                        const double* t0, const int32_t* ti, const int64_t* ntt, int32_t grad, int32_t ntbv,

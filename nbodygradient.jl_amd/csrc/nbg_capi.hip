@@ -1197,6 +1197,99 @@ int nbg_batch_transits(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const doubl
   return rc;
 }
 
+int nbg_batch_snapshot(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* t0,
+                       const int64_t* nt, const double* times, int32_t grad, const double* x, const double* v,
+                       const double* m, const uint8_t* pair, const double* jac_init, double* vsky, double* bsky2,
+                       double* dbvdq0, double* dbvdelements) {
+  NBG_ARG(ctx != nullptr, "null context");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(nsys > 0 && n && h && t0 && nt && times && x && v && m && jac_init && vsky && bsky2,
+          "missing per-system arrays");
+  NBG_ARG(!grad || (dbvdq0 && dbvdelements), "grad run needs dbvdq0 and dbvdelements");
+  Layout L;
+  L.desc.resize(nsys);
+  std::vector<int64_t> ob(nsys + 1, 0), oj(nsys + 1, 0), op(nsys + 1, 0), ot(nsys + 1, 0), oo(nsys + 1, 0),
+      oq(nsys + 1, 0);
+  for (int s = 0; s < nsys; ++s) {
+    NBG_ARG(n[s] >= NB_MIN && n[s] <= NB_MAX, "n must be in [2, 12]");
+    NBG_ARG(std::isfinite(h[s]) && h[s] != 0.0 && nt[s] >= 1, "h finite and non-zero, nt >= 1");
+    ob[s + 1] = ob[s] + n[s];
+    oj[s + 1] = oj[s] + 49LL * n[s] * n[s];
+    op[s + 1] = op[s] + (int64_t)n[s] * n[s];
+    ot[s + 1] = ot[s] + nt[s];
+    oo[s + 1] = oo[s] + (int64_t)n[s] * nt[s];
+    oq[s + 1] = oq[s] + 14LL * n[s] * n[s] * nt[s];
+  }
+  for (int64_t q = 0; q < ot[nsys]; ++q) NBG_ARG(std::isfinite(times[q]), "times must be finite");
+  for (int s = 0; s < nsys; ++s) {
+    nbg::SysDesc& d = L.desc[s];
+    d.n = n[s]; d.mode = nbg::MODE_SNAPSHOT; d.grad = grad ? 1 : 0; d.ti = 0; d.ntbv = 1;
+    d.any_kick = pair ? any_kick(n[s], pair + op[s]) : 0;
+    d.nsteps = nt[s]; d.ntt = 0; d.h = h[s]; d.hstep = h[s]; d.hlast = 0.0; d.t0 = t0[s];
+  }
+  finalize_layout(L);
+  // order the queue by the integration each system has ahead (its time span
+  // in steps), not by its number of snapshot times
+  L.cost.resize(nsys);
+  for (int s = 0; s < nsys; ++s) {
+    double span = 0.0, tc = t0[s];
+    for (int64_t j = 0; j < nt[s]; ++j) {
+      span += std::fabs(times[ot[s] + j] - tc);
+      tc = times[ot[s] + j];
+    }
+    L.cost[s] = span / std::fabs(h[s]) + 1.0;
+  }
+  group_layout(L);
+  Store S;
+  auto done = [&](int rc) {
+    S.release();
+    return rc;
+  };
+  if (int rc = store_alloc(ctx, S, L, false)) return done(rc);
+  std::vector<double> hs((size_t)L.n_state), hj((size_t)L.n_jac), ht((size_t)L.n_traj, 0.0);
+  std::vector<int64_t> hi((size_t)L.n_int, 0);
+  std::vector<uint8_t> hp((size_t)L.n_pair, 0);
+  HostRec R;
+  for (int s = 0; s < nsys; ++s) {
+    const nbg::SysDesc& d = L.desc[s];
+    pack_system(d, x + 3 * ob[s], v + 3 * ob[s], nullptr, nullptr, m + ob[s], pair ? pair + op[s] : nullptr, nullptr,
+                nullptr, nullptr, jac_init + oj[s], t0[s], nullptr, nullptr, R);
+    std::copy(R.state.begin(), R.state.end(), hs.begin() + d.off_state);
+    std::copy(R.jac.begin(), R.jac.end(), hj.begin() + d.off_jac);
+    std::copy(R.pair.begin(), R.pair.end(), hp.begin() + d.off_pair);
+    std::copy(times + ot[s], times + ot[s + 1], ht.begin() + d.off_traj);
+  }
+  auto up = [&](void* dst, const void* src, size_t bytes) -> int {
+    NBG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+  };
+  if (int rc = up(S.state.p, hs.data(), hs.size() * sizeof(double))) return done(rc);
+  if (int rc = up(S.jac.p, hj.data(), hj.size() * sizeof(double))) return done(rc);
+  if (int rc = up(S.ints.p, hi.data(), hi.size() * sizeof(int64_t))) return done(rc);
+  if (int rc = up(S.pair.p, hp.data(), std::max<size_t>(hp.size(), 1))) return done(rc);
+  if (int rc = up(S.traj.p, ht.data(), ht.size() * sizeof(double))) return done(rc);
+  if (int rc = run_groups(ctx, S, L, 0, nullptr)) return done(rc);
+  hipError_t e = hipMemcpy(ht.data(), S.traj.p, ht.size() * sizeof(double), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    ctx->err = std::string("snapshot download: ") + hipGetErrorString(e);
+    return done(-2);
+  }
+  for (int s = 0; s < nsys; ++s) {
+    const nbg::SysDesc& d = L.desc[s];
+    const int k = d.n;
+    const int64_t ns = d.nsteps;
+    const double* base = ht.data() + d.off_traj;
+    std::copy(base + nbg::snap_off_vsky(k, ns), base + nbg::snap_off_vsky(k, ns) + k * ns, vsky + oo[s]);
+    std::copy(base + nbg::snap_off_bsky2(k, ns), base + nbg::snap_off_bsky2(k, ns) + k * ns, bsky2 + oo[s]);
+    if (dbvdq0) std::copy(base + nbg::snap_off_dq(k, ns), base + nbg::snap_off_dq(k, ns) + (oq[s + 1] - oq[s]),
+                          dbvdq0 + oq[s]);
+    if (dbvdelements)
+      std::copy(base + nbg::snap_off_del(k, ns), base + nbg::snap_off_del(k, ns) + (oq[s + 1] - oq[s]),
+                dbvdelements + oq[s]);
+  }
+  return done(0);
+}
+
 // ---------------------------------------------------------------------------
 // Serving: requests (one system each) admitted into a device arena as their
 // HBM footprint allows, advanced `chunk` steps per round by the persistent

@@ -168,3 +168,50 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def snapshot_batch(specs, times, states=None, grad=True, pair=None, ctx=None):
+    """TransitSnapshot for many systems in one launch (nbg_batch_snapshot):
+    system s starts at specs[s].t0 from states[s] = (x, v, jac_init) (host
+    init_nbody when omitted), steps with specs[s].h and is integrated to each
+    of times[s].  Returns per system a dict with vsky, bsky2 [n, nt] and, with
+    grad, dbvdq0, dbvdelements [2, n, nt, 7, n]."""
+    ctx = ctx if ctx is not None else _lib.default_context()
+    specs = list(specs)
+    if states is None:
+        states = initial_states(specs)
+    ns = len(specs)
+    n = np.array([s.n for s in specs], dtype=np.int32)
+    h = np.array([s.h for s in specs], dtype=np.float64)
+    t0 = np.array([s.t0 for s in specs], dtype=np.float64)
+    tl = [np.asarray(t, dtype=np.float64) for t in times]
+    nt = np.array([len(t) for t in tl], dtype=np.int64)
+    tt = np.concatenate(tl)
+    x = np.concatenate([np.asarray(st[0], order="F").ravel(order="F") for st in states])
+    v = np.concatenate([np.asarray(st[1], order="F").ravel(order="F") for st in states])
+    m = np.concatenate([s.elements[:, 0] for s in specs])
+    ji = np.concatenate([np.asarray(st[2], order="F").ravel(order="F") for st in states])
+    pr = None
+    if pair is not None:
+        pr = np.concatenate([np.asarray(p, dtype=np.uint8).ravel(order="F") for p in pair])
+    n64 = n.astype(np.int64)
+    vs = np.zeros(int((n64 * nt).sum()))
+    bs = np.zeros(int((n64 * nt).sum()))
+    dq = np.zeros(int((14 * n64 ** 2 * nt).sum())) if grad else None
+    de = np.zeros(int((14 * n64 ** 2 * nt).sum())) if grad else None
+    rc = _lib.lib().nbg_batch_snapshot(ctx.handle, ns, i32ptr(n), dptr(h), dptr(t0), i64ptr(nt), dptr(tt), int(grad),
+                                       dptr(x), dptr(v), dptr(m), u8ptr(pr), dptr(ji), dptr(vs), dptr(bs), dptr(dq),
+                                       dptr(de))
+    ctx.check(rc, "nbg_batch_snapshot")
+    out, oo, oq = [], 0, 0
+    for k, c in zip(n64, nt):
+        k, c = int(k), int(c)
+        r = {"vsky": vs[oo:oo + k * c].reshape((k, c), order="F"),
+             "bsky2": bs[oo:oo + k * c].reshape((k, c), order="F")}
+        if grad:
+            r["dbvdq0"] = dq[oq:oq + 14 * k * k * c].reshape((2, k, c, 7, k), order="F")
+            r["dbvdelements"] = de[oq:oq + 14 * k * k * c].reshape((2, k, c, 7, k), order="F")
+        out.append(r)
+        oo += k * c
+        oq += 14 * k * k * c
+    return out

@@ -48,3 +48,25 @@ def test_snapshot_matches_oracle(case, grad):
     assert s.t[0] == times[-1] and np.array_equal(s.x, os_.x)
     if grad:
         assert np.abs(ts.dbvdq0[0, 1:]).max() > 0
+
+
+def test_snapshot_batch_matches_oracle():
+    """Many systems' snapshots in one persistent launch (nbg_batch_snapshot),
+    body counts across the launch classes, each bitwise equal to the oracle."""
+    print("gpu: snapshot batch", flush=True)
+    from nbodygradient_amd.batch import snapshot_batch
+    rng = np.random.default_rng(96)
+    specs, times = [], []
+    for q, n in enumerate([3, 5, 7, 9, 10, 4, 8]):
+        sp = W.c2_system(rng, n)
+        specs.append(sp)
+        times.append(sp.t0 + np.sort(rng.uniform(0.0, 60.0, size=2 + q % 3)) * sp.h)
+    states = W.initial_states(specs, processes=1)
+    for grad in (True, False):
+        res = snapshot_batch(specs, times, states, grad=grad)
+        for sp, tj, (x, v, ji), r in zip(specs, times, states, res):
+            os_ = O.OState(x, v, sp.elements[:, 0], t=sp.t0, jac_init=ji)
+            vs, bs, dq, de = O.transit_snapshot(os_, sp.h, tj, grad=grad)
+            assert np.array_equal(r["vsky"], vs) and np.array_equal(r["bsky2"], bs), sp.n
+            if grad:
+                assert np.array_equal(r["dbvdq0"], dq) and np.array_equal(r["dbvdelements"], de), sp.n
