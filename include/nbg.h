@@ -188,6 +188,20 @@ int nbg_batch_snapshot(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const doubl
                        const double* m, const uint8_t* pair, const double* jac_init, double* vsky, double* bsky2,
                        double* dbvdq0, double* dbvdelements);
 
+/* CartesianOutput for many systems in one persistent launch (the batched
+ * form of nbg_integrate_cartesian; no counterpart in the reference).  System
+ * s (n[s] bodies, x, v, m slices as for nbg_batch_create, jac_step = I,
+ * errors zero, time t0[s]) takes nstep[s] >= 1 steps of h[s] in the direction
+ * check_step(t0[s], tmax[s]) and records the state before each.  records:
+ * 1 = xerror/verror, 2 = jac_step, 4 = jac_error (the latter two need grad).
+ * Records are concatenated in system order, each as nbg_integrate_cartesian
+ * lays them out (out_t: nstep; out_x ...: 3 x n x nstep; out_jac ...:
+ * 7n x 7n x nstep). */
+int nbg_batch_cartesian(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* tmax,
+                        const double* t0, const int64_t* nstep, int32_t grad, int32_t records, const double* x,
+                        const double* v, const double* m, const uint8_t* pair, double* out_t, double* out_x,
+                        double* out_v, double* out_xerror, double* out_verror, double* out_jac, double* out_jac_error);
+
 /* One-shot convenience: create, run, download, destroy. */
 int nbg_batch_transits(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* tmax,
                        const double* t0, const int32_t* ti, const int64_t* ntt, int32_t grad, int32_t ntbv,

@@ -1290,6 +1290,89 @@ int nbg_batch_snapshot(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const doubl
   return done(0);
 }
 
+int nbg_batch_cartesian(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* tmax,
+                        const double* t0, const int64_t* nstep, int32_t grad, int32_t records, const double* x,
+                        const double* v, const double* m, const uint8_t* pair, double* out_t, double* out_x,
+                        double* out_v, double* out_xerror, double* out_verror, double* out_jac, double* out_jac_error) {
+  NBG_ARG(ctx != nullptr, "null context");
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(nsys > 0 && n && h && tmax && t0 && nstep && x && v && m && out_t && out_x && out_v,
+          "missing per-system arrays");
+  NBG_ARG((records & ~(nbg::TRAJ_ERR | nbg::TRAJ_JAC | nbg::TRAJ_JACERR)) == 0, "unknown record flags");
+  NBG_ARG(!(records & nbg::TRAJ_ERR) || (out_xerror && out_verror), "error records need out_xerror and out_verror");
+  NBG_ARG(!(records & nbg::TRAJ_JAC) || out_jac, "Jacobian records need out_jac");
+  NBG_ARG(!(records & nbg::TRAJ_JACERR) || out_jac_error, "Jacobian error records need out_jac_error");
+  NBG_ARG(grad || !(records & (nbg::TRAJ_JAC | nbg::TRAJ_JACERR)), "Jacobian records need grad != 0");
+  Layout L;
+  L.desc.resize(nsys);
+  std::vector<int64_t> ob(nsys + 1, 0), op(nsys + 1, 0), o1(nsys + 1, 0), o3(nsys + 1, 0), oj(nsys + 1, 0);
+  for (int s = 0; s < nsys; ++s) {
+    NBG_ARG(n[s] >= NB_MIN && n[s] <= NB_MAX, "n must be in [2, 12]");
+    NBG_ARG(std::isfinite(h[s]) && h[s] != 0.0 && nstep[s] >= 1, "h finite and non-zero, nstep >= 1");
+    ob[s + 1] = ob[s] + n[s];
+    op[s + 1] = op[s] + (int64_t)n[s] * n[s];
+    o1[s + 1] = o1[s] + nstep[s];
+    o3[s + 1] = o3[s] + 3LL * n[s] * nstep[s];
+    oj[s + 1] = oj[s] + 49LL * n[s] * n[s] * nstep[s];
+  }
+  for (int s = 0; s < nsys; ++s) {
+    nbg::SysDesc& d = L.desc[s];
+    d.n = n[s]; d.mode = nbg::MODE_CARTESIAN; d.grad = grad ? 1 : 0; d.ti = 0; d.ntbv = 1;
+    d.any_kick = pair ? any_kick(n[s], pair + op[s]) : 0;
+    d.traj = records;
+    d.nsteps = nstep[s]; d.ntt = 0; d.h = h[s] * check_step(t0[s], tmax[s]); d.hstep = h[s]; d.hlast = 0.0;
+    d.t0 = t0[s];
+  }
+  finalize_layout(L);
+  Store S;
+  auto done = [&](int rc) {
+    S.release();
+    return rc;
+  };
+  if (int rc = store_alloc(ctx, S, L, false)) return done(rc);
+  std::vector<double> hs((size_t)L.n_state), hj((size_t)L.n_jac);
+  std::vector<int64_t> hi((size_t)L.n_int, 0);
+  std::vector<uint8_t> hp((size_t)L.n_pair, 0);
+  HostRec R;
+  for (int s = 0; s < nsys; ++s) {
+    const nbg::SysDesc& d = L.desc[s];
+    pack_system(d, x + 3 * ob[s], v + 3 * ob[s], nullptr, nullptr, m + ob[s], pair ? pair + op[s] : nullptr, nullptr,
+                nullptr, nullptr, nullptr, t0[s], nullptr, nullptr, R);
+    std::copy(R.state.begin(), R.state.end(), hs.begin() + d.off_state);
+    std::copy(R.jac.begin(), R.jac.end(), hj.begin() + d.off_jac);
+    std::copy(R.pair.begin(), R.pair.end(), hp.begin() + d.off_pair);
+  }
+  auto up = [&](void* dst, const void* src, size_t bytes) -> int {
+    NBG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return 0;
+  };
+  if (int rc = up(S.state.p, hs.data(), hs.size() * sizeof(double))) return done(rc);
+  if (int rc = up(S.jac.p, hj.data(), hj.size() * sizeof(double))) return done(rc);
+  if (int rc = up(S.ints.p, hi.data(), hi.size() * sizeof(int64_t))) return done(rc);
+  if (int rc = up(S.pair.p, hp.data(), std::max<size_t>(hp.size(), 1))) return done(rc);
+  if (int rc = run_groups(ctx, S, L, 0, nullptr)) return done(rc);
+  for (int s = 0; s < nsys; ++s) {
+    const nbg::SysDesc& d = L.desc[s];
+    const int k = d.n;
+    const int64_t ns = d.nsteps;
+    const double* base = S.traj.p + d.off_traj;
+    auto get = [&](double* dst, int64_t off, int64_t len) -> int {
+      NBG_HIP(hipMemcpy(dst, base + off, (size_t)len * sizeof(double), hipMemcpyDeviceToHost));
+      return 0;
+    };
+    const int64_t n3 = 3LL * k * ns, nj = 49LL * k * k * ns;
+    int rc = get(out_t + o1[s], 0, ns);
+    if (!rc) rc = get(out_x + o3[s], nbg::traj_off_x(k, ns), n3);
+    if (!rc) rc = get(out_v + o3[s], nbg::traj_off_v(k, ns), n3);
+    if (!rc && (records & nbg::TRAJ_ERR)) rc = get(out_xerror + o3[s], nbg::traj_off_xe(k, ns), n3);
+    if (!rc && (records & nbg::TRAJ_ERR)) rc = get(out_verror + o3[s], nbg::traj_off_ve(k, ns), n3);
+    if (!rc && (records & nbg::TRAJ_JAC)) rc = get(out_jac + oj[s], nbg::traj_off_jac(k, ns, records), nj);
+    if (!rc && (records & nbg::TRAJ_JACERR)) rc = get(out_jac_error + oj[s], nbg::traj_off_jacerr(k, ns, records), nj);
+    if (rc) return done(rc);
+  }
+  return done(0);
+}
+
 // ---------------------------------------------------------------------------
 // Serving: requests (one system each) admitted into a device arena as their
 // HBM footprint allows, advanced `chunk` steps per round by the persistent

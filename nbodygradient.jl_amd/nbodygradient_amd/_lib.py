@@ -57,6 +57,8 @@ def lib():
         [D, D, D, D, D, D, D]
     L.nbg_integrate_snapshot.argtypes = [VP, C.c_int32, C.c_double, C.c_int64, D, C.c_int32, D] + st + [D, D, D, D, D]
     L.nbg_batch_snapshot.argtypes = [VP, C.c_int32, I32, D, D, I64, D, C.c_int32, D, D, D, U8, D, D, D, D, D]
+    L.nbg_batch_cartesian.argtypes = [VP, C.c_int32, I32, D, D, D, I64, C.c_int32, C.c_int32, D, D, D, U8,
+                                      D, D, D, D, D, D, D]
     L.nbg_batch_create.argtypes = [VP, C.c_int32, I32, D, D, D, I32, I64, C.c_int32, C.c_int32, D, D, D, U8, D,
                                    C.POINTER(VP)]
     L.nbg_batch_create_elements.argtypes = [VP, C.c_int32, I32, D, D, D, I32, I64, C.c_int32, C.c_int32, D, D, U8,

@@ -215,3 +215,57 @@ def snapshot_batch(specs, times, states=None, grad=True, pair=None, ctx=None):
         oo += k * c
         oq += 14 * k * k * c
     return out
+
+
+def cartesian_batch(specs, nstep, states=None, grad=True, errors=True, jacobian=False, pair=None, ctx=None):
+    """CartesianOutput for many systems in one launch (nbg_batch_cartesian):
+    system s takes nstep[s] steps of specs[s].h from specs[s].t0 towards
+    specs[s].tmax, starting from states[s] = (x, v, jac_init) (host init_nbody
+    when omitted), and records the state before each step.  Returns per
+    system a dict of t [nstep], x, v (and xerror, verror) [3, n, nstep], and
+    with jacobian=True jac_step, jac_error [7n, 7n, nstep]."""
+    ctx = ctx if ctx is not None else _lib.default_context()
+    specs = list(specs)
+    if states is None:
+        states = initial_states(specs)
+    ns = len(specs)
+    n = np.array([s.n for s in specs], dtype=np.int32)
+    h = np.array([s.h for s in specs], dtype=np.float64)
+    tmax = np.array([s.tmax for s in specs], dtype=np.float64)
+    t0 = np.array([s.t0 for s in specs], dtype=np.float64)
+    nst = np.asarray(nstep, dtype=np.int64)
+    x = np.concatenate([np.asarray(st[0], order="F").ravel(order="F") for st in states])
+    v = np.concatenate([np.asarray(st[1], order="F").ravel(order="F") for st in states])
+    m = np.concatenate([s.elements[:, 0] for s in specs])
+    pr = None
+    if pair is not None:
+        pr = np.concatenate([np.asarray(p, dtype=np.uint8).ravel(order="F") for p in pair])
+    flags = (1 if errors else 0) | (6 if jacobian else 0)
+    n64 = n.astype(np.int64)
+    T = np.zeros(int(nst.sum()))
+    X = np.zeros(int((3 * n64 * nst).sum()))
+    V = np.zeros_like(X)
+    XE = np.zeros_like(X) if errors else None
+    VE = np.zeros_like(X) if errors else None
+    J = np.zeros(int((49 * n64 ** 2 * nst).sum())) if jacobian else None
+    JE = np.zeros_like(J) if jacobian else None
+    rc = _lib.lib().nbg_batch_cartesian(ctx.handle, ns, i32ptr(n), dptr(h), dptr(tmax), dptr(t0), i64ptr(nst),
+                                        int(grad), flags, dptr(x), dptr(v), dptr(m), u8ptr(pr), dptr(T), dptr(X),
+                                        dptr(V), dptr(XE), dptr(VE), dptr(J), dptr(JE))
+    ctx.check(rc, "nbg_batch_cartesian")
+    out, o1, o3, oj = [], 0, 0, 0
+    for k, c in zip(n64, nst):
+        k, c = int(k), int(c)
+        r = {"t": T[o1:o1 + c], "x": X[o3:o3 + 3 * k * c].reshape((3, k, c), order="F"),
+             "v": V[o3:o3 + 3 * k * c].reshape((3, k, c), order="F")}
+        if errors:
+            r["xerror"] = XE[o3:o3 + 3 * k * c].reshape((3, k, c), order="F")
+            r["verror"] = VE[o3:o3 + 3 * k * c].reshape((3, k, c), order="F")
+        if jacobian:
+            r["jac_step"] = J[oj:oj + 49 * k * k * c].reshape((7 * k, 7 * k, c), order="F")
+            r["jac_error"] = JE[oj:oj + 49 * k * k * c].reshape((7 * k, 7 * k, c), order="F")
+        out.append(r)
+        o1 += c
+        o3 += 3 * k * c
+        oj += 49 * k * k * c
+    return out

@@ -104,3 +104,20 @@ def test_cartesian_backward_no_jacobian_and_no_grad():
     ps = _pstate(x, v, m, sp.t0)
     nb.Integrator(sp.h, sp.t0 + 100.0)(ps, nb.CartesianOutput(5, 0))
     assert np.array_equal(ps.x, x) and ps.t[0] == sp.t0
+
+
+def test_cartesian_batch_matches_oracle():
+    """Many systems' records in one persistent launch (nbg_batch_cartesian),
+    body counts across the launch classes, bitwise equal to the oracle."""
+    say("gpu: cartesian batch")
+    from nbodygradient_amd.batch import cartesian_batch
+    rng = np.random.default_rng(97)
+    specs = [W.c2_system(rng, n) for n in (3, 6, 8, 10, 2)]
+    nstep = [40, 25, 31, 17, 50]
+    states = W.initial_states(specs, processes=1)
+    res = cartesian_batch(specs, nstep, states, grad=True, errors=True, jacobian=True)
+    for sp, N, (x, v, _), r in zip(specs, nstep, states, res):
+        os_ = O.OState(x, v, sp.elements[:, 0], t=sp.t0)
+        rec = O.cartesian_output(os_, sp.h, sp.tmax, N)
+        for k in ("t", "x", "v", "xerror", "verror", "jac_step", "jac_error"):
+            assert np.array_equal(r[k], rec[k]), (sp.n, k)
