@@ -174,7 +174,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (systems are independent: a "
-              "multi-GPU run would partition the batch, see DESIGN.md)" % args.gpus, file=sys.stderr)
+              "multi-GPU run would partition the batch across GPUs by estimated cost)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import nbodygradient_amd as nb

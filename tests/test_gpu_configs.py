@@ -1,4 +1,4 @@
-"""The baseline's configurations as GPU parity cases.
+"""The workloads C0, C1 and C3 as GPU parity cases.
 
 C0: the single 3-body system, full size (4000 steps), bit for bit against the
     oracle.
