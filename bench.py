@@ -1,31 +1,39 @@
 """Benchmark: AHL21 + Jacobian + transit timing, batched on one MI355X.
 
-Workload (default): 4096 independent systems, n ~ U{3..9} bodies, periods
+Workload (headline): the 4096-system batch -- n ~ U{3..9} bodies, periods
 log-uniform in [2, 200] d, h = P_in/30, tmax ~ U[500, 4000] d, FP64 state and
-Jacobian, every transit time with its full d t / d elements gradient.  A
-"step" of this benchmark is one complete pass of the hot path over the whole
-batch: (intr)(s, tt; grad=true) for all 4096 systems in one persistent launch
-per body count, inputs already resident in HBM.
+Jacobian, every transit time with its full d t / d elements gradient --
+resident in HBM.  One benchmark "step" is one ROUND over the whole batch
+(nbg_batch_round): every system advances K integrator steps of its own run
+(Transits.jl:140-170: the AHL21 step with Jacobian, the transit search after
+every step, calc_dtdelements! when its run ends) in one persistent launch;
+systems whose run has ended restart from their initial records, so every
+round carries the same mix.  `value` = system-steps (main steps + one per
+recorded transit) of the timed rounds / their wall time.
 
-Reported beside it:
-  plain_gpu    the one-system-per-launch form on a stratified sample, per-step
-               latency measured per n and summed over the batch's step counts;
+Reported beside it (side legs, each dropped -- never the headline -- when the
+time budget would not fit it):
+  c2_makespan  the same batch run once to completion (every system to its tmax);
+  plain_gpu    the one-system-per-launch form (nbg_integrate_transits), per-n
+               step latency on a sample, summed over the batch's step counts;
   cpu_baseline the CPU restatement (oracle) on a seeded sample of the same
-               systems, one system per thread on all host cores;
+               systems, one system per thread on the host cores; also on one core;
   roofline     algorithmic FP64 flops W(n) = 2016 n^2 (n-1) + 126 n^3 per
-               system-step (the structure-aware Jacobian-chain work), over the
-               batch's device time, against the FP64 peak (78.6 TFLOP/s);
-  serving      the C4 request trace through the serving loop (p50/p99);
-  fit_batch    the C3 fit-style batch (16384 perturbed 5-body systems, tmax
-               scaled), initial conditions computed on the GPU from the
-               elements, beside the host's init_nbody for the same systems.
+               system-step over the dominant launch's HIP-event time, against
+               the FP64 vector peak (78.6 TFLOP/s);
+  fit_batch    a subset of the 16384-vector fit batch (perturbations of one
+               6-body system: star + 5 planets) at its full tmax = 3000 d, initial
+               conditions computed on the GPU;
+  serving      request traces of integrate-plus-gradient calls at full tmax:
+               the arrivals at 2000/s (overload), and at a rate below the
+               measured capacity (latency p50/p99).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--systems N]
-                       [--tmax-scale S] [--no-plain] [--no-cpu] [--serve R]
-                       [--fit SYSTEMS] [--autotune FRAC] [--coop auto|off|K]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--round-steps S]
+                       [--budget SECONDS] [--no-side] ...
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -36,13 +44,13 @@ sys.path.insert(0, os.path.join(ROOT, "nbodygradient.jl_amd"))
 import numpy as np  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per system-step of nbg_run<9> (every n <= 9 in one launch), from
-# the rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the same counters'
-# reading of a copy of known size (profiles/nbg_run_c2_x0.05_counters.md:
-# 700.9 GB read + 914.1 GB written for the 4,174,120 system-steps nbg_run<9>
-# carries of the C2 x0.05 batch, in one dispatch)
-HBM_BYTES_PER_UNIT = (700.9e9 + 914.1e9) / 4174120
+# HBM bytes per system-step of nbg_run<9> in rounds of the 4096-system batch,
+# from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the same counters'
+# reading of nbg_copy_kernel of known size (the figures and the commands are in
+# profiles/nbg_run_c2_round_counters.md)
+HBM_BYTES_PER_UNIT = None
 METRIC = "system-steps/s w/ Jacobian, batched vs plain GPU; p50/p99 request latency"
+T_START = time.perf_counter()
 
 
 def W(n):
@@ -50,7 +58,7 @@ def W(n):
 
 
 def log(msg):
-    print(msg, file=sys.stderr, flush=True)
+    print("[%6.1fs] %s" % (time.perf_counter() - T_START, msg), file=sys.stderr, flush=True)
 
 
 def plain_latency(specs, states, per_n=2, steps_cap=1500):
@@ -76,45 +84,46 @@ def plain_latency(specs, states, per_n=2, steps_cap=1500):
 
 
 def fit_batch(count, tmax_scale):
-    """The C3 fit-style evaluation: `count` Gaussian perturbations of one
-    5-body system, every transit time with its dt/delements, the initial
-    conditions (x, v, jac_init) computed on the GPU from the elements."""
+    """The fit-style evaluation: `count` Gaussian perturbations of one 6-body
+    system (star + 5 planets), every transit time with its dt/delements, the
+    initial conditions (x, v, jac_init) computed on the GPU from the elements."""
     from nbodygradient_amd import workloads as Wk
     from nbodygradient_amd.batch import Batch
-    specs = Wk.c3(count)
+    specs = Wk.c3(16384)[:count]
     for s in specs:
         s.tmax *= tmax_scale
     t = time.perf_counter()
     b = Batch(specs, device_ic=True)
     t_ic = time.perf_counter() - t
+    t = time.perf_counter()
     st = b.run()
+    wall = time.perf_counter() - t
     t_run = st["kernel_ms"] / 1e3
     res = b.download(derivatives=False)
     b.close()
-    t = time.perf_counter()
-    Wk.initial_states(specs[:1024])
-    t_host = (time.perf_counter() - t) * len(specs) / 1024
-    out = {"workload": "C3: %d perturbations of the 5-body base system (P = 5, 8.1, 13.2, 21.6, 35.0 d), h = 0.1 d, "
-                       "tmax 3000 d x%g, Jacobian + transit times + dt/delements" % (len(specs), tmax_scale),
-           "system_steps": st["system_steps"], "transits": st["transits"], "device_s": t_run,
-           "system_steps_per_s": st["system_steps"] / t_run, "transits_per_s": st["transits"] / t_run,
-           "ic_on_gpu_s": t_ic, "ic_host_s_estimate": t_host,
-           "ic_host_basis": "init_nbody in host processes for 1024 systems, scaled to the batch",
+    out = {"workload": "fit batch: the first %d of 16384 perturbations of a 6-body system (star + 5 planets, "
+                       "P = 5, 8.1, 13.2, 21.6, 35.0 d), h = 0.1 d, tmax %g d, Jacobian + transit times + "
+                       "dt/delements, initial conditions on the GPU" % (len(specs), 3000.0 * tmax_scale),
+           "systems": len(specs), "system_steps": st["system_steps"], "transits": st["transits"],
+           "device_s": t_run, "wall_s": wall,
+           "system_steps_per_s": st["system_steps"] / wall, "transits_with_gradients_per_s": st["transits"] / wall,
+           "ic_on_gpu_s": t_ic,
+           "all_16384_estimate_s": wall * 16384 / len(specs),
+           "estimate_basis": "independent systems, throughput-bound batch: time x 16384 / subset",
            "max_count": int(max(int(r["count"].max()) for r in res))}
-    log("fit batch: %d systems, %d system-steps in %.2f s (%.3g/s), %d transits; IC on the GPU %.2f s "
-        "(host estimate %.1f s)" % (len(specs), st["system_steps"], t_run, out["system_steps_per_s"], st["transits"],
-                                    t_ic, t_host))
+    log("fit batch: %d systems, %d system-steps in %.2f s (%.3g/s), %d transits; IC on the GPU %.2f s" %
+        (len(specs), st["system_steps"], wall, out["system_steps_per_s"], st["transits"], t_ic))
     return out
 
 
-def cpu_baseline(specs, states, sample, threads, cap_steps=2000):
+def cpu_baseline(specs, states, sample, threads, cap_steps):
+    """The oracle (CPU restatement) on the first `sample` systems, each capped
+    at `cap_steps` steps, one system per OpenMP thread."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import ctypes as C
     import oracle_lib as O
     from nbodygradient_amd.workloads import SystemSpec
     idx = list(range(min(sample, len(specs))))
-    # cap each sampled system at `cap_steps` steps: the rate per system-step
-    # does not depend on the run length, and the cap keeps threads balanced
     sp = [SystemSpec(specs[i].elements, specs[i].t0, specs[i].h, min(specs[i].tmax, cap_steps * specs[i].h))
           for i in idx]
     st = [states[i] for i in idx]
@@ -145,109 +154,153 @@ def cpu_baseline(specs, states, sample, threads, cap_steps=2000):
     return done.value, secs, len(sp)
 
 
+def serving(trace_count, rate, deadline, chunk, log_fn, label):
+    """Requests of the serving trace (n ~ U{2..10}, tmax log-uniform in
+    [100, 8000] d, Poisson arrivals) offered at `rate` per second through the
+    serving loop, for at most `deadline` seconds."""
+    from nbodygradient_amd import workloads as Wk
+    from nbodygradient_amd.server import Server, serve
+    trace = Wk.c4(count=trace_count, rate=rate)
+    arrivals = np.array([t for t, _ in trace])
+    sspecs = [sp for _, sp in trace]
+    sstates = Wk.initial_states(sspecs)
+    sv = Server(max_bytes=64 << 30, chunk=chunk)
+    t = time.perf_counter()
+    sres = serve(sv, sspecs, sstates, arrivals, log=log_fn, deadline=deadline)
+    wall = time.perf_counter() - t
+    work = sv.work()
+    sv.close()
+    lat = np.asarray(sres["latency_s"])
+    fin = np.isfinite(lat)
+    nfin = int(fin.sum())
+    # unfinished requests count as latencies of at least their time in the
+    # system: a percentile that falls on one of them is reported as censored
+    lat_c = np.where(fin, lat, np.inf)
+    srt = np.sort(lat_c)
+
+    def pct(p):
+        k = min(len(srt) - 1, int(math.ceil(p / 100.0 * len(srt))) - 1)
+        return float(srt[k]) if np.isfinite(srt[k]) else None
+
+    steps_req = np.array([sp.nsteps for sp in sspecs], dtype=np.float64)
+    out = {"workload": "%s: %d requests of the serving trace (n ~ U{2..10}, tmax log-uniform in [100, 8000] d, "
+                       "h = P_in/30), Poisson arrivals at %g/s, Jacobian + transit times + dt/delements" %
+                       (label, len(sspecs), rate),
+           "offered_requests_per_s": rate, "chunk_steps": chunk, "deadline_s": deadline, "wall_s": wall,
+           "finished": nfin, "unfinished_at_deadline": len(sspecs) - nfin,
+           "p50_s": pct(50), "p90_s": pct(90), "p99_s": pct(99),
+           "max_finished_s": float(lat[fin].max()) if nfin else None,
+           "completed_requests_per_s": nfin / wall,
+           "system_steps": work["system_steps"], "system_steps_per_s": work["system_steps"] / wall,
+           "mean_main_steps_per_request": float(steps_req.mean()), "rounds": work["rounds"]}
+    log_fn("serving %s: %d/%d finished in %.1f s, p50 %s, p99 %s, %.3g system-steps/s" %
+           (label, nfin, len(sspecs), wall, out["p50_s"], out["p99_s"], out["system_steps_per_s"]))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=20, help="timed rounds")
+    ap.add_argument("--warmup", type=int, default=3, help="untimed rounds before them")
+    ap.add_argument("--round-steps", type=int, default=500, metavar="K",
+                    help="integrator steps every system advances per round")
     ap.add_argument("--systems", type=int, default=4096)
-    ap.add_argument("--tmax-scale", type=float, default=1.0)
+    ap.add_argument("--budget", type=float, default=450.0, metavar="SECONDS",
+                    help="wall-clock budget of the whole run: a side leg that would not fit is dropped")
+    ap.add_argument("--no-side", action="store_true", help="headline only")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-makespan", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=256)
-    ap.add_argument("--serve", type=int, default=1000, metavar="REQUESTS",
-                    help="serving leg: the first REQUESTS requests of the C4 trace (0 = skip)")
-    ap.add_argument("--serve-tmax-scale", type=float, default=0.1)
+    ap.add_argument("--cpu-cap", type=int, default=2000, help="steps per sampled system (all-core run)")
+    ap.add_argument("--cpu1-sample", type=int, default=32)
+    ap.add_argument("--cpu1-cap", type=int, default=600, help="steps per sampled system (one-core run)")
+    ap.add_argument("--fit", type=int, default=1024, metavar="SYSTEMS",
+                    help="fit-batch leg: the first SYSTEMS of the 16384 vectors at full tmax (0: skip)")
+    ap.add_argument("--fit-tmax-scale", type=float, default=1.0)
+    ap.add_argument("--serve", type=int, default=1, help="serving legs (0: skip)")
     ap.add_argument("--serve-chunk", type=int, default=128)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy of MIB MiB (read MIB, write MIB) through an elementwise "
                          "kernel: the reference reading for the FETCH_SIZE / WRITE_SIZE counters")
-    ap.add_argument("--fit", type=int, default=16384, metavar="SYSTEMS",
-                    help="C3 fit-style batch leg with initial conditions on the GPU (0: skip)")
-    ap.add_argument("--fit-tmax-scale", type=float, default=0.1)
-    ap.add_argument("--autotune", type=float, default=0.05, metavar="FRAC",
-                    help="before the timed steps, pick the launch configuration with the autotuner on a copy of "
-                         "the batch with step counts x FRAC (0: off; only with --coop auto)")
-    ap.add_argument("--coop", default="auto", metavar="auto|off|K",
-                    help="two-wave workgroups for the longest systems: automatic selection (default), off, or the "
-                         "K longest systems")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported (systems are independent: a "
               "multi-GPU run would partition the batch across GPUs by estimated cost)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    deadline = T_START + args.budget
 
     import nbodygradient_amd as nb
     from nbodygradient_amd import workloads as Wk
     from nbodygradient_amd.batch import Batch
 
-    t_setup = time.perf_counter()
     specs = Wk.c2(args.systems)
-    if args.tmax_scale != 1.0:
-        for s in specs:
-            s.tmax *= args.tmax_scale
     states = Wk.initial_states(specs)
     ctx = nb.default_context()
-    if args.coop == "off":
-        ctx.set_coop(0, 0)
-    elif args.coop != "auto":
-        ctx.set_coop(2, int(args.coop))
     name, cus = ctx.info()
-    log("device %s, %d CUs; %d systems prepared in %.1fs" % (name, cus, len(specs), time.perf_counter() - t_setup))
+    log("device %s, %d CUs; %d systems prepared" % (name, cus, len(specs)))
 
     # prime code loading with a tiny batch (not part of the workload)
     pb = Batch(specs[:8], states[:8])
     pb.run()
     pb.close()
 
+    K = args.round_steps
     batch = Batch(specs, states, grad=True)
     tune = None
-    if args.autotune > 0 and args.coop == "auto":
+    if not args.no_autotune:
         t = time.perf_counter()
-        tune = batch.autotune(args.autotune)
+        tune = batch.autotune_round(K)
         c = tune["candidates"][tune["best"]]
-        log("autotune (x%g copy, %.1f s): %s -> coop mode %d max %d, groups/CU %d" %
-            (args.autotune, time.perf_counter() - t, ", ".join("%.0f" % q["ms"] for q in tune["candidates"]),
+        log("autotune (one round of %d steps per candidate, %.1f s): %s ms -> coop mode %d max %d, groups/CU %d" %
+            (K, time.perf_counter() - t, ", ".join("%.0f" % q["ms"] for q in tune["candidates"]),
              c["coop_mode"], c["coop_max"], c["groups_per_cu"]))
     for w in range(args.warmup):
-        batch.reset()
-        batch.run()
-    ms = []
-    stats = None
+        st = batch.round(K)
+        log("warmup round %d: %.1f ms device, %d system-steps" % (w, st["kernel_ms"], st["system_steps"]))
+    ms, units, trans, newton = [], 0, 0, 0
+    launches = {}
     for k in range(args.steps):
-        batch.reset()
         t = time.perf_counter()
-        stats = batch.run()
+        st = batch.round(K)
         wall = time.perf_counter() - t
         ms.append(wall * 1e3)
-        log("step %d: %.1f ms wall, %.1f ms device, %d system-steps, %d transits" %
-            (k, wall * 1e3, stats["kernel_ms"], stats["system_steps"], stats["transits"]))
+        units += st["system_steps"]
+        trans += st["transits"]
+        newton += st["newton_iters"]
+        for cls, L in st["launches"].items():
+            a = launches.setdefault(cls, {"ms": 0.0, "units": {}, "grid": L["grid"]})
+            a["ms"] += L["ms"]
+            for n, u in L["units"].items():
+                a["units"][n] = a["units"].get(n, 0) + u
+        log("round %d: %.1f ms wall, %.1f ms device, %d system-steps, %d transits" %
+            (k, wall * 1e3, st["kernel_ms"], st["system_steps"], st["transits"]))
+    t_timed = sum(ms) / 1e3
+    value = units / t_timed
     ms_per_step = float(np.mean(ms))
-    sys_steps = stats["system_steps"]
-    value = sys_steps / (ms_per_step / 1e3)
-    res = batch.download(derivatives=False)
-    # per-launch roofline: algorithmic FP64 flops W(n) x units of the launch for
-    # n (main steps + one per recorded transit) over that launch's duration,
-    # timed with HIP events on the stream it ran on
-    launches = {}
-    for cls, L in stats["launches"].items():
+    # per-launch roofline over the timed rounds: algorithmic FP64 flops W(n)
+    # x units of the launch for n over its duration, from HIP events on the
+    # stream it ran on
+    lrep = {}
+    for cls, L in launches.items():
         flops = sum(W(n) * u for n, u in L["units"].items())
-        name = "nbg_run_coop<10>" if cls == 1 else "nbg_run<%d>" % cls  # class 1: two-wave workgroups
-        launches[str(cls)] = {"kernel": name, "ms": L["ms"], "grid": L["grid"],
-                              "units": sum(L["units"].values()),
-                              "units_by_n": {str(n): u for n, u in sorted(L["units"].items())}, "flops": flops,
-                              "tflops": flops / (L["ms"] / 1e3) / 1e12 if L["ms"] > 0 else 0.0}
-    # the launches of a batch run concurrently and end together (each takes
-    # its own class first, then smaller ones); the dominant one is the launch
-    # that carries the most algorithmic work
-    dom = max(launches, key=lambda k: launches[k]["flops"])
-    achieved = launches[dom]["tflops"]
-    total_flops = sum(L["flops"] for L in launches.values())
-    log("download: %d systems, max count %d" % (len(res), max(int(r["count"].max()) for r in res)))
-    for n in sorted(launches, key=int):
-        L = launches[n]
-        log("launch %s: grid %d, %.1f ms, %d units %s, %.3f TFLOP/s" % (L["kernel"], L["grid"], L["ms"], L["units"],
-                                                                        L["units_by_n"], L["tflops"]))
+        kname = "nbg_run_coop<10>" if cls == 1 else "nbg_run<%d>" % cls
+        lrep[str(cls)] = {"kernel": kname, "ms_total": L["ms"], "ms_per_round": L["ms"] / args.steps,
+                          "grid": L["grid"], "units": sum(L["units"].values()),
+                          "units_by_n": {str(n): u for n, u in sorted(L["units"].items())}, "flops": flops,
+                          "tflops": flops / (L["ms"] / 1e3) / 1e12 if L["ms"] > 0 else 0.0}
+    dom = max(lrep, key=lambda k: lrep[k]["flops"])
+    D = lrep[dom]
+    achieved = D["tflops"]
+    for c in sorted(lrep, key=int):
+        L = lrep[c]
+        log("launch %s: grid %d, %.1f ms per round, %d units %s, %.3f TFLOP/s" %
+            (L["kernel"], L["grid"], L["ms_per_round"], L["units"], L["units_by_n"], L["tflops"]))
+    traffic = None
+    if HBM_BYTES_PER_UNIT is not None:
+        traffic = HBM_BYTES_PER_UNIT * D["units"] / args.steps
     out = {
         "metric": METRIC,
         "value": value,
@@ -261,72 +314,96 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "C2: %d systems, n~U{3..9}, P log-U[2,200] d, h=P_in/30, tmax~U[500,4000] d%s, "
-                               "Jacobian + transit times + dt/delements" %
-                               (len(specs), "" if args.tmax_scale == 1.0 else " x%g" % args.tmax_scale),
-                   "systems": len(specs), "system_steps": sys_steps, "transits": stats["transits"],
-                   "newton_iters": stats["newton_iters"], "parallelism": "1 GPU, one persistent launch per body-count class (n <= 9 in one)",
+        "config": {"workload": "4096-system batch: n~U{3..9}, P log-U[2,200] d, h=P_in/30, tmax~U[500,4000] d, "
+                               "Jacobian + transit times + dt/delements; one step = one round in which every "
+                               "system advances %d integrator steps (finished systems restart)" % K,
+                   "systems": len(specs), "round_steps": K, "system_steps_timed": units,
+                   "transits_timed": trans, "newton_iters_timed": newton,
+                   "parallelism": "1 GPU, one persistent launch per body-count class (n <= 9 in one)",
                    "autotune": tune},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS,
-                     "traffic": HBM_BYTES_PER_UNIT * launches[dom]["units"],
-                     "traffic_basis": "rocprofv3 FETCH_SIZE/WRITE_SIZE of nbg_run<9> on C2 x0.05, calibrated on nbg_copy_kernel: "
-                                      "%.0f B per system-step x this launch's units" % HBM_BYTES_PER_UNIT,
-                     "kernel": launches[dom]["kernel"], "launch_ms": launches[dom]["ms"],
-                     "note": "FP64 compute bound (VALU); MI355X FP64 vector and matrix peaks are both 78.6 "
-                             "TFLOP/s; W(n) = 2016 n^2 (n-1) + 126 n^3 flops per system-step"},
-        "launches": launches,
-        "batch_tflops": total_flops / (stats["kernel_ms"] / 1e3) / 1e12,
-        "transits_per_s": stats["transits"] / (ms_per_step / 1e3),
+        "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
+                     "kernel": D["kernel"], "launch_ms": D["ms_per_round"], "units_per_launch": D["units"] / args.steps,
+                     "note": "FP64 vector (VALU) peak 78.6 TFLOP/s; no MFMA in the kernel; W(n) = 2016 n^2 (n-1) + "
+                             "126 n^3 flops per system-step; traffic = HBM bytes per launch from calibrated "
+                             "rocprofv3 FETCH_SIZE + WRITE_SIZE"},
+        "launches": lrep,
+        "transits_with_gradients_per_s": trans / t_timed,
     }
-    if args.serve > 0:
-        from nbodygradient_amd.server import Server, serve
-        ctx.set_coop(1, 128)  # serving rounds keep the defaults, whatever the batch's autotuner chose
+    log("headline: %.4g system-steps/s (%.1f ms per round), %s at %.3f TFLOP/s = %.4f of peak" %
+        (value, ms_per_step, D["kernel"], achieved, achieved / FP64_PEAK_TFLOPS))
+    skipped = []
+
+    def fits(est, leg):
+        if args.no_side:
+            return False
+        if time.perf_counter() + est > deadline:
+            skipped.append(leg)
+            log("side leg %s dropped: %.0f s left, needs ~%.0f s" % (leg, deadline - time.perf_counter(), est))
+            return False
+        return True
+
+    if not args.no_makespan and fits(45, "c2_makespan"):
+        ctx.set_coop(1, 128)  # whole runs: the automatic two-wave selection, four one-wave groups per CU
         ctx.set_launch(0)
-        trace = Wk.c4(count=args.serve)
-        arrivals = np.array([t for t, _ in trace])
-        sspecs = [sp for _, sp in trace]
-        for sp in sspecs:
-            sp.tmax *= args.serve_tmax_scale
-        sstates = Wk.initial_states(sspecs)
-        sv = Server(max_bytes=32 << 30, chunk=args.serve_chunk)
-        sres = serve(sv, sspecs, sstates, arrivals, log=log)
-        sv.close()
-        lat = sres["latency_s"]
-        ssteps = sum(sp.nsteps for sp in sspecs)
-        out["serving"] = {
-            "workload": "C4 trace: first %d requests, Poisson arrivals at 2000/s, n~U{2..10}, tmax log-U[100,8000] d "
-                        "x%g, h=P_in/30, Jacobian + transit times + dt/delements" % (len(sspecs), args.serve_tmax_scale),
-            "chunk_steps": args.serve_chunk, "p50_s": float(np.percentile(lat, 50)),
-            "p99_s": float(np.percentile(lat, 99)), "max_s": float(np.max(lat)),
-            "sustained_requests_per_s": len(sspecs) / sres["wall_s"], "wall_s": sres["wall_s"],
-            "main_steps_per_s": ssteps / sres["wall_s"], "rounds": sres["rounds"],
-            "offered_requests_per_s": len(sspecs) / float(arrivals[-1])}
-        log("serving: %d requests, p50 %.3f s, p99 %.3f s, %.1f requests/s sustained, %.3g main steps/s" %
-            (len(sspecs), out["serving"]["p50_s"], out["serving"]["p99_s"], out["serving"]["sustained_requests_per_s"],
-             out["serving"]["main_steps_per_s"]))
-    if args.fit > 0:
-        out["fit_batch"] = fit_batch(args.fit, args.fit_tmax_scale)
-    if not args.no_plain:
+        batch.reset()
+        t = time.perf_counter()
+        st = batch.run()
+        wall = time.perf_counter() - t
+        out["c2_makespan"] = {"wall_s": wall, "device_s": st["kernel_ms"] / 1e3, "system_steps": st["system_steps"],
+                              "transits": st["transits"], "system_steps_per_s": st["system_steps"] / wall,
+                              "what": "the 4096-system batch once to completion (every system to its tmax), "
+                                      "the longest systems on two-wave workgroups (automatic)"}
+        log("makespan: %.2f s wall, %d system-steps, %d transits" % (wall, st["system_steps"], st["transits"]))
+    plain_rate = None
+    if not args.no_plain and fits(15, "plain_gpu"):
         lat = plain_latency(specs, states)
-        t_plain = 0.0
-        for sp, r in zip(specs, res):
-            tr = int(np.minimum(r["count"], sp.ntt()).sum())
-            t_plain += lat[sp.n] * (sp.nsteps + tr)
-        plain_rate = sys_steps / t_plain
+        # the batch's work in whole runs, priced at the one-system form's per-step latency
+        t_plain, tot = 0.0, 0
+        for sp in specs:
+            t_plain += lat[sp.n] * sp.nsteps
+            tot += sp.nsteps
+        plain_rate = tot / t_plain
         out["plain_gpu"] = {"value": plain_rate, "unit": "system-steps/s",
-                            "method": "one system per launch (nbg_integrate_transits); per-n step latency from 2 "
-                                      "systems x <=1500 steps, summed over the batch"}
+                            "method": "one system per call (nbg_integrate_transits, host arrays in and out); per-n "
+                                      "step latency from 2 systems x <= 1500 steps, priced over the batch's main steps"}
         out["batched_vs_plain"] = value / plain_rate
-    if not args.no_cpu:
-        # one thread per host core of this process's CPU share (at most 16)
+    if not args.no_cpu and fits(40, "cpu_baseline"):
         threads = max(1, min(16, len(os.sched_getaffinity(0))))
-        done, secs, ns = cpu_baseline(specs, states, args.cpu_sample, threads)
+        done, secs, ns = cpu_baseline(specs, states, args.cpu_sample, threads, args.cpu_cap)
         cpu_rate = done / secs
         out["cpu_baseline"] = {"value": cpu_rate, "unit": "system-steps/s", "cores": threads, "kind": "port",
-                               "sample": "first %d of the %d systems (seeded), each capped at 2000 steps, %d system-steps "
-                                         "in %.1f s, OpenMP one system per thread" % (ns, len(specs), done, secs)}
+                               "sample": "first %d of the %d systems (seeded), each capped at %d steps: %d system-steps "
+                                         "in %.1f s, OpenMP one system per thread" %
+                                         (ns, len(specs), args.cpu_cap, done, secs)}
         out["batched_vs_cpu"] = value / cpu_rate
+        log("cpu: %d threads, %.4g system-steps/s" % (threads, cpu_rate))
+        done1, secs1, ns1 = cpu_baseline(specs, states, args.cpu1_sample, 1, args.cpu1_cap)
+        out["cpu_baseline_1core"] = {"value": done1 / secs1, "unit": "system-steps/s", "cores": 1, "kind": "port",
+                                     "sample": "first %d systems, each capped at %d steps: %d system-steps in %.1f s" %
+                                               (ns1, args.cpu1_cap, done1, secs1)}
+        log("cpu 1 core: %.4g system-steps/s" % (done1 / secs1))
+    if args.fit > 0 and fits(40, "fit_batch"):
+        out["fit_batch"] = fit_batch(args.fit, args.fit_tmax_scale)
+    if args.serve > 0 and fits(40, "serving_overload"):
+        ctx.set_coop(1, 128)  # serving rounds keep the defaults, whatever the autotuner chose
+        ctx.set_launch(0)
+        # 2000 requests at the trace's 2000/s: far more work than the GPU
+        # completes, so the server's system-step rate is its capacity
+        ov = serving(2000, 2000.0, 30.0, args.serve_chunk, log, "overload")
+        out["serving_overload"] = ov
+        # capacity in requests/s: that rate over the trace's mean main steps per
+        # request (transit units make it a slight overestimate); the latency
+        # run offers half of it
+        per_req = ov["mean_main_steps_per_request"]
+        cap = ov["system_steps_per_s"] / per_req if per_req > 0 else 0.0
+        rate = max(1.0, round(0.5 * cap))
+        nreq = int(min(4000, max(50, rate * 12.0)))
+        out["serving_capacity_requests_per_s"] = cap
+        if fits(110, "serving_latency"):
+            out["serving"] = serving(nreq, rate, 100.0, args.serve_chunk, log, "latency at half capacity")
+    out["side_legs_dropped"] = skipped
+    out["bench_wall_s"] = time.perf_counter() - T_START
     if args.calibrate_copy > 0:
         import ctypes as C
         from nbodygradient_amd import _lib
@@ -337,6 +414,7 @@ def main():
                                    "GB_per_s": 2 * nbytes / (cms.value / 1e3) / 1e9, "kernel": "nbg_copy_kernel"}
         log("copy calibration: nbg_copy_kernel, %d MiB read + %d MiB written, %.3f ms, %.0f GB/s" %
             (args.calibrate_copy, args.calibrate_copy, cms.value, 2 * nbytes / (cms.value / 1e3) / 1e9))
+    batch.close()
     print(json.dumps(out), flush=True)
 
 

@@ -150,6 +150,11 @@ int nbg_batch_create_elements(nbg_ctx* ctx, int32_t nsys, const int32_t* n, cons
  * system), concatenated in system order; any pointer may be NULL. */
 int nbg_batch_initial(nbg_batch* b, double* x, double* v, double* jac_init);
 
+/* Autotuner for rounds (no counterpart in the reference): from the batch's
+ * initial state, one round of `steps` steps (nbg_batch_round) under each
+ * candidate launch configuration of nbg_batch_autotune_candidate; keeps the
+ * fastest in b's context and resets the batch. */
+int nbg_batch_autotune_round(nbg_batch* b, int64_t steps, int32_t* best, double* cand_ms, int32_t max_cand);
 /* Autotuner (no counterpart in the reference): runs a copy of batch b with
  * every system's step count scaled by frac (0 < frac <= 1) under each
  * candidate launch configuration -- the workgroup form (two-wave workgroups
@@ -170,6 +175,16 @@ int nbg_batch_reset(nbg_batch* b);
  * within a launch, launches overlapped on streams; the longest systems may
  * run on two-wave workgroups in a launch of their own (nbg_set_coop). */
 int nbg_batch_run(nbg_batch* b, nbg_batch_stats* stats);
+/* One round of a batch (the unit of the throughput benchmark; no
+ * counterpart in the reference): every system advances at most `steps`
+ * integrator steps of its own run -- Transits.jl:140-170, transit search and,
+ * at a run's end, calc_dtdelements! included -- in one set of persistent
+ * launches.  With restart != 0, systems whose run had ended before the round
+ * first start it again from their initial records (state, Jacobians,
+ * counters; earlier transit outputs are overwritten as the new run counts).
+ * Stats are the work of this round alone: its steps plus one unit per
+ * transit recorded, its transits and Newton iterations. */
+int nbg_batch_round(nbg_batch* b, int64_t steps, int32_t restart, nbg_batch_stats* stats);
 /* Copy results to host (any pointer may be NULL). */
 int nbg_batch_download(nbg_batch* b, double* tt, double* dtdq0, double* dtdelements, int64_t* count, double* x,
                        double* v);
@@ -237,6 +252,9 @@ int nbg_server_finished(nbg_server* s, int64_t* ids, int32_t max_ids, int32_t* n
 int nbg_server_fetch(nbg_server* s, int64_t id, double* tt, double* dtdq0, double* dtdelements, int64_t* count,
                      double* x, double* v);
 int nbg_server_status(nbg_server* s, int64_t* pending, int64_t* active, int64_t* finished, int64_t* bytes_free);
+/* Work done by every round so far: system-steps (main steps plus one per
+ * recorded transit when gradients are on), rounds and their device time. */
+int nbg_server_work(nbg_server* s, int64_t* system_steps, int64_t* rounds, double* device_ms);
 /* Device bytes one request of n bodies and ntt transit slots occupies. */
 int64_t nbg_server_footprint(int32_t n, int32_t ntbv, int64_t ntt);
 

@@ -29,6 +29,9 @@ extern "C" hipError_t nbg_launch_copy(const void* src, void* dst, size_t bytes, 
 extern "C" hipError_t nbg_launch_coop(nbg::GroupList gl, nbg::SysBufs bufs, nbg::CoopBufs coop, int64_t chunk, int grid,
                                       unsigned long long* units, hipStream_t stream);
 extern "C" int64_t nbg_coop_stride();
+extern "C" hipError_t nbg_launch_rewind(const nbg::SysDesc* descs, int nsys, double* state, const double* state0,
+                                        double* jac, const double* jac0, int64_t* ints, int cu_count,
+                                        hipStream_t stream);
 // Initial conditions kernel (nbg_ic.hip)
 extern "C" hipError_t nbg_launch_ic(const void* descs, int nsys, const double* in, double* state, double* jac,
                                     hipStream_t stream);
@@ -191,6 +194,7 @@ struct nbg_batch {
   Layout L;
   Store S;
   int64_t steps_main = 0;
+  int64_t order_chunk = 0;  // the visit length the device queue order is for (0: whole runs)
 };
 
 #define NBG_HIP(expr)                                                                 \
@@ -532,8 +536,6 @@ int run_single(nbg_ctx* ctx, nbg::SysDesc d, double* x, double* v, double* xe, d
     if (JE) std::memcpy(JE, R.jac.data() + NN, NN * sizeof(double));
     if (dqdt)
       for (int q = 0; q < s7; ++q) dqdt[q] = R.state[nbg::off_dqdt(n) + q];
-  } else if (dqdt && D.mode == nbg::MODE_TRANSITS) {
-    for (int q = 0; q < s7; ++q) dqdt[q] = R.state[nbg::off_dqdt(n) + q];
   }
   if (D.mode == nbg::MODE_SNAPSHOT) {
     const int64_t nt = D.nsteps;
@@ -1010,17 +1012,46 @@ int nbg_batch_reset(nbg_batch* b) {
   return 0;
 }
 
-int nbg_batch_run(nbg_batch* b, nbg_batch_stats* stats) {
-  if (!b) return -1;
+namespace {
+
+// Order a batch's device queues for visits of `chunk` steps (0: whole runs):
+// longest first by the steps one visit takes, min(chunk, nsteps + 1), times
+// the per-step cost of the body count; the descriptors are uploaded again.
+int batch_order(nbg_batch* b, int64_t chunk) {
   nbg_ctx* ctx = b->ctx;
-  std::lock_guard<std::mutex> lk(ctx->mu);
-  double ms = 0.0;
-  std::vector<float> gms;
-  std::vector<unsigned long long> gun;
-  std::vector<int> ggrid;
-  if (int rc = run_groups(ctx, b->S, b->L, 0, &ms, &gms, &gun, nullptr, &ggrid)) return rc;
-  if (stats) {
-    std::memset(stats, 0, sizeof(*stats));
+  if (b->order_chunk == chunk) return 0;
+  Layout& L = b->L;
+  L.cost.clear();
+  if (chunk > 0) {
+    L.cost.resize(L.desc.size());
+    for (size_t i = 0; i < L.desc.size(); ++i) L.cost[i] = (double)std::min<int64_t>(chunk, L.desc[i].nsteps + 1);
+  }
+  group_layout(L);
+  std::vector<nbg::SysDesc> sorted(L.desc.size());
+  for (size_t k = 0; k < L.order.size(); ++k) sorted[k] = L.desc[L.order[k]];
+  NBG_HIP(hipMemcpy(b->S.desc.p, sorted.data(), sorted.size() * sizeof(nbg::SysDesc), hipMemcpyHostToDevice));
+  b->order_chunk = chunk;
+  return 0;
+}
+
+// Per-launch figures of one run_groups call (launch classes, then the
+// cooperative launch as class 1).  With per_visit, the totals are the work
+// of this call's visits (the launches' unit counters); otherwise they are
+// the batch's run totals from its counters.
+int fill_stats(nbg_batch* b, double ms, const std::vector<float>& gms, const std::vector<unsigned long long>& gun,
+               const std::vector<int>& ggrid, bool per_visit, nbg_batch_stats* stats) {
+  nbg_ctx* ctx = b->ctx;
+  std::memset(stats, 0, sizeof(*stats));
+  const size_t G = b->L.gn.size();
+  const bool coop = ggrid.size() > G && ggrid[G] > 0;
+  if (per_visit) {
+    for (size_t g = 0; g <= G; ++g) {
+      if (g == G && !coop) break;
+      stats->transits += (int64_t)gun[13 * g + 0];
+      stats->newton_iters += (int64_t)gun[13 * g + 1];
+      for (int n = 2; n < 13; ++n) stats->system_steps += (int64_t)gun[13 * g + n];
+    }
+  } else {
     std::vector<int64_t> hi((size_t)b->L.n_int);
     NBG_HIP(hipMemcpy(hi.data(), b->S.ints.p, hi.size() * sizeof(int64_t), hipMemcpyDeviceToHost));
     int64_t tr = 0, it = 0;
@@ -1031,21 +1062,55 @@ int nbg_batch_run(nbg_batch* b, nbg_batch_stats* stats) {
     stats->transits = tr;
     stats->newton_iters = it;
     stats->system_steps = b->steps_main + (b->L.desc.empty() || !b->L.desc[0].grad ? 0 : tr);
-    stats->kernel_ms = ms;
-    const size_t G = b->L.gn.size();
-    for (size_t g = 0; g < G; ++g) {
-      const int c = b->L.gn[g];
-      stats->launch_ms[c] = gms[g];
-      stats->launch_grid[c] = ggrid[g];
-      for (int n = 0; n < 13; ++n) stats->launch_units[c][n] = (int64_t)gun[13 * g + n];
-    }
-    if (ggrid.size() > G && ggrid[G] > 0) {  // the cooperative launch, reported as class 1
-      stats->launch_ms[1] = gms[G];
-      stats->launch_grid[1] = ggrid[G];
-      for (int n = 0; n < 13; ++n) stats->launch_units[1][n] = (int64_t)gun[13 * G + n];
-    }
+  }
+  stats->kernel_ms = ms;
+  for (size_t g = 0; g < G; ++g) {
+    const int c = b->L.gn[g];
+    stats->launch_ms[c] = gms[g];
+    stats->launch_grid[c] = ggrid[g];
+    for (int n = 2; n < 13; ++n) stats->launch_units[c][n] = (int64_t)gun[13 * g + n];
+  }
+  if (coop) {
+    stats->launch_ms[1] = gms[G];
+    stats->launch_grid[1] = ggrid[G];
+    for (int n = 2; n < 13; ++n) stats->launch_units[1][n] = (int64_t)gun[13 * G + n];
   }
   return 0;
+}
+
+// One round of a batch: with restart, systems that have finished start
+// again from their initial records; then every system advances up to
+// `steps` steps (0: to its end).
+int batch_round(nbg_batch* b, int64_t steps, bool restart, nbg_batch_stats* stats) {
+  nbg_ctx* ctx = b->ctx;
+  NBG_HIP(hipSetDevice(ctx->device));
+  if (int rc = batch_order(b, steps)) return rc;
+  if (restart)
+    NBG_HIP(nbg_launch_rewind(b->S.desc.p, (int)b->L.desc.size(), b->S.state.p, b->S.state0.p, b->S.jac.p,
+                              b->S.jac0.p, b->S.ints.p, ctx->cu_count, 0));
+  double ms = 0.0;
+  std::vector<float> gms;
+  std::vector<unsigned long long> gun;
+  std::vector<int> ggrid;
+  if (int rc = run_groups(ctx, b->S, b->L, steps, &ms, &gms, &gun, nullptr, &ggrid)) return rc;
+  nbg_batch_stats tmp;
+  return fill_stats(b, ms, gms, gun, ggrid, steps > 0, stats ? stats : &tmp);
+}
+
+}  // namespace
+
+int nbg_batch_run(nbg_batch* b, nbg_batch_stats* stats) {
+  if (!b) return -1;
+  std::lock_guard<std::mutex> lk(b->ctx->mu);
+  return batch_round(b, 0, false, stats);
+}
+
+int nbg_batch_round(nbg_batch* b, int64_t steps, int32_t restart, nbg_batch_stats* stats) {
+  if (!b) return -1;
+  nbg_ctx* ctx = b->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(steps > 0, "steps must be positive");
+  return batch_round(b, steps, restart != 0, stats);
 }
 
 // Autotuner: each candidate launch configuration runs a copy of the batch
@@ -1112,6 +1177,36 @@ int nbg_batch_autotune(nbg_batch* b, double frac, int32_t* best, double* cand_ms
   ctx->max_groups_per_cu = kTune[bi].groups_per_cu;
   if (best) *best = bi;
   return done(0);
+}
+
+int nbg_batch_autotune_round(nbg_batch* b, int64_t steps, int32_t* best, double* cand_ms, int32_t max_cand) {
+  if (!b) return -1;
+  nbg_ctx* ctx = b->ctx;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(steps > 0, "steps must be positive");
+  const int nc = (int)(sizeof(kTune) / sizeof(kTune[0]));
+  const int keep_mode = ctx->coop_mode, keep_max = ctx->coop_max, keep_g = ctx->max_groups_per_cu;
+  int bi = 0;
+  double bms = 0.0;
+  for (int c = 0; c < nc; ++c) {
+    int rc = nbg_batch_reset(b);
+    ctx->coop_mode = kTune[c].coop_mode;
+    ctx->coop_max = kTune[c].coop_max;
+    ctx->max_groups_per_cu = kTune[c].groups_per_cu;
+    nbg_batch_stats st;
+    if (rc == 0) rc = batch_round(b, steps, false, &st);
+    if (rc) {
+      ctx->coop_mode = keep_mode; ctx->coop_max = keep_max; ctx->max_groups_per_cu = keep_g;
+      return rc;
+    }
+    if (cand_ms && c < max_cand) cand_ms[c] = st.kernel_ms;
+    if (c == 0 || st.kernel_ms < bms) { bms = st.kernel_ms; bi = c; }
+  }
+  ctx->coop_mode = kTune[bi].coop_mode;
+  ctx->coop_max = kTune[bi].coop_max;
+  ctx->max_groups_per_cu = kTune[bi].groups_per_cu;
+  if (best) *best = bi;
+  return nbg_batch_reset(b);
 }
 
 int nbg_batch_autotune_candidate(int32_t index, int32_t* coop_mode, int32_t* coop_max, int32_t* groups_per_cu) {
@@ -1470,6 +1565,8 @@ struct nbg_server {
   std::map<int64_t, ServeReq*> finished;
   int64_t next_id = 1;
   int64_t rounds = 0;
+  int64_t units = 0;       // system-steps (main steps + one per recorded transit) over every round
+  double device_ms = 0.0;  // device time over every round
 };
 
 int nbg_server_create(nbg_ctx* ctx, int64_t max_bytes, int64_t chunk_steps, nbg_server** out) {
@@ -1604,8 +1701,12 @@ int nbg_server_step(nbg_server* sv, int32_t* n_finished, double* kernel_ms) {
   B.done_count = sv->done.p;
   B.done_list = sv->done.p + 1;
   double ms = 0.0;
-  if (int rc = run_groups(ctx, sv->S, L, sv->chunk, &ms, nullptr, nullptr, &B)) return rc;
+  std::vector<unsigned long long> gun;
+  if (int rc = run_groups(ctx, sv->S, L, sv->chunk, &ms, nullptr, &gun, &B)) return rc;
   sv->rounds += 1;
+  for (size_t g = 0; g < gun.size() / 13; ++g)
+    for (int n = 2; n < 13; ++n) sv->units += (int64_t)gun[13 * g + n];
+  sv->device_ms += ms;
   if (kernel_ms) *kernel_ms = ms;
   for (ServeReq* r : sv->active) r->steps_run = std::min<int64_t>(r->d.nsteps + 1, r->steps_run + sv->chunk);
   int nd = 0;
@@ -1699,6 +1800,15 @@ int nbg_server_status(nbg_server* sv, int64_t* pending, int64_t* active, int64_t
   if (active) *active = (int64_t)sv->active.size();
   if (finished) *finished = (int64_t)sv->finished.size();
   if (bytes_free) *bytes_free = sv->arena.free_bytes();
+  return 0;
+}
+
+int nbg_server_work(nbg_server* sv, int64_t* system_steps, int64_t* rounds, double* device_ms) {
+  if (!sv) return -1;
+  std::lock_guard<std::mutex> lk(sv->ctx->mu);  // steps update these under the context's lock
+  if (system_steps) *system_steps = sv->units;
+  if (rounds) *rounds = sv->rounds;
+  if (device_ms) *device_ms = sv->device_ms;
   return 0;
 }
 

@@ -1502,8 +1502,12 @@ __device__ void run_system(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int6
 #pragma unroll 1
   for (int64_t is = istep0; is < iend; ++is) {
     if (D.mode == MODE_CARTESIAN) write_snapshot<NB>(S, D, B, is, lane);
+    // dq/dt of the run's last step (every earlier one is overwritten); only
+    // a gradient step forms it: ahl21!(s, h) leaves s.dqdt as it was
+    // (ahl21_no_grad.jl:6-18), and the transit search restores it
+    // (set_state!, timing.jl:25-26)
     const bool last = (is + 1 == D.nsteps);
-    const bool needQ = last && (D.mode == MODE_TRANSITS || D.hlast == 0.0);
+    const bool needQ = grad && last && (D.mode == MODE_TRANSITS || D.hlast == 0.0);
     ahl21_step<NB>(S, D.h, grad, needQ, anyKick, lane);
     if (D.mode == MODE_TRANSITS) {
       if (lane == 0) S.t = D.t0 + ((double)(is + 1) * D.h);
@@ -1512,7 +1516,7 @@ __device__ void run_system(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int6
     }
   }
   const bool done = (iend == D.nsteps);
-  if (done && D.mode == MODE_STEPS && D.hlast != 0.0) ahl21_step<NB>(S, D.hlast, grad, true, anyKick, lane);
+  if (done && D.mode == MODE_STEPS && D.hlast != 0.0) ahl21_step<NB>(S, D.hlast, grad, grad, anyKick, lane);
   store_system<NB>(S, D, B, lane);
   if (lane == 0) ints[NB] = done ? D.nsteps + 1 : iend;
   if (done && D.mode == MODE_TRANSITS && grad) dtdelements<NB>(S, D, B, lane);
@@ -1640,7 +1644,7 @@ __device__ void coop_integrate(Smem<NB>& S, const SysDesc& D, const SysBufs& B, 
 #pragma unroll 1
     for (int64_t is = istep0; is < iend; ++is) {
       const bool last = (is + 1 == D.nsteps);
-      ahl21_step<NB>(S, D.h, grad, last, anyKick, lane);
+      ahl21_step<NB>(S, D.h, grad, grad && last, anyKick, lane);  // dq/dt as run_system forms it
       if (lane == 0) S.t = D.t0 + ((double)(is + 1) * D.h);
       wsync();
       const int ti = D.ti;
@@ -1760,10 +1764,27 @@ __device__ __forceinline__ void dispatch_dtdel(unsigned char* raw, const SysDesc
   }
 }
 
+// Work accounting at the end of one visit of a system (lane 0): the steps
+// it took in this visit plus, with gradients, one unit per transit it
+// recorded (its final gradient step) go to units[n]; the transits and the
+// Newton iterations of the visit to units[0] and units[1].  A system that
+// reached its end in this visit is appended to the done list.
+__device__ inline void visit_done(const SysDesc& D, const int64_t* ints, int64_t before, int64_t tr0, int64_t it0,
+                                  const SysBufs& B, unsigned long long* units) {
+  if (before > D.nsteps) return;  // finished in an earlier visit: nothing ran
+  const int64_t after = ints[D.n];
+  const int64_t steps = (after > D.nsteps ? D.nsteps : after) - before;
+  const int64_t tr = ints[D.n + 1] - tr0, it = ints[D.n + 2] - it0;
+  atomicAdd(&units[D.n], (unsigned long long)(steps + (D.grad ? tr : 0)));
+  if (tr) atomicAdd(&units[0], (unsigned long long)tr);
+  if (it) atomicAdd(&units[1], (unsigned long long)it);
+  if (after > D.nsteps && B.done_list) B.done_list[atomicAdd(B.done_count, 1)] = D.tag;
+}
+
 // Persistent cooperative kernel: each two-wave workgroup takes systems
 // (MODE_TRANSITS, at most HI bodies) from the queues of its group list in
 // order and runs each to completion (or for `chunk` steps), as nbg_run does.
-// units[0..12] counts work units by body count; units[13] collects wait
+// units[0..12] counts work as visit_done does; units[13] collects wait
 // failures.
 template <int HI>
 __global__ __launch_bounds__(2 * WAVE) void nbg_run_coop(GroupList gl, SysBufs B, CoopBufs C, int64_t chunk,
@@ -1793,6 +1814,7 @@ __global__ __launch_bounds__(2 * WAVE) void nbg_run_coop(GroupList gl, SysBufs B
     const SysDesc D = gl.descs[g][idx];
     const int64_t* ints = B.ints + D.off_int;
     const int64_t before = ints[D.n];
+    const int64_t tr0 = ints[D.n + 1], it0 = ints[D.n + 2];
     __syncthreads();
     dispatch_coop<2, HI>(raw[wave], wave, D, B, R, jobJ, chunk, lane, units + 13, C);
     if (wave == 0) {  // unconditionally, so that wave 1 always leaves its loop
@@ -1803,10 +1825,7 @@ __global__ __launch_bounds__(2 * WAVE) void nbg_run_coop(GroupList gl, SysBufs B
     const bool finished = before <= D.nsteps && ints[D.n] > D.nsteps;  // during this visit
     if (wave == 0 && D.grad && finished) dispatch_dtdel<2, HI>(raw[0], D, B, lane);
     __syncthreads();
-    if (threadIdx.x == 0 && finished) {
-      atomicAdd(&units[D.n], (unsigned long long)(D.nsteps + (D.grad ? ints[D.n + 1] : 0)));
-      if (B.done_list) B.done_list[atomicAdd(B.done_count, 1)] = D.tag;
-    }
+    if (threadIdx.x == 0) visit_done(D, ints, before, tr0, it0, B, units);
     __syncthreads();
   }
 }
@@ -1815,9 +1834,10 @@ __global__ __launch_bounds__(2 * WAVE) void nbg_run_coop(GroupList gl, SysBufs B
 // systems from the queues of its group list in order -- its own class first,
 // then classes of fewer bodies, each ordered longest first -- and runs each
 // to completion (or for `chunk` steps).  LDS is sized for HI bodies; Smem<N>
-// grows with N, so every smaller layout fits in it.  `units` (13 entries,
-// indexed by body count) collects the work units this launch completed:
-// main steps plus one per recorded transit when gradients are on.
+// grows with N, so every smaller layout fits in it.  `units` (13 entries)
+// collects the work of every visit (visit_done): by body count, main steps
+// plus one per recorded transit when gradients are on; transits and Newton
+// iterations in entries 0 and 1.
 template <int HI>
 __global__ __launch_bounds__(WAVE) void nbg_run(GroupList gl, SysBufs B, int64_t chunk,
                                                 unsigned long long* __restrict__ units) {
@@ -1841,12 +1861,10 @@ __global__ __launch_bounds__(WAVE) void nbg_run(GroupList gl, SysBufs B, int64_t
     const SysDesc D = gl.descs[g][idx];
     const int64_t* ints = B.ints + D.off_int;
     const int64_t before = ints[D.n];  // step cursor (lane 0 reads and writes it)
+    const int64_t tr0 = ints[D.n + 1], it0 = ints[D.n + 2];
     dispatch_n<2, HI>(raw, D, B, chunk, lane);
     wsync();
-    if (lane == 0 && before <= D.nsteps && ints[D.n] > D.nsteps) {  // finished during this visit
-      atomicAdd(&units[D.n], (unsigned long long)(D.nsteps + (D.grad ? ints[D.n + 1] : 0)));
-      if (B.done_list) B.done_list[atomicAdd(B.done_count, 1)] = D.tag;
-    }
+    if (lane == 0) visit_done(D, ints, before, tr0, it0, B, units);
   }
 }
 

@@ -43,6 +43,36 @@ extern "C" hipError_t nbg_launch_copy(const void* src, void* dst, size_t bytes, 
   return hipGetLastError();
 }
 
+// Rewind of a batch's finished systems (step cursor past the last step) to
+// their initial records -- state, Jacobians and counters -- so that the next
+// round starts them again.  One workgroup per system; transit outputs are
+// left as they are (a restarted run overwrites them as it counts again).
+__global__ __launch_bounds__(256) void nbg_rewind_kernel(const nbg::SysDesc* __restrict__ descs, int nsys,
+                                                         double* __restrict__ state, const double* __restrict__ state0,
+                                                         double* __restrict__ jac, const double* __restrict__ jac0,
+                                                         int64_t* __restrict__ ints) {
+  for (int s = blockIdx.x; s < nsys; s += gridDim.x) {
+    const nbg::SysDesc& D = descs[s];
+    const bool fin = ints[D.off_int + D.n] > D.nsteps;
+    __syncthreads();  // every thread has read the cursor before any clears it
+    if (fin) {
+      for (int64_t q = threadIdx.x; q < nbg::state_len(D.n); q += blockDim.x) state[D.off_state + q] = state0[D.off_state + q];
+      for (int64_t q = threadIdx.x; q < nbg::jac_len(D.n); q += blockDim.x) jac[D.off_jac + q] = jac0[D.off_jac + q];
+      for (int64_t q = threadIdx.x; q < nbg::int_len(D.n); q += blockDim.x) ints[D.off_int + q] = 0;
+    }
+    __syncthreads();
+  }
+}
+
+extern "C" hipError_t nbg_launch_rewind(const nbg::SysDesc* descs, int nsys, double* state, const double* state0,
+                                        double* jac, const double* jac0, int64_t* ints, int cu_count,
+                                        hipStream_t stream) {
+  if (nsys <= 0) return hipSuccess;
+  const int grid = nsys < 8 * cu_count ? nsys : 8 * cu_count;
+  hipLaunchKernelGGL(nbg_rewind_kernel, dim3(grid), dim3(256), 0, stream, descs, nsys, state, state0, jac, jac0, ints);
+  return hipGetLastError();
+}
+
 extern "C" size_t nbg_lds_bytes(int cls) {
   switch (cls) {
     case 9: return nbg::lds_bytes_upto<9>();

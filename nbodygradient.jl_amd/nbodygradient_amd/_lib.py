@@ -68,6 +68,8 @@ def lib():
     L.nbg_batch_autotune_candidate.argtypes = [C.c_int32, I32, I32, I32]
     L.nbg_batch_reset.argtypes = [VP]
     L.nbg_batch_run.argtypes = [VP, C.POINTER(BatchStats)]
+    L.nbg_batch_round.argtypes = [VP, C.c_int64, C.c_int32, C.POINTER(BatchStats)]
+    L.nbg_batch_autotune_round.argtypes = [VP, C.c_int64, I32, D, C.c_int32]
     L.nbg_batch_download.argtypes = [VP, D, D, D, I64, D, D]
     L.nbg_batch_destroy.argtypes = [VP]
     L.nbg_set_launch.argtypes = [VP, C.c_int32, C.c_int32]
@@ -82,6 +84,7 @@ def lib():
     L.nbg_server_finished.argtypes = [VP, I64, C.c_int32, I32]
     L.nbg_server_fetch.argtypes = [VP, C.c_int64, D, D, D, I64, D, D]
     L.nbg_server_status.argtypes = [VP, I64, I64, I64, I64]
+    L.nbg_server_work.argtypes = [VP, I64, I64, D]
     L.nbg_server_footprint.argtypes = [C.c_int32, C.c_int32, C.c_int64]
     L.nbg_server_footprint.restype = C.c_int64
     L.nbg_copy_calibrate.argtypes = [VP, C.c_int64, C.c_int32, D]

@@ -110,6 +110,23 @@ class Batch:
             cands.append({"coop_mode": a.value, "coop_max": b_.value, "groups_per_cu": g.value, "ms": float(ms[k])})
         return {"frac": frac, "best": best.value, "candidates": cands}
 
+    def autotune_round(self, steps):
+        """nbg_batch_autotune_round: time one round of `steps` steps from the
+        initial state under each candidate launch configuration, keep the
+        fastest in the context and reset the batch."""
+        L = _lib.lib()
+        nc = L.nbg_batch_autotune_candidate(-1, None, None, None)
+        ms = np.zeros(nc)
+        best = C.c_int32(-1)
+        self.ctx.check(L.nbg_batch_autotune_round(self._h, C.c_int64(steps), C.byref(best), dptr(ms), nc),
+                       "nbg_batch_autotune_round")
+        cands = []
+        for k in range(nc):
+            a, b_, g = C.c_int32(), C.c_int32(), C.c_int32()
+            L.nbg_batch_autotune_candidate(k, C.byref(a), C.byref(b_), C.byref(g))
+            cands.append({"coop_mode": a.value, "coop_max": b_.value, "groups_per_cu": g.value, "ms": float(ms[k])})
+        return {"round_steps": int(steps), "best": best.value, "candidates": cands}
+
     def _offsets(self):
         nb = self.ntbv
         self._off_tt = np.concatenate([[0], np.cumsum(nb * self.n.astype(np.int64) * self.ntt)])
@@ -120,14 +137,28 @@ class Batch:
     def main_steps(self):
         return int(sum(s.nsteps for s in self.specs))
 
-    def run(self):
-        st = BatchStats()
-        self.ctx.check(_lib.lib().nbg_batch_run(self._h, C.byref(st)), "nbg_batch_run")
+    @staticmethod
+    def _stats(st):
         return {"system_steps": st.system_steps, "transits": st.transits, "newton_iters": st.newton_iters,
                 "kernel_ms": st.kernel_ms,
                 "launches": {c: {"ms": st.launch_ms[c], "grid": st.launch_grid[c],
                                  "units": {n: st.launch_units[c][n] for n in range(13) if st.launch_units[c][n] > 0}}
                              for c in range(13) if st.launch_ms[c] > 0}}
+
+    def run(self):
+        """Integrate every system to the end of its run (nbg_batch_run)."""
+        st = BatchStats()
+        self.ctx.check(_lib.lib().nbg_batch_run(self._h, C.byref(st)), "nbg_batch_run")
+        return self._stats(st)
+
+    def round(self, steps, restart=True):
+        """One round (nbg_batch_round): every system advances at most `steps`
+        steps of its run; with restart, systems whose run had ended start it
+        again from their initial records first.  The stats are this round's."""
+        st = BatchStats()
+        self.ctx.check(_lib.lib().nbg_batch_round(self._h, C.c_int64(steps), int(bool(restart)), C.byref(st)),
+                       "nbg_batch_round")
+        return self._stats(st)
 
     def reset(self):
         self.ctx.check(_lib.lib().nbg_batch_reset(self._h), "nbg_batch_reset")

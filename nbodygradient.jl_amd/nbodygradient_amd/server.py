@@ -83,6 +83,13 @@ class Server:
         self.ctx.check(_lib.lib().nbg_server_status(self._h, *[C.byref(q) for q in vals]), "nbg_server_status")
         return dict(zip(["pending", "active", "finished", "bytes_free"], [q.value for q in vals]))
 
+    def work(self):
+        """System-steps (main steps + one per recorded transit), rounds and
+        device ms over every round so far."""
+        st, rd, ms = C.c_int64(0), C.c_int64(0), C.c_double(0.0)
+        self.ctx.check(_lib.lib().nbg_server_work(self._h, C.byref(st), C.byref(rd), C.byref(ms)), "nbg_server_work")
+        return {"system_steps": st.value, "rounds": rd.value, "device_ms": ms.value}
+
     def close(self):
         if getattr(self, "_h", None):
             _lib.lib().nbg_server_destroy(self._h)
@@ -95,11 +102,13 @@ class Server:
             pass
 
 
-def serve(server, specs, states, arrivals, keep_results=False, log=None):
+def serve(server, specs, states, arrivals, keep_results=False, log=None, deadline=None):
     """Drive `server` in real time: request k is submitted at arrivals[k]
     seconds after the start (a separate thread), rounds run back to back on
     the calling thread.  Returns per-request latency (s, arrival to result
-    fetched), the wall time and, optionally, the results by request index."""
+    fetched; NaN for a request still unfinished when `deadline` seconds
+    have passed, where the loop stops), the wall time and, optionally, the
+    results by request index."""
     order = np.argsort(arrivals, kind="stable")
     n = len(specs)
     submit_t = np.zeros(n)
@@ -132,6 +141,8 @@ def serve(server, specs, states, arrivals, keep_results=False, log=None):
     rounds = 0
     device_ms = 0.0
     while ndone < n and not errors:
+        if deadline is not None and time.perf_counter() - t0 > deadline and submitted.is_set():
+            break
         nf, ms = server.step()
         rounds += 1
         device_ms += ms
@@ -152,4 +163,5 @@ def serve(server, specs, states, arrivals, keep_results=False, log=None):
         raise errors[0]
     wall = time.perf_counter() - t0
     lat = done_t - submit_t
-    return {"latency_s": lat, "wall_s": wall, "rounds": rounds, "device_ms": device_ms, "results": results}
+    return {"latency_s": lat, "wall_s": wall, "rounds": rounds, "device_ms": device_ms, "results": results,
+            "finished": ndone}

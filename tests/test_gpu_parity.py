@@ -132,13 +132,17 @@ def test_steps_many_sizes(n):
     _compare_state(ps, os_)
 
 
-def _transit_run(el, t0, h, tmax, ti=0, grad=True, ntbv=1):
+def _transit_run(el, t0, h, tmax, ti=0, grad=True, ntbv=1, dqdt=None):
     ic = nb.ElementsIC(t0, el.shape[0], el)
     s = nb.State(ic)
+    if dqdt is not None:
+        s.dqdt[:] = dqdt
     tt = nb.TransitTiming(tmax, ic, ti) if ntbv == 1 else nb.TransitParameters(tmax, ic, ti)
     nb.Integrator(h, tmax)(s, tt, grad=grad)
     x, v, jinit = O.init_nbody(t0, el)
     os_ = O.OState(x, v, el[:, 0], t=t0, jac_init=jinit)
+    if dqdt is not None:
+        os_.dqdt[:] = dqdt
     ott, odq, odel, ocount = O.integrate_transits(os_, h, tmax, tt.ntt, ti=ti, grad=grad, ntbv=ntbv)
     return s, tt, os_, ott, odq, odel, ocount
 
@@ -148,7 +152,11 @@ def _transit_run(el, t0, h, tmax, ti=0, grad=True, ntbv=1):
 def test_transit_timing(ti, grad):
     say("gpu: transit timing ti=%d grad=%s" % (ti, grad))
     el, t0 = transit_case()
-    s, tt, os_, ott, odq, odel, ocount = _transit_run(el, t0, 0.04, 120.0, ti=ti, grad=grad)
+    # a nonzero dq/dt on entry: a grad-free run must hand it back unchanged,
+    # as the reference does (ahl21!(s, h) never writes s.dqdt and the transit
+    # search restores it with set_state!, timing.jl:25-26)
+    dq_in = np.linspace(-1.0, 1.0, 7 * el.shape[0])
+    s, tt, os_, ott, odq, odel, ocount = _transit_run(el, t0, 0.04, 120.0, ti=ti, grad=grad, dqdt=dq_in)
     assert _same(tt.count, ocount) and tt.count.sum() > 0
     assert _same(tt.tt, ott)
     if grad:
@@ -157,6 +165,10 @@ def test_transit_timing(ti, grad):
         _compare_state(s, os_)
     else:
         assert _same(s.x, os_.x) and _same(s.v, os_.v)
+        assert _same(s.xerror, os_.xerror) and _same(s.verror, os_.verror)
+        assert _same(os_.dqdt, dq_in) and _same(s.dqdt, dq_in)
+        n7 = 7 * el.shape[0]
+        assert _same(s.jac_step, np.eye(n7)) and _same(s.jac_error, np.zeros((n7, n7)))
 
 
 def test_transit_parameters():
