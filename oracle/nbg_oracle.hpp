@@ -988,6 +988,16 @@ template <class T> void kickfast_ng(State<T>& s, T h) {
 // phic! with Jacobian (ahl21.jl:392-552) and without (ahl21_no_grad.jl:62-104).
 // Pinned by the restated test_phic.jl finite differences.
 // --------------------------------------------------------------------------
+// Test knob (never changed by the parity tests or the CPU baseline): factors
+// on the two 4th-order correction coefficients -- kicked pairs (phic!) and
+// Kepler pairs (phisalpha!) -- so that a test can plant a wrong coefficient
+// and show that the order-of-accuracy check catches it.  1.0 leaves every
+// result bit for bit as it is.
+inline double g_correction_scale[2] = {1.0, 1.0};
+template <class T> T correction_coeff(T c, int which) {
+  return g_correction_scale[which] == 1.0 ? c : c * T(g_correction_scale[which]);
+}
+
 template <class T> void phic(State<T>& s, Derivatives<T>& d, T h) {
   const int n = s.n;
   std::fill(s.a.begin(), s.a.end(), T(0));
@@ -996,7 +1006,7 @@ template <class T> void phic(State<T>& s, Derivatives<T>& d, T h) {
   std::fill(d.dotdadq.begin(), d.dotdadq.end(), T(0));
   std::fill(d.jac_phi.begin(), d.jac_phi.end(), T(0));
   const T GN = G<T>();
-  T coeff = ((h * h * h) / T(36)) * GN;
+  T coeff = correction_coeff(((h * h * h) / T(36)) * GN, 0);
   for (int i = 0; i < n - 1; ++i) {
     const int indi = 7 * i;
     for (int j = i + 1; j < n; ++j) {
@@ -1135,7 +1145,7 @@ template <class T> void phic_ng(State<T>& s, T h) {
         s.A(k, j) += s.m[i] * fac;
       }
     }
-  T coeff = ((h * h * h) / T(36)) * GN;
+  T coeff = correction_coeff(((h * h * h) / T(36)) * GN, 0);
   for (int i = 0; i < n - 1; ++i)
     for (int j = i + 1; j < n; ++j) {
       if (!s.P(i, j)) continue;
@@ -1168,7 +1178,7 @@ template <class T> void phisalpha(State<T>& s, Derivatives<T>& d, T h, T alpha) 
   std::fill(d.dotdadq.begin(), d.dotdadq.end(), T(0));
   for (int k = 0; k < 3; ++k) s.rij[k] = s.aij[k] = T(0);
   const T GN = G<T>();
-  T coeff = ((((alpha * (h * h * h)) / T(96)) * T(2)) * GN);
+  T coeff = correction_coeff((((alpha * (h * h * h)) / T(96)) * T(2)) * GN, 1);
   for (int i = 0; i < n - 1; ++i)
     for (int j = i + 1; j < n; ++j) {
       if (s.P(i, j)) continue;
@@ -1273,7 +1283,7 @@ template <class T> void phisalpha_ng(State<T>& s, T h, T alpha) {
   std::fill(s.a.begin(), s.a.end(), T(0));
   for (int k = 0; k < 3; ++k) s.rij[k] = s.aij[k] = T(0);
   const T GN = G<T>();
-  T coeff = ((((alpha * (h * h * h)) / T(96)) * T(2)) * GN);
+  T coeff = correction_coeff((((alpha * (h * h * h)) / T(96)) * T(2)) * GN, 1);
   for (int i = 0; i < n - 1; ++i)
     for (int j = i + 1; j < n; ++j) {
       if (s.P(i, j)) continue;

@@ -314,6 +314,12 @@ void orc_pair_kepler(const double* x0, const double* v0, double k, double h, int
 }
 
 // ---------------------------------------------------------------- stepping
+// Test knob: see oracle::g_correction_scale.
+void orc_set_correction_scale(double kicked, double kepler) {
+  g_correction_scale[0] = kicked;
+  g_correction_scale[1] = kepler;
+}
+
 int orc_ahl21_step(int n, double h, int grad, double* x, double* v, double* xe, double* ve, const double* m,
                    const uint8_t* pair, double* jac, double* jace, double* dqdt) {
   State<double> s(n);
@@ -457,10 +463,15 @@ int orc_integrate_transits(int n, double h, double tmax, int ti, int grad, int n
 // the transit times with respect to the elements (test_transit_timing.jl):
 // dtde_num[i, k, iq, jq] (n x ntt x 7 x n); element iq = 0..5 (P..Omega),
 // 6 = mass, as in the reference test.  ntbv = 1.
-void orc_fd_dtdelements(int n, double t0, const double* elements, const double* H, double h, double tmax, int ti,
-                        double dq0, long long ntt, double* dtde_num) {
-  const size_t tot = (size_t)n * ntt * 7 * n;
-  std::fill(dtde_num, dtde_num + tot, 0.0);
+// Central finite differences in binary128 of every transit output with
+// respect to the elements, as test_transit_timing.jl and
+// test_transit_parameters.jl:46-80 form them: element (jq, ivary) +- dq0
+// (the mass is varied last, ivary = 1), both runs without gradients, output
+// ntbv x n x ntt x 7 x n (column-major; ntbv = 1 times, 3 times, v_sky and b^2).
+void orc_fd_dtbvdelements(int n, double t0, const double* elements, const double* H, double h, double tmax, int ti,
+                          double dq0, long long ntt, int ntbv, double* out) {
+  const size_t tot = (size_t)ntbv * n * ntt * 7 * n;
+  std::fill(out, out + tot, 0.0);
   std::vector<f128> el((size_t)n * 7), Hq((size_t)n * n);
   for (int q = 0; q < 7 * n; ++q) el[q] = elements[q];
   for (int q = 0; q < n * n; ++q) Hq[q] = H[q];
@@ -479,14 +490,20 @@ void orc_fd_dtdelements(int n, double t0, const double* elements, const double* 
       std::vector<f128> ep = el, em = el;
       ep[jq + (size_t)n * ivary] += (f128)dq0;
       em[jq + (size_t)n * ivary] -= (f128)dq0;
-      Transits<f128> Tp(n, ntt, ti, 1), Tm(n, ntt, ti, 1);
+      Transits<f128> Tp(n, ntt, ti, ntbv), Tm(n, ntt, ti, ntbv);
       run(ep, Tp);
       run(em, Tm);
       for (int i : Tm.occs)
         for (long long k = 0; k < std::min(Tm.count[i], ntt); ++k)
-          dtde_num[(size_t)i + (size_t)n * ((size_t)k + (size_t)ntt * ((size_t)iq + 7 * (size_t)jq))] =
-              (double)((Tp.tt[Tp.itt(0, i, k)] - Tm.tt[Tm.itt(0, i, k)]) / (2 * (f128)dq0));
+          for (int b = 0; b < ntbv; ++b)
+            out[(size_t)b + (size_t)ntbv * ((size_t)i + (size_t)n * ((size_t)k + (size_t)ntt * ((size_t)iq + 7 * (size_t)jq)))] =
+                (double)((Tp.tt[Tp.itt(b, i, k)] - Tm.tt[Tm.itt(b, i, k)]) / (2 * (f128)dq0));
     }
+}
+
+void orc_fd_dtdelements(int n, double t0, const double* elements, const double* H, double h, double tmax, int ti,
+                        double dq0, long long ntt, double* dtde_num) {
+  orc_fd_dtbvdelements(n, t0, elements, H, h, tmax, ti, dq0, ntt, 1, dtde_num);
 }
 
 // CPU baseline: integrate `nsys` independent systems (same layout as the

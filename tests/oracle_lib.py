@@ -324,3 +324,24 @@ def fd_dtdelements(t0, elements, h, tmax, ti, ntt, H=None, dq0=1e-10):
     lib().orc_fd_dtdelements(n, C.c_double(t0), dp(el), dp(H), C.c_double(h), C.c_double(tmax), ti,
                              C.c_double(dq0), C.c_longlong(ntt), dp(out))
     return out
+
+
+def fd_dtbvdelements(t0, elements, h, tmax, ti, ntt, ntbv=3, H=None, dq0=1e-10):
+    """Binary128 central differences of every transit output (ntbv = 1: the
+    times; 3: times, v_sky, b^2) with respect to the elements:
+    ntbv x n x ntt x 7 x n (test_transit_parameters.jl:46-80)."""
+    el = f64(elements)
+    n = el.shape[0]
+    if H is None:
+        H = hierarchy_nested(n)
+    H = f64(H)
+    out = np.zeros((ntbv, n, ntt, 7, n), order="F")
+    lib().orc_fd_dtbvdelements(n, C.c_double(t0), dp(el), dp(H), C.c_double(h), C.c_double(tmax), ti,
+                               C.c_double(dq0), C.c_longlong(ntt), ntbv, dp(out))
+    return out
+
+
+def set_correction_scale(kicked=1.0, kepler=1.0):
+    """Test knob: factors on the 4th-order correction coefficients of phic!
+    (kicked pairs) and phisalpha! (Kepler pairs); 1.0 is the reference."""
+    lib().orc_set_correction_scale(C.c_double(kicked), C.c_double(kepler))

@@ -45,14 +45,34 @@ def test_hierarchy_fully_nested():
     assert np.array_equal(H2, np.array([[-1, 1], [-1, -1]], dtype=float))
 
 
-def test_center_of_mass_at_rest_and_fixture_coordinates():
+def test_center_of_mass_at_rest():
     # The last hierarchy row is the centre of mass: it starts at the origin at
-    # rest.  The Cartesian fixture (test/coordinates.txt) was written from the
-    # same elements; positions agree to the file's printed precision.
+    # rest.
     el = elements()
     x, v, _ = O.init_nbody(7257.93115525, el)
     m = el[:, 0]
     assert np.max(np.abs(x @ m)) < 1e-15 and np.max(np.abs(v @ m)) < 1e-15
+
+
+def test_cartesian_fixture_route():
+    """test/coordinates.txt is the fixture of the Cartesian route
+    (test_initial_conditions.jl:38-48 builds CartesianIC from the file and
+    from the array read from it and checks that both are the same IC).  It is
+    not the init_nbody image of test/elements.txt (the planets' positions
+    differ from it by up to 0.24 AU), so only what the Cartesian route fixes
+    is checked: file and array give the same IC; x, v are the file's columns
+    transposed; jac_init is the identity (InitialConditions.jl:259-282); the
+    masses are those of the elements fixture."""
+    import nbodygradient_amd as nb
+    from refcases import HERE
+    import os
     co = coordinates()
     assert co.shape == (8, 7)
-    assert np.allclose(co[:, 0], m)
+    a = nb.CartesianIC(7257.93115525, 8, os.path.join(HERE, "golden", "coordinates.txt"))
+    b = nb.CartesianIC(7257.93115525, 8, co)
+    xa, va, ja = nb.init_nbody(a)
+    xb, vb, jb = nb.init_nbody(b)
+    assert np.array_equal(xa, xb) and np.array_equal(va, vb) and np.array_equal(ja, jb)
+    assert np.array_equal(xa, co[:, 1:4].T) and np.array_equal(va, co[:, 4:7].T)
+    assert np.array_equal(ja, np.eye(56))
+    assert np.array_equal(a.m, elements()[:, 0])

@@ -85,3 +85,22 @@ def test_transit_parameters_times_match_transit_timing():
         k = int(c3[i])
         if k:
             assert np.all(tt3[1, i, :k] > 0) and np.all(tt3[2, i, :k] >= 0)
+
+
+def test_grad_free_run_keeps_dqdt_and_jacobian():
+    """(intr)(s, tt; grad=false) leaves s.dqdt, s.jac_step and s.jac_error as
+    they were: ahl21!(s, h) never writes them (ahl21_no_grad.jl:6-18) and
+    detect_transits! restores the whole state after each search (set_state!,
+    timing.jl:25-26, Integrator.jl:123-134), although findtransit! steps with
+    the Jacobian (timing.jl:57)."""
+    el, t0 = transit_case()
+    x, v, jac_init = O.init_nbody(t0, el)
+    s = O.OState(x, v, el[:, 0], t=t0, jac_init=jac_init)
+    n7 = 7 * el.shape[0]
+    dq_in = np.linspace(-1.0, 1.0, n7)
+    s.dqdt[:] = dq_in
+    ntt = O.ntt_rule(120.0, el)
+    _, _, _, count = O.integrate_transits(s, 0.04, 120.0, ntt, grad=False)
+    assert count.sum() > 0
+    assert np.array_equal(s.dqdt, dq_in)
+    assert np.array_equal(s.jac_step, np.eye(n7)) and np.array_equal(s.jac_error, np.zeros((n7, n7)))
