@@ -104,3 +104,20 @@ def test_grad_free_run_keeps_dqdt_and_jacobian():
     assert count.sum() > 0
     assert np.array_equal(s.dqdt, dq_in)
     assert np.array_equal(s.jac_step, np.eye(n7)) and np.array_equal(s.jac_error, np.zeros((n7, n7)))
+
+
+def test_c1_golden_fixture_is_a_transit_record():
+    """The committed C1 oracle run (tests/golden/c1_oracle.npz) is what its
+    generator says: each planet transits about every period over 4000 days,
+    and consecutive times are spaced by its period within 2 %."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c1_oracle.npz"))
+    from nbodygradient_amd import workloads as W
+    spec = W.c1()
+    P = spec.elements[:, 1]
+    for i in range(1, spec.n):
+        c = int(g["count"][i])
+        assert abs(c - 4000.0 / P[i]) <= 2
+        d = np.diff(g["tt"][i, :c])
+        assert np.all(np.abs(d / P[i] - 1.0) < 0.02)
+    assert g["dtdelements_sel"].shape == (8, (spec.ntt() + 24) // 25, 7, 8)
