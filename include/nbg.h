@@ -230,9 +230,12 @@ int nbg_batch_transits(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const doubl
  * is one (intr)(s, tt; grad) run as in nbg_integrate_transits.  A server owns
  * a device arena of max_bytes; requests are admitted first come first served
  * while their HBM footprint (nbg_server_footprint) fits, and every call of
- * nbg_server_step advances all admitted requests by up to chunk_steps steps
- * in one set of persistent launches, so requests that arrive while others run
- * join at the next round.  Results are bitwise those of the one-system call.
+ * nbg_server_step advances all admitted requests in one set of persistent
+ * launches, so requests that arrive while others run join at the next round.
+ * Rounds are balanced in time: the requests of the largest body count among
+ * the admitted ones advance up to chunk_steps steps, the others up to
+ * chunk_steps times the ratio of the two body counts' per-step costs.
+ * Results are bitwise those of the one-system call.
  * submit, finished, fetch and status may be called from other threads while
  * a step runs; steps are serialised with the context's other calls. */
 typedef struct nbg_server nbg_server;

@@ -104,11 +104,15 @@ void group_layout(Layout& L);
 // (transit search included) measured on MI355X for n = 3, 5, 7, 9 and fitted
 // by a quadratic.  It only orders work (longest first), shares the GPU
 // between launches and picks the systems worth a second wave.
+double step_cost(int nb) {
+  const double n = nb;
+  return std::max(10.0, 3.0 * n * n + 39.0 * n - 55.0);
+}
+
 double sys_cost(const Layout& L, int i) {
   const auto& d = L.desc[i];
-  const double n = d.n;
   const double steps = L.cost.empty() ? (double)(d.nsteps + 1) : L.cost[i];
-  return steps * std::max(10.0, 3.0 * n * n + 39.0 * n - 55.0);
+  return steps * step_cost(d.n);
 }
 
 void finalize_layout(Layout& L) {
@@ -1673,15 +1677,23 @@ int nbg_server_step(nbg_server* sv, int32_t* n_finished, double* kernel_ms) {
   if (n_finished) *n_finished = 0;
   if (kernel_ms) *kernel_ms = 0.0;
   if (sv->active.empty()) return 0;
-  // one round over every active request
+  // one round over every active request.  Rounds are balanced in time
+  // rather than in steps: the costliest body count among the active
+  // requests advances `chunk` steps, and a request whose steps cost less
+  // (per the per-step cost model) proportionally more, so that no workgroup
+  // idles through most of a round waiting for the costliest ones.
   Layout L;
   L.desc.resize(sv->active.size());
   L.cost.resize(sv->active.size());
+  int nmax = 2;
+  for (ServeReq* r : sv->active) nmax = std::max(nmax, r->d.n);
+  const double cmax = step_cost(nmax);
   for (size_t k = 0; k < sv->active.size(); ++k) {
     ServeReq* r = sv->active[k];
     L.desc[k] = r->d;
     L.desc[k].tag = (int32_t)k;
-    L.cost[k] = (double)std::max<int64_t>(1, std::min<int64_t>(sv->chunk, r->d.nsteps + 1 - r->steps_run));
+    L.desc[k].chunk = std::max<int64_t>(1, std::llround((double)sv->chunk * cmax / step_cost(r->d.n)));
+    L.cost[k] = (double)std::max<int64_t>(1, std::min<int64_t>(L.desc[k].chunk, r->d.nsteps + 1 - r->steps_run));
   }
   group_layout(L);
   const size_t na = sv->active.size();
@@ -1708,7 +1720,10 @@ int nbg_server_step(nbg_server* sv, int32_t* n_finished, double* kernel_ms) {
     for (int n = 2; n < 13; ++n) sv->units += (int64_t)gun[13 * g + n];
   sv->device_ms += ms;
   if (kernel_ms) *kernel_ms = ms;
-  for (ServeReq* r : sv->active) r->steps_run = std::min<int64_t>(r->d.nsteps + 1, r->steps_run + sv->chunk);
+  for (size_t k = 0; k < na; ++k) {
+    ServeReq* r = sv->active[k];
+    r->steps_run = std::min<int64_t>(r->d.nsteps + 1, r->steps_run + L.desc[k].chunk);
+  }
   int nd = 0;
   NBG_HIP(hipMemcpy(&nd, sv->done.p, sizeof(int), hipMemcpyDeviceToHost));
   std::vector<int> tags((size_t)std::max(nd, 0));

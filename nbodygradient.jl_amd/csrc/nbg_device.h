@@ -1816,7 +1816,7 @@ __global__ __launch_bounds__(2 * WAVE) void nbg_run_coop(GroupList gl, SysBufs B
     const int64_t before = ints[D.n];
     const int64_t tr0 = ints[D.n + 1], it0 = ints[D.n + 2];
     __syncthreads();
-    dispatch_coop<2, HI>(raw[wave], wave, D, B, R, jobJ, chunk, lane, units + 13, C);
+    dispatch_coop<2, HI>(raw[wave], wave, D, B, R, jobJ, D.chunk > 0 ? D.chunk : chunk, lane, units + 13, C);
     if (wave == 0) {  // unconditionally, so that wave 1 always leaves its loop
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       if (lane == 0) ring_store(&R.done, 1);
@@ -1862,7 +1862,7 @@ __global__ __launch_bounds__(WAVE) void nbg_run(GroupList gl, SysBufs B, int64_t
     const int64_t* ints = B.ints + D.off_int;
     const int64_t before = ints[D.n];  // step cursor (lane 0 reads and writes it)
     const int64_t tr0 = ints[D.n + 1], it0 = ints[D.n + 2];
-    dispatch_n<2, HI>(raw, D, B, chunk, lane);
+    dispatch_n<2, HI>(raw, D, B, D.chunk > 0 ? D.chunk : chunk, lane);
     wsync();
     if (lane == 0) visit_done(D, ints, before, tr0, it0, B, units);
   }

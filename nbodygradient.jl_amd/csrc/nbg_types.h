@@ -46,6 +46,7 @@ struct SysDesc {
   int32_t tag;       // reported in SysBufs.done_list when the system finishes
   int32_t traj;      // MODE_CARTESIAN: TRAJ_* flags of the optional records
   int64_t nsteps;    // full steps in the run
+  int64_t chunk;     // > 0: at most this many steps per visit (overrides the launch's chunk)
   int64_t ntt;
   double h;          // signed step used by the loop
   double hstep;      // intr.h (unsigned) for the first transit-time guess
