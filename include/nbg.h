@@ -72,6 +72,25 @@ int nbg_integrate_transits(nbg_ctx* ctx, int32_t n, double h, double tmax, int32
                            const double* jac_init, int64_t ntt, double* tt, double* dtdq0, double* dtdelements,
                            int64_t* count);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): nbg_integrate_transits with the same arguments and the
+ * same results, which calls no hipMalloc, hipFree or other allocating
+ * routine.  Its device memory is one array of NBG_NOALLOC_SCRATCH_BYTES that
+ * the library holds on every GPU it runs on until the process ends; a system
+ * whose record (state, three 7n x 7n Jacobians, work area, tt, dtdq0 and
+ * dtdelements: about 16 * ntbv * n * ntt * 7n + 24 * (7n)^2 bytes) does not
+ * fit is rejected with an error before any device work.  Every call on one
+ * GPU uses that GPU's one array: calls that overlap on the same GPU (through
+ * different contexts, or other processes' threads sharing the library) race;
+ * calls through one context are serialised. */
+#define NBG_NOALLOC_SCRATCH_BYTES (32u << 20)
+int nbg_integrate_transits_noalloc(nbg_ctx* ctx, int32_t n, double h, double tmax, int32_t ti, int32_t grad,
+                                   int32_t ntbv, double* t, double* x, double* v, double* xerror, double* verror,
+                                   const double* m, const uint8_t* pair, double* jac_step, double* jac_error,
+                                   double* dqdt, const double* jac_init, int64_t ntt, double* tt, double* dtdq0,
+                                   double* dtdelements, int64_t* count);
+
 /* (intr::Integrator)(s, o::CartesianOutput) -- src/outputs/Outputs.jl:26-49:
  * nstep steps of h * check_step(*t, tmax) (tmax = intr.tmax sets only the
  * direction), recording the state before each step k = 0..nstep-1:

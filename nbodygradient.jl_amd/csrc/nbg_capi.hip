@@ -32,6 +32,13 @@ extern "C" int64_t nbg_coop_stride();
 extern "C" hipError_t nbg_launch_rewind(const nbg::SysDesc* descs, int nsys, double* state, const double* state0,
                                         double* jac, const double* jac0, int64_t* ints, int cu_count,
                                         hipStream_t stream);
+// Device memory of nbg_integrate_transits_noalloc: a file-scope array in this
+// translation unit's code object, present on every GPU the library runs on
+// until the process ends.  Host code never uses its name as an address (on
+// the host it stands for a host-side shadow); it uses the device address that
+// hipGetSymbolAddress returns for each device (scratch_dev below).
+static __device__ __align__(256) unsigned char scratch[NBG_NOALLOC_SCRATCH_BYTES];
+
 // Initial conditions kernel (nbg_ic.hip)
 extern "C" hipError_t nbg_launch_ic(const void* descs, int nsys, const double* in, double* state, double* jac,
                                     hipStream_t stream);
@@ -191,6 +198,7 @@ struct nbg_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   hipEvent_t gev[2 * 16] = {};  // start/end of each body-count launch (timed on its own stream)
   Store single;  // reused by the single-system entry points
+  unsigned char* scratch_dev = nullptr;  // device address of `scratch` on this context's device
 };
 
 struct nbg_batch {
@@ -614,6 +622,20 @@ int nbg_ctx_create(nbg_ctx** out, int device) {
   if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) { delete ctx; return -3; }
   for (auto& e : ctx->gev)
     if (hipEventCreate(&e) != hipSuccess) { delete ctx; return -3; }
+  // the device address of the no-allocation scratch array on this device
+  // (each GPU holds its own copy at its own address)
+  {
+    static std::mutex mu;
+    static std::map<int, unsigned char*> addr;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = addr.find(device);
+    if (it == addr.end()) {
+      void* p = nullptr;
+      if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) { delete ctx; return -3; }
+      it = addr.emplace(device, static_cast<unsigned char*>(p)).first;
+    }
+    ctx->scratch_dev = it->second;
+  }
   *out = ctx;
   return 0;
 }
@@ -773,6 +795,126 @@ int nbg_integrate_transits(nbg_ctx* ctx, int32_t n, double h, double tmax, int32
   for (int q = 0; q < n; ++q) count[q] = 0;
   return run_single(ctx, d, x, v, xerror, verror, m, pair, jac_step, jac_error, dqdt, jac_init, t, tt, dtdq0,
                     dtdelements, count);
+}
+
+int nbg_integrate_transits_noalloc(nbg_ctx* ctx, int32_t n, double h, double tmax, int32_t ti, int32_t grad,
+                                   int32_t ntbv, double* t, double* x, double* v, double* xerror, double* verror,
+                                   const double* m, const uint8_t* pair, double* jac_step, double* jac_error,
+                                   double* dqdt, const double* jac_init, int64_t ntt, double* tt, double* dtdq0,
+                                   double* dtdelements, int64_t* count) {
+  if (int rc = check_common(ctx, n, h, x, v, m)) return rc;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(t && tt && count, "t, tt and count are required");
+  NBG_ARG(ti >= 0 && ti < n, "ti out of range");
+  NBG_ARG(ntbv == 1 || ntbv == 3, "ntbv must be 1 (TransitTiming) or 3 (TransitParameters)");
+  NBG_ARG(ntt >= 0, "ntt must be >= 0");
+  NBG_ARG(!grad || (jac_step && jac_error && dqdt && jac_init && dtdq0 && dtdelements),
+          "grad run needs jac_step, jac_error, dqdt, jac_init, dtdq0 and dtdelements");
+  NBG_ARG(!grad || mass_rows_ok(n, jac_step, jac_error), "jac_step mass rows must be unit rows, jac_error's zero");
+  NBG_ARG(ctx->scratch_dev != nullptr, "no scratch address for this device");
+  // the system's record, piece by piece, in the scratch array (byte offsets
+  // rounded up to 256)
+  const int64_t s7 = 7 * (int64_t)n, NN = s7 * s7;
+  const int64_t ntv = (int64_t)ntbv * n * ntt, ndq = ntv * s7;
+  auto up = [](int64_t b) { return (b + 255) / 256 * 256; };
+  int64_t o = 0;
+  const int64_t o_desc = o; o += up((int64_t)sizeof(nbg::SysDesc));
+  const int64_t o_queue = o; o += up(16);
+  const int64_t o_units = o; o += up(14 * 8);
+  const int64_t o_state = o; o += up(8 * nbg::state_len(n));
+  const int64_t o_jac = o; o += up(8 * nbg::jac_len(n));
+  const int64_t o_ws = o; o += up(8 * nbg::ws_len(n));
+  const int64_t o_tt = o; o += up(8 * ntv);
+  const int64_t o_dq = o; o += up(8 * ndq);
+  const int64_t o_del = o; o += up(8 * ndq);
+  const int64_t o_int = o; o += up(8 * nbg::int_len(n));
+  const int64_t o_pair = o; o += up((int64_t)n * n);
+  if (o > (int64_t)NBG_NOALLOC_SCRATCH_BYTES) {
+    ctx->err = "nbg_integrate_transits_noalloc: this system needs " + std::to_string(o) +
+               " bytes of device scratch, more than the " + std::to_string((long long)NBG_NOALLOC_SCRATCH_BYTES) +
+               " the library holds (n, ntt or ntbv too large); use nbg_integrate_transits";
+    return -1;
+  }
+  const double t0 = *t;
+  nbg::SysDesc d{};
+  d.n = n; d.mode = nbg::MODE_TRANSITS; d.grad = grad ? 1 : 0; d.ti = ti; d.ntbv = ntbv;
+  d.any_kick = any_kick(n, pair);
+  d.nsteps = std::llabs(jround(tmax / h));
+  d.ntt = ntt; d.h = h * check_step(t0, tmax + t0); d.hstep = h; d.hlast = 0.0; d.t0 = t0;
+  // every buffer pointer starts at its own piece, so every offset is zero
+  d.off_state = d.off_jac = d.off_ws = d.off_tt = d.off_dq = d.off_del = d.off_int = d.off_pair = d.off_traj = 0;
+  d.tag = 0;
+  unsigned char* base = ctx->scratch_dev;
+  auto dev = [&](int64_t off) { return base + off; };
+  NBG_HIP(hipSetDevice(ctx->device));
+  hipStream_t st = ctx->streams[0];
+  const size_t n3 = 3 * (size_t)n;
+  // inputs: descriptor, state (x, v, Kahan errors, m, dq/dt, gsave = 0, t),
+  // Jacobians, counters (count starts at 0), pair mask, the output arrays'
+  // current contents (entries beyond count stay as given)
+  NBG_HIP(hipMemcpyAsync(dev(o_desc), &d, sizeof(d), hipMemcpyHostToDevice, st));
+  NBG_HIP(hipMemsetAsync(dev(o_queue), 0, 16, st));
+  NBG_HIP(hipMemsetAsync(dev(o_units), 0, 14 * 8, st));
+  NBG_HIP(hipMemsetAsync(dev(o_state), 0, 8 * nbg::state_len(n), st));
+  NBG_HIP(hipMemcpyAsync(dev(o_state + 8 * nbg::off_x(n)), x, 8 * n3, hipMemcpyHostToDevice, st));
+  NBG_HIP(hipMemcpyAsync(dev(o_state + 8 * nbg::off_v(n)), v, 8 * n3, hipMemcpyHostToDevice, st));
+  if (xerror) NBG_HIP(hipMemcpyAsync(dev(o_state + 8 * nbg::off_xe(n)), xerror, 8 * n3, hipMemcpyHostToDevice, st));
+  if (verror) NBG_HIP(hipMemcpyAsync(dev(o_state + 8 * nbg::off_ve(n)), verror, 8 * n3, hipMemcpyHostToDevice, st));
+  NBG_HIP(hipMemcpyAsync(dev(o_state + 8 * nbg::off_m(n)), m, 8 * (size_t)n, hipMemcpyHostToDevice, st));
+  if (dqdt) NBG_HIP(hipMemcpyAsync(dev(o_state + 8 * nbg::off_dqdt(n)), dqdt, 8 * s7, hipMemcpyHostToDevice, st));
+  NBG_HIP(hipMemcpyAsync(dev(o_state + 8 * nbg::off_t(n)), &t0, 8, hipMemcpyHostToDevice, st));
+  if (grad) {
+    NBG_HIP(hipMemcpyAsync(dev(o_jac), jac_step, 8 * NN, hipMemcpyHostToDevice, st));
+    NBG_HIP(hipMemcpyAsync(dev(o_jac + 8 * NN), jac_error, 8 * NN, hipMemcpyHostToDevice, st));
+    NBG_HIP(hipMemcpyAsync(dev(o_jac + 16 * NN), jac_init, 8 * NN, hipMemcpyHostToDevice, st));
+  }
+  NBG_HIP(hipMemsetAsync(dev(o_int), 0, 8 * nbg::int_len(n), st));
+  if (pair) NBG_HIP(hipMemcpyAsync(dev(o_pair), pair, (size_t)n * n, hipMemcpyHostToDevice, st));
+  else NBG_HIP(hipMemsetAsync(dev(o_pair), 0, (size_t)n * n, st));
+  if (ntv) NBG_HIP(hipMemcpyAsync(dev(o_tt), tt, 8 * ntv, hipMemcpyHostToDevice, st));
+  if (ndq) {
+    if (dtdq0) NBG_HIP(hipMemcpyAsync(dev(o_dq), dtdq0, 8 * ndq, hipMemcpyHostToDevice, st));
+    else NBG_HIP(hipMemsetAsync(dev(o_dq), 0, 8 * ndq, st));
+    if (dtdelements) NBG_HIP(hipMemcpyAsync(dev(o_del), dtdelements, 8 * ndq, hipMemcpyHostToDevice, st));
+    else NBG_HIP(hipMemsetAsync(dev(o_del), 0, 8 * ndq, st));
+  }
+  // one persistent workgroup of the system's launch class takes the system
+  nbg::GroupList gl{};
+  gl.ng = 1;
+  gl.nsys[0] = 1;
+  gl.descs[0] = reinterpret_cast<const nbg::SysDesc*>(dev(o_desc));
+  gl.queue[0] = reinterpret_cast<int*>(dev(o_queue));
+  nbg::SysBufs B;
+  B.state = reinterpret_cast<double*>(dev(o_state));
+  B.jac = reinterpret_cast<double*>(dev(o_jac));
+  B.ws = reinterpret_cast<double*>(dev(o_ws));
+  B.tt = reinterpret_cast<double*>(dev(o_tt));
+  B.dq = reinterpret_cast<double*>(dev(o_dq));
+  B.del = reinterpret_cast<double*>(dev(o_del));
+  B.ints = reinterpret_cast<int64_t*>(dev(o_int));
+  B.pair = dev(o_pair);
+  B.traj = nullptr;
+  B.done_count = nullptr;
+  B.done_list = nullptr;
+  NBG_HIP(nbg_launch_group(nbg_class_of(n), gl, B, 0, 1, reinterpret_cast<unsigned long long*>(dev(o_units)), st));
+  // outputs, straight into the caller's arrays
+  NBG_HIP(hipMemcpyAsync(x, dev(o_state + 8 * nbg::off_x(n)), 8 * n3, hipMemcpyDeviceToHost, st));
+  NBG_HIP(hipMemcpyAsync(v, dev(o_state + 8 * nbg::off_v(n)), 8 * n3, hipMemcpyDeviceToHost, st));
+  if (xerror) NBG_HIP(hipMemcpyAsync(xerror, dev(o_state + 8 * nbg::off_xe(n)), 8 * n3, hipMemcpyDeviceToHost, st));
+  if (verror) NBG_HIP(hipMemcpyAsync(verror, dev(o_state + 8 * nbg::off_ve(n)), 8 * n3, hipMemcpyDeviceToHost, st));
+  NBG_HIP(hipMemcpyAsync(t, dev(o_state + 8 * nbg::off_t(n)), 8, hipMemcpyDeviceToHost, st));
+  NBG_HIP(hipMemcpyAsync(count, dev(o_int), 8 * (size_t)n, hipMemcpyDeviceToHost, st));
+  if (grad) {
+    NBG_HIP(hipMemcpyAsync(jac_step, dev(o_jac), 8 * NN, hipMemcpyDeviceToHost, st));
+    NBG_HIP(hipMemcpyAsync(jac_error, dev(o_jac + 8 * NN), 8 * NN, hipMemcpyDeviceToHost, st));
+    NBG_HIP(hipMemcpyAsync(dqdt, dev(o_state + 8 * nbg::off_dqdt(n)), 8 * s7, hipMemcpyDeviceToHost, st));
+  }
+  if (ntv) NBG_HIP(hipMemcpyAsync(tt, dev(o_tt), 8 * ntv, hipMemcpyDeviceToHost, st));
+  if (ndq && dtdq0) NBG_HIP(hipMemcpyAsync(dtdq0, dev(o_dq), 8 * ndq, hipMemcpyDeviceToHost, st));
+  if (ndq && dtdelements) NBG_HIP(hipMemcpyAsync(dtdelements, dev(o_del), 8 * ndq, hipMemcpyDeviceToHost, st));
+  NBG_HIP(hipStreamSynchronize(st));
+  NBG_HIP(hipGetLastError());
+  return 0;
 }
 
 int nbg_integrate_cartesian(nbg_ctx* ctx, int32_t n, double h, double tmax, int64_t nstep, int32_t grad,
