@@ -48,7 +48,7 @@ FP64_PEAK_TFLOPS = 78.6
 # from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the same counters'
 # reading of nbg_copy_kernel of known size (the figures and the commands are in
 # profiles/nbg_run_c2_round_counters.md)
-HBM_BYTES_PER_UNIT = None
+HBM_BYTES_PER_UNIT = 373.3e3
 METRIC = "system-steps/s w/ Jacobian, batched vs plain GPU; p50/p99 request latency"
 T_START = time.perf_counter()
 
@@ -242,9 +242,9 @@ def main():
     name, cus = ctx.info()
     log("device %s, %d CUs; %d systems prepared" % (name, cus, len(specs)))
 
-    # prime code loading with a tiny batch (not part of the workload)
+    # prime code loading with a few steps of a tiny batch (not part of the workload)
     pb = Batch(specs[:8], states[:8])
-    pb.run()
+    pb.round(20)
     pb.close()
 
     K = args.round_steps
@@ -324,9 +324,10 @@ def main():
         "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": D["kernel"], "launch_ms": D["ms_per_round"], "units_per_launch": D["units"] / args.steps,
-                     "note": "FP64 vector (VALU) peak 78.6 TFLOP/s; no MFMA in the kernel; W(n) = 2016 n^2 (n-1) + "
-                             "126 n^3 flops per system-step; traffic = HBM bytes per launch from calibrated "
-                             "rocprofv3 FETCH_SIZE + WRITE_SIZE"},
+                     "note": "FP64 vector (VALU) peak 78.6 TFLOP/s; no MFMA in the kernel (a measured pair-update "
+                             "probe ran 28 % slower on the FP64 matrix cores); W(n) = 2016 n^2 (n-1) + 126 n^3 flops "
+                             "per system-step; traffic = 373.3 KB per system-step (calibrated rocprofv3 FETCH_SIZE + "
+                             "WRITE_SIZE of these rounds) x the launch's system-steps per round"},
         "launches": lrep,
         "transits_with_gradients_per_s": trans / t_timed,
     }

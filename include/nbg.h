@@ -307,6 +307,16 @@ int nbg_set_coop_wait(nbg_ctx* ctx, int64_t polls);
  * reading for HBM byte counters and the box's achieved copy bandwidth. */
 int nbg_copy_calibrate(nbg_ctx* ctx, int64_t bytes, int32_t reps, double* ms);
 
+/* Diagnostic only (no counterpart in the reference): the Jacobian row update
+ * of one Kepler pair at n = 9 (12 rows x 63 columns, J + jac_ij * J with
+ * Kahan compensation, ahl21.jl:31-42) repeated `reps` times by each of `grid`
+ * one-wave workgroups on synthetic data in LDS, with mode 0 on the VALU
+ * (168 fused multiply-adds per lane, as the integrator does) or mode 1 on
+ * the FP64 matrix cores (16 v_mfma_f64_16x16x4f64 per update); *ms = kernel
+ * time, j_out (12 x 63 row-major, may be NULL) = workgroup 0's final rows.
+ * The measurement behind the choice of unit for the chain products. */
+int nbg_probe_row_update(nbg_ctx* ctx, int32_t mode, int32_t reps, int32_t grid, double* ms, double* j_out);
+
 /* Diagnostic only: evaluates the device pair formulas (nbg_pair.h) on the
  * host CPU so the test suite can compare them bit for bit with the CPU
  * restatement without a GPU.  Not used by any integration path.

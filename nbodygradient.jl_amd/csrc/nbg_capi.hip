@@ -39,6 +39,7 @@ extern "C" hipError_t nbg_launch_rewind(const nbg::SysDesc* descs, int nsys, dou
 // hipGetSymbolAddress returns for each device (scratch_dev below).
 static __device__ __align__(256) unsigned char scratch[NBG_NOALLOC_SCRATCH_BYTES];
 
+extern "C" hipError_t nbg_launch_probe(int mode, int reps, int grid, double* out, hipStream_t stream);
 // Initial conditions kernel (nbg_ic.hip)
 extern "C" hipError_t nbg_launch_ic(const void* descs, int nsys, const double* in, double* state, double* jac,
                                     hipStream_t stream);
@@ -1997,6 +1998,31 @@ int nbg_copy_calibrate(nbg_ctx* ctx, int64_t bytes, int32_t reps, double* ms) {
     return -2;
   }
   *ms = t / reps;
+  return 0;
+}
+
+int nbg_probe_row_update(nbg_ctx* ctx, int32_t mode, int32_t reps, int32_t grid, double* ms, double* j_out) {
+  if (!ctx) return -1;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(mode == 0 || mode == 1, "mode must be 0 (VALU) or 1 (MFMA)");
+  NBG_ARG(reps > 0 && grid > 0 && ms, "reps and grid must be positive, ms non-null");
+  NBG_HIP(hipSetDevice(ctx->device));
+  DevVec<double> out;
+  NBG_HIP(out.ensure(12 * 63));
+  hipStream_t st = ctx->streams[0];
+  NBG_HIP(hipEventRecord(ctx->ev0, st));
+  hipError_t e = nbg_launch_probe(mode, reps, grid, out.p, st);
+  if (e == hipSuccess) e = hipEventRecord(ctx->ev1, st);
+  if (e == hipSuccess) e = hipEventSynchronize(ctx->ev1);
+  float t = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, ctx->ev0, ctx->ev1);
+  if (e == hipSuccess && j_out) e = hipMemcpy(j_out, out.p, 12 * 63 * sizeof(double), hipMemcpyDeviceToHost);
+  out.release();
+  if (e != hipSuccess) {
+    ctx->err = std::string("row-update probe: ") + hipGetErrorString(e);
+    return -2;
+  }
+  *ms = t;
   return 0;
 }
 
