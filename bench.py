@@ -24,9 +24,10 @@ time budget would not fit it):
   fit_batch    a subset of the 16384-vector fit batch (perturbations of one
                6-body system: star + 5 planets) at its full tmax = 3000 d, initial
                conditions computed on the GPU;
-  serving      request traces of integrate-plus-gradient calls at full tmax:
-               the arrivals at 2000/s (overload), and at a rate below the
-               measured capacity (latency p50/p99).
+  serving      integrate-plus-gradient requests of the serving trace at full
+               tmax: the server's capacity kept full (closed loop), the
+               trace's 2000/s arrivals (overload), and arrivals at half the
+               capacity (latency p50/p99).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--round-steps S]
                        [--budget SECONDS] [--no-side] ...
@@ -192,9 +193,35 @@ def serving(trace_count, rate, deadline, chunk, log_fn, label):
            "max_finished_s": float(lat[fin].max()) if nfin else None,
            "completed_requests_per_s": nfin / wall,
            "system_steps": work["system_steps"], "system_steps_per_s": work["system_steps"] / wall,
+           "device_busy_frac": work["device_ms"] / 1e3 / wall,
            "mean_main_steps_per_request": float(steps_req.mean()), "rounds": work["rounds"]}
     log_fn("serving %s: %d/%d finished in %.1f s, p50 %s, p99 %s, %.3g system-steps/s" %
            (label, nfin, len(sspecs), wall, out["p50_s"], out["p99_s"], out["system_steps_per_s"]))
+    return out
+
+
+def serving_capacity(chunk, log_fn, concurrency=2048, duration=15.0):
+    """The server kept full (closed loop, `concurrency` requests of the
+    serving trace in flight, each finished one replaced at once) for
+    `duration` seconds: its sustained system-step rate, and that rate over the
+    trace's mean work per request as requests/s."""
+    from nbodygradient_amd import workloads as Wk
+    from nbodygradient_amd.server import Server, saturate
+    trace = Wk.c4(count=4096)
+    sspecs = [sp for _, sp in trace]
+    sstates = Wk.initial_states(sspecs)
+    sv = Server(max_bytes=64 << 30, chunk=chunk)
+    r = saturate(sv, sspecs, sstates, concurrency, duration)
+    sv.close()
+    per_req = float(np.mean([sp.nsteps + 0.0 for sp in sspecs]))
+    rate = r["system_steps"] / r["wall_s"]
+    out = {"what": "closed loop: %d requests of the serving trace (n ~ U{2..10}, tmax log-uniform in [100, 8000] d) "
+                   "always in flight, each finished one replaced at once, for %g s" % (concurrency, duration),
+           "system_steps_per_s": rate, "mean_main_steps_per_request": per_req,
+           "requests_per_s": rate / per_req, "completed": r["completed"], "rounds": r["rounds"],
+           "device_busy_frac": r["device_ms"] / 1e3 / r["wall_s"], "chunk_steps": chunk}
+    log_fn("serving capacity: %.3g system-steps/s = %.1f requests/s (%d completed, %d rounds, device busy %.0f %%)" %
+           (rate, out["requests_per_s"], r["completed"], r["rounds"], 100 * out["device_busy_frac"]))
     return out
 
 
@@ -386,23 +413,20 @@ def main():
         log("cpu 1 core: %.4g system-steps/s" % (done1 / secs1))
     if args.fit > 0 and fits(40, "fit_batch"):
         out["fit_batch"] = fit_batch(args.fit, args.fit_tmax_scale)
-    if args.serve > 0 and fits(40, "serving_overload"):
+    if args.serve > 0 and fits(30, "serving_capacity"):
         ctx.set_coop(1, 128)  # serving rounds keep the defaults, whatever the autotuner chose
         ctx.set_launch(0)
-        # 2000 requests at the trace's 2000/s: far more work than the GPU
-        # completes, so the server's system-step rate is its capacity
-        ov = serving(2000, 2000.0, 30.0, args.serve_chunk, log, "overload")
-        out["serving_overload"] = ov
-        # capacity in requests/s: that rate over the trace's mean main steps per
-        # request (transit units make it a slight overestimate); the latency
-        # run offers half of it
-        per_req = ov["mean_main_steps_per_request"]
-        cap = ov["system_steps_per_s"] / per_req if per_req > 0 else 0.0
+        out["serving_capacity"] = serving_capacity(args.serve_chunk, log)
+    if args.serve > 0 and fits(35, "serving_overload"):
+        # the trace's own rate, 2000 requests/s of full-tmax requests: several
+        # times the capacity, so requests queue; bounded at 25 s
+        out["serving_overload"] = serving(2000, 2000.0, 25.0, args.serve_chunk, log, "overload at 2000/s")
+    if args.serve > 0 and "serving_capacity" in out and fits(110, "serving_latency"):
+        # latency at half the measured capacity: 12 s of arrivals, drained
+        cap = out["serving_capacity"]["requests_per_s"]
         rate = max(1.0, round(0.5 * cap))
         nreq = int(min(4000, max(50, rate * 12.0)))
-        out["serving_capacity_requests_per_s"] = cap
-        if fits(110, "serving_latency"):
-            out["serving"] = serving(nreq, rate, 100.0, args.serve_chunk, log, "latency at half capacity")
+        out["serving"] = serving(nreq, rate, 100.0, args.serve_chunk, log, "latency at half capacity")
     out["side_legs_dropped"] = skipped
     out["bench_wall_s"] = time.perf_counter() - T_START
     if args.calibrate_copy > 0:

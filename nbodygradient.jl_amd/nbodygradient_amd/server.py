@@ -165,3 +165,35 @@ def serve(server, specs, states, arrivals, keep_results=False, log=None, deadlin
     lat = done_t - submit_t
     return {"latency_s": lat, "wall_s": wall, "rounds": rounds, "device_ms": device_ms, "results": results,
             "finished": ndone}
+
+
+def saturate(server, specs, states, concurrency, duration, log=None):
+    """Closed loop: keep `concurrency` requests in the server (each finished
+    request is replaced at once by the next of specs, cycling) for `duration`
+    seconds of rounds, so that the GPU never waits for arrivals.  Returns the
+    system-steps (main steps + one per recorded transit), requests completed
+    and rounds over that time: the server's sustained capacity."""
+    n = len(specs)
+    nxt = 0
+    live = 0
+    while live < concurrency:
+        server.submit(specs[nxt % n], states[nxt % n])
+        nxt += 1
+        live += 1
+    w0 = server.work()
+    t0 = time.perf_counter()
+    done = 0
+    while time.perf_counter() - t0 < duration:
+        server.step()
+        for rid in server.finished():
+            server.fetch(rid)
+            done += 1
+            server.submit(specs[nxt % n], states[nxt % n])
+            nxt += 1
+        if log and server.work()["rounds"] % 50 == 0:
+            log("saturate: %d completed, %s" % (done, server.status()))
+    wall = time.perf_counter() - t0
+    w1 = server.work()
+    return {"wall_s": wall, "system_steps": w1["system_steps"] - w0["system_steps"], "completed": done,
+            "rounds": w1["rounds"] - w0["rounds"], "device_ms": w1["device_ms"] - w0["device_ms"]}
+
