@@ -6,7 +6,7 @@ C1: the 8-body near-resonant system: bit for bit against the oracle over its
     first 120 days (2000 steps); over its first 1000 days (16,667 steps) the
     transit times with and without gradients are equal bit for bit, as in the
     reference's own tests (test_transit_timing.jl:97).
-C3: fit-style vectors around one 5-body base: a batch is bit for bit the
+C3: fit-style vectors around one 6-body base (star + 5 planets): a batch is bit for bit the
     oracle, and the element gradient predicts the change of every transit time
     between two nearby parameter vectors to first order.
 (C2 is the batched bench workload, test_gpu_batch.py; C4 the serving tests,
