@@ -317,6 +317,13 @@ int nbg_copy_calibrate(nbg_ctx* ctx, int64_t bytes, int32_t reps, double* ms);
  * The measurement behind the choice of unit for the chain products. */
 int nbg_probe_row_update(nbg_ctx* ctx, int32_t mode, int32_t reps, int32_t grid, double* ms, double* j_out);
 
+/* Diagnostic only (no counterpart in the reference; the reference calls
+ * Julia's Base sin/cos/exp/sinh/cosh/cbrt): out[i] = f(x[i]) evaluated by the
+ * device build of the portable libm (nbg_libm.h) for fn 0 sin, 1 cos, 2 exp,
+ * 3 sinh, 4 cosh, 5 cbrt; x and out are host arrays of `count` doubles.  Lets
+ * a test compare the device build bit for bit with the host build. */
+int nbg_libm_eval_device(nbg_ctx* ctx, int32_t fn, int64_t count, const double* x, double* out);
+
 /* Diagnostic only: evaluates the device pair formulas (nbg_pair.h) on the
  * host CPU so the test suite can compare them bit for bit with the CPU
  * restatement without a GPU.  Not used by any integration path.

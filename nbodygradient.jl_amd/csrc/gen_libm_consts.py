@@ -1,4 +1,5 @@
-"""Derive the constants of nbg_libm.h from the mathematics (mpmath, 200 digits).
+"""Derive the constants of nbg_libm.h from the mathematics (mpmath, 200 digits;
+1600 bits for the 2/pi chunks).
 
 Run:  python gen_libm_consts.py  > consts.txt   (the header embeds the printed values)
 
@@ -6,6 +7,10 @@ Run:  python gen_libm_consts.py  > consts.txt   (the header embeds the printed v
   split into C1 + C2 + C3 where C1 and C2 carry at most `bits` significant bits,
   so that k*C1 and k*C2 are exact for the |k| the reduction admits.
 * Polynomial coefficients are the Taylor coefficients 1/k! rounded to double.
+* Payne & Hanek (1983) reduction: 2/pi as 24 doubles c_i = M_i * 2^E_i, each
+  the 53-bit rounding of 2/pi minus the ones before (mantissa and exponent
+  printed apart, since the tail lies far below the double range), and pi/2
+  as a double-double.
 """
 import mpmath as mp
 
@@ -47,3 +52,13 @@ if __name__ == "__main__":
     print("LOG2E", hexd(to_double_bits(1 / ln2, 53)))
     for k in range(2, 24):
         print("INV_FACT_%d" % k, hexd(to_double_bits(1 / mp.factorial(k), 53)))
+    with mp.workprec(1600):  # the 24 chunks reach 2^-1330
+        r = 2 / mp.pi
+        for i in range(24):
+            e = int(mp.floor(mp.log(abs(r), 2)))
+            c = to_double_bits(r, 53)
+            print("TWO_OVER_PI_CHUNK %d {%s, %d}" % (i, hexd(c / mp.mpf(2) ** e), e))
+            r -= c
+    ph = to_double_bits(pio2, 53)
+    print("PIO2_HI", hexd(ph))
+    print("PIO2_LO", hexd(to_double_bits(pio2 - ph, 53)))

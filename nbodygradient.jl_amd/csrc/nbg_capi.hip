@@ -40,6 +40,7 @@ extern "C" hipError_t nbg_launch_rewind(const nbg::SysDesc* descs, int nsys, dou
 static __device__ __align__(256) unsigned char scratch[NBG_NOALLOC_SCRATCH_BYTES];
 
 extern "C" hipError_t nbg_launch_probe(int mode, int reps, int grid, double* out, hipStream_t stream);
+extern "C" hipError_t nbg_launch_libm(int fn, int64_t count, const double* x, double* out, hipStream_t stream);
 // Initial conditions kernel (nbg_ic.hip)
 extern "C" hipError_t nbg_launch_ic(const void* descs, int nsys, const double* in, double* state, double* jac,
                                     hipStream_t stream);
@@ -2023,6 +2024,30 @@ int nbg_probe_row_update(nbg_ctx* ctx, int32_t mode, int32_t reps, int32_t grid,
     return -2;
   }
   *ms = t;
+  return 0;
+}
+
+int nbg_libm_eval_device(nbg_ctx* ctx, int32_t fn, int64_t count, const double* x, double* out) {
+  if (!ctx) return -1;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(fn >= 0 && fn <= 5, "fn must be 0 (sin) .. 5 (cbrt)");
+  NBG_ARG(count >= 0 && (count == 0 || (x && out)), "count must be >= 0 and x, out non-null");
+  if (count == 0) return 0;
+  NBG_HIP(hipSetDevice(ctx->device));
+  DevVec<double> dx, dout;
+  hipStream_t st = ctx->streams[0];
+  hipError_t e = dx.ensure(count);
+  if (e == hipSuccess) e = dout.ensure(count);
+  if (e == hipSuccess) e = hipMemcpyAsync(dx.p, x, count * sizeof(double), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = nbg_launch_libm(fn, count, dx.p, dout.p, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout.p, count * sizeof(double), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  dx.release();
+  dout.release();
+  if (e != hipSuccess) {
+    ctx->err = std::string("nbg_libm_eval_device: ") + hipGetErrorString(e);
+    return -2;
+  }
   return 0;
 }
 

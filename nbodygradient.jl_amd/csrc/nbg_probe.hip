@@ -16,7 +16,12 @@
 // touched inside the loop.  The two forms round differently (the matrix core
 // sums four products per instruction in its own order), so their results
 // agree to rounding, not bit for bit.
+//
+// Also here: libm_kernel, the device build of nbg_libm.h evaluated on an
+// array, so that a test can compare it bit for bit with the host build.
 #include <hip/hip_runtime.h>
+
+#include "nbg_libm.h"
 
 namespace {
 
@@ -134,7 +139,33 @@ __global__ __launch_bounds__(64) void probe_kernel(int reps, double* __restrict_
     for (int q = l; q < ROWS * N7; q += 64) out[q] = J[(q / N7) * COLS + q % N7];
 }
 
+// out[i] = f(x[i]) for fn 0 sin, 1 cos, 2 exp, 3 sinh, 4 cosh, 5 cbrt (one
+// element per lane, grid-stride).
+__global__ __launch_bounds__(256) void libm_kernel(int fn, int64_t count, const double* __restrict__ x,
+                                                   double* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x) {
+    const double v = x[i];
+    double y;
+    switch (fn) {
+      case 0: y = nbg::sin(v); break;
+      case 1: y = nbg::cos(v); break;
+      case 2: y = nbg::exp(v); break;
+      case 3: y = nbg::sinh(v); break;
+      case 4: y = nbg::cosh(v); break;
+      default: y = nbg::cbrt(v); break;
+    }
+    out[i] = y;
+  }
+}
+
 }  // namespace
+
+extern "C" hipError_t nbg_launch_libm(int fn, int64_t count, const double* x, double* out, hipStream_t stream) {
+  const int64_t blocks = (count + 255) / 256;
+  const int grid = (int)(blocks < 4096 ? (blocks > 0 ? blocks : 1) : 4096);
+  hipLaunchKernelGGL(libm_kernel, dim3(grid), dim3(256), 0, stream, fn, count, x, out);
+  return hipGetLastError();
+}
 
 // Runs the probe: `grid` one-wave workgroups, `reps` updates each; *ms is the
 // kernel's time (HIP events), out (12 x 63, row-major) workgroup 0's final J.

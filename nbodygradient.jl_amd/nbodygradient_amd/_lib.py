@@ -89,6 +89,7 @@ def lib():
     L.nbg_server_footprint.restype = C.c_int64
     L.nbg_copy_calibrate.argtypes = [VP, C.c_int64, C.c_int32, D]
     L.nbg_probe_row_update.argtypes = [VP, C.c_int32, C.c_int32, C.c_int32, D, D]
+    L.nbg_libm_eval_device.argtypes = [VP, C.c_int32, C.c_int64, D, D]
     L.nbg_selftest_pair_host.argtypes = [D, D, C.c_double, C.c_double, C.c_int32, D]
     _lib = L
     return L

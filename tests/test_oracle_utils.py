@@ -26,6 +26,28 @@ def test_libm_ulp(name, lo, hi, bound):
     assert ulp.max() <= bound, (name, ulp.max())
 
 
+@pytest.mark.parametrize("lo,hi", [(6.6e5, 1e7), (1e7, 1e20), (1e20, 1e100), (1e100, 1.79e308)])
+def test_libm_sincos_large_arguments(lo, hi):
+    """|x * 2/pi| >= 2^20 takes the Payne-Hanek reduction."""
+    x = 10 ** np.random.default_rng(11).uniform(np.log10(lo), np.log10(hi), 40000)
+    x[::2] *= -1
+    for name in ("sin", "cos"):
+        _, ulp = O.libm_eval(name, x)
+        assert ulp.max() <= 1.0, (name, lo, ulp.max())
+
+
+def test_libm_sincos_hardest_reduction():
+    """The double closest to a multiple of pi/2 (x mod pi/2 ~ 4.7e-19), the
+    largest doubles and the first reductions past the Cody-Waite range."""
+    big = np.finfo(float).max
+    x = np.array([6381956970095103 * 2.0 ** 797, 2.0 ** 1023, big, -big, 2.0 ** 20 * np.pi / 2, -(2.0 ** 20) * np.pi / 2])
+    for name in ("sin", "cos"):
+        _, ulp = O.libm_eval(name, x)
+        assert ulp.max() <= 1.0, (name, ulp)
+    c, _ = O.libm_eval("cos", x[:1])
+    assert abs(c[0] + 4.687165924254628e-19) < 1e-33
+
+
 def test_libm_cbrt_wide_range_and_specials():
     rng = np.random.default_rng(3)
     x = np.concatenate([10 ** rng.uniform(-300, 300, 50000), -(10 ** rng.uniform(-300, 300, 5000)), [5e-324, 2.0 ** -1060]])
