@@ -177,9 +177,10 @@ int nbg_batch_autotune_round(nbg_batch* b, int64_t steps, int32_t* best, double*
 /* Autotuner (no counterpart in the reference): runs a copy of batch b with
  * every system's step count scaled by frac (0 < frac <= 1) under each
  * candidate launch configuration -- the workgroup form (two-wave workgroups
- * for the longest systems: off, automatic, or the 16/32/64 longest) and the
- * one-wave workgroups per CU -- and keeps the fastest in b's context (the
- * values nbg_set_coop and nbg_set_launch set).  cand_ms[c] receives candidate
+ * for the longest systems: off, automatic, or the 16/32/64 longest), the
+ * one-wave workgroups per CU and the systems per wave (one, or two for the
+ * paired body counts) -- and keeps the fastest in b's context (the values
+ * nbg_set_coop, nbg_set_launch and nbg_set_systems_per_wave set).  cand_ms[c] receives candidate
  * c's device time for c < max_cand (may be NULL), *best the chosen index.
  * The copy needs as much device memory as b.  The batch itself is not run. */
 int nbg_batch_autotune(nbg_batch* b, double frac, int32_t* best, double* cand_ms, int32_t max_cand);
@@ -187,6 +188,9 @@ int nbg_batch_autotune(nbg_batch* b, double frac, int32_t* best, double* cand_ms
  * and its workgroups per CU, 0 = as LDS allows); returns the number of
  * candidates. */
 int nbg_batch_autotune_candidate(int32_t index, int32_t* coop_mode, int32_t* coop_max, int32_t* groups_per_cu);
+/* The systems per wave (1 or 2) of autotuner candidate `index`; -1 when
+ * there is no such candidate. */
+int32_t nbg_batch_autotune_systems_per_wave(int32_t index);
 /* Restore every system to its initial state (device-side copy). */
 int nbg_batch_reset(nbg_batch* b);
 /* Integrate all systems to completion: one persistent launch per class of
@@ -294,6 +298,18 @@ int nbg_set_grid_cap(nbg_ctx* ctx, int32_t grid_cap);
  * batch's end); 2: the longest ones regardless, in batch runs and serving
  * rounds.  At most max_systems per launch (default 128); n <= 10. */
 int nbg_set_coop(nbg_ctx* ctx, int32_t mode, int32_t max_systems);
+/* Systems per wave of batch runs and rounds (no counterpart in the
+ * reference; the pair-to-lane mapping of the pair sweeps ahl21.jl:25-78).
+ * k = 1 (default): one system per wave, a level's Kepler pairs one per lane.
+ * k = 2: transit runs of equal body count run two per wave in lockstep
+ * (every n <= 9; n = 10 too where the n = 12 launch takes it; two records
+ * with their Jacobians in HBM and one shared work area fit the LDS of the
+ * launch's one-system layouts, so occupancy is unchanged): the Kepler drifts
+ * and pair Jacobians of both systems' levels share the lanes; the rest of
+ * each step, and the transit searches, run for one system, then the other.
+ * Results are bitwise those of k = 1.  Serving rounds keep one system per
+ * wave. */
+int nbg_set_systems_per_wave(nbg_ctx* ctx, int32_t k);
 /* Diagnostic only (no counterpart in the reference): bound every wait between
  * the two waves of a cooperative workgroup to `polls` polls (0: the defaults,
  * ~1 s for a free search slot and ~2 min for the next search).  A wait that

@@ -45,6 +45,10 @@ extern "C" hipError_t nbg_launch_libm(int fn, int64_t count, const double* x, do
 extern "C" hipError_t nbg_launch_ic(const void* descs, int nsys, const double* in, double* state, double* jac,
                                     hipStream_t stream);
 extern "C" int nbg_coop_max_bodies();
+// Two systems per wave (nbg_duo.hip)
+extern "C" hipError_t nbg_launch_group_duo(int cls, nbg::GroupList gl, nbg::SysBufs bufs, int64_t chunk, int grid,
+                                           unsigned long long* units, hipStream_t stream);
+extern "C" int nbg_duo_pairs(int cls, int n);
 
 namespace {
 
@@ -96,6 +100,7 @@ struct Layout {
   std::vector<int> gn, goff, gcnt;
   std::vector<double> gcost;
   std::vector<double> cost;  // optional per-system cost (default: from nsteps)
+  int pairs = 0;             // queues ordered for two systems per wave (nbg_run_duo)
 };
 
 double check_step(double t0, double tmax) {  // Integrator.jl:249-259
@@ -130,7 +135,7 @@ void finalize_layout(Layout& L) {
     const int n = d.n;
     d.off_state = s; s += nbg::state_len(n);
     d.off_jac = j; j += nbg::jac_len(n);
-    d.off_ws = w; w += nbg::ws_len(n);
+    d.off_ws = w; w += nbg::ws_len_batch(n);
     d.off_tt = t; t += (int64_t)d.ntbv * n * d.ntt;
     d.off_dq = q; d.off_del = q; q += (int64_t)d.ntbv * n * d.ntt * 7 * n;
     d.off_int = in; in += nbg::int_len(n);
@@ -168,6 +173,51 @@ void group_layout(Layout& L) {
     L.gcnt.back() += 1;
     L.gcost.back() += cost(L.order[k]);
   }
+  if (!L.pairs) return;
+  // Two systems per wave: a launch takes queue positions 2k and 2k + 1
+  // together.  Within each group, systems of a body count that runs in pairs
+  // are matched with the next system of the same count (longest with
+  // longest), every other system with the next such unmatched one; the
+  // units keep the longest-first order of their first system, and a unit of
+  // one system (at most one per group) goes last, so that every pair starts
+  // at an even position.
+  for (size_t g = 0; g < L.gn.size(); ++g) {
+    const int off = L.goff[g], cnt = L.gcnt[g], cls = L.gn[g];
+    std::vector<int> sub(L.order.begin() + off, L.order.begin() + off + cnt);
+    std::vector<std::vector<int>> pos(NB_MAX + 1);  // positions in sub by body count
+    for (int k = 0; k < cnt; ++k) pos[L.desc[sub[k]].n].push_back(k);
+    std::vector<size_t> cur(NB_MAX + 1, 0);
+    std::vector<char> used(cnt, 0);
+    std::vector<std::vector<int>> units;
+    int single = -1;  // unit holding one unmatched system, waiting for a second
+    for (int k = 0; k < cnt; ++k) {
+      if (used[k]) continue;
+      used[k] = 1;
+      const int n = L.desc[sub[k]].n;
+      if (nbg_duo_pairs(cls, n)) {
+        auto& pn = pos[n];
+        while (cur[n] < pn.size() && used[pn[cur[n]]]) ++cur[n];
+        if (cur[n] < pn.size()) {
+          const int j = pn[cur[n]++];
+          used[j] = 1;
+          units.push_back({sub[k], sub[j]});
+          continue;
+        }
+      }
+      if (single >= 0) {
+        units[single].push_back(sub[k]);
+        single = -1;
+      } else {
+        units.push_back({sub[k]});
+        single = (int)units.size() - 1;
+      }
+    }
+    int at = off;
+    for (size_t u = 0; u < units.size(); ++u)
+      if ((int)u != single)
+        for (int i : units[u]) L.order[at++] = i;
+    if (single >= 0) L.order[at++] = units[single][0];
+  }
 }
 
 bool mass_rows_ok(int n, const double* J, const double* JE) {
@@ -194,6 +244,7 @@ struct nbg_ctx {
   int grid_cap = 0;
   int coop_mode = 1;   // 0 off, 1 systems longer than the batch's throughput bound, 2 the longest ones
   int coop_max = 128;  // at most this many cooperative systems per batch
+  int systems_per_wave = 1;  // batches: 2 runs transit systems of equal body count two per wave (nbg_run_duo)
   int64_t coop_polls = 0;  // > 0: bound on every inter-wave wait (diagnostic); 0: the defaults
   int nstreams = 4;
   std::vector<hipStream_t> streams;
@@ -209,6 +260,7 @@ struct nbg_batch {
   Store S;
   int64_t steps_main = 0;
   int64_t order_chunk = 0;  // the visit length the device queue order is for (0: whole runs)
+  int order_pairs = 0;      // ... and whether it pairs systems (two per wave)
 };
 
 #define NBG_HIP(expr)                                                                 \
@@ -348,6 +400,10 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
         ++kg[g];
         ++K;
       }
+      if (L.pairs && (kg[g] & 1)) {  // keep the pairs of the one-wave queue at even positions
+        --kg[g];
+        --K;
+      }
     }
   }
   const bool timed_coop = K > 0 && group_ms != nullptr;
@@ -404,7 +460,9 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
     int64_t work = 0;  // systems this launch can take (none left when all went to cooperative workgroups)
     for (int e = 0; e < gl.ng; ++e) work += gl.nsys[e];
     if (timed) NBG_HIP(hipEventRecord(ctx->gev[2 * g], st));
-    if (work > 0) NBG_HIP(nbg_launch_group(L.gn[g], gl, B, chunk, grid, S.units.p + 13 * g, st));
+    if (work > 0)
+      NBG_HIP(L.pairs ? nbg_launch_group_duo(L.gn[g], gl, B, chunk, grid, S.units.p + 13 * g, st)
+                      : nbg_launch_group(L.gn[g], gl, B, chunk, grid, S.units.p + 13 * g, st));
     if (timed) NBG_HIP(hipEventRecord(ctx->gev[2 * g + 1], st));
   }
   for (int s = 0; s < ns; ++s) {
@@ -688,6 +746,14 @@ int nbg_set_coop(nbg_ctx* ctx, int32_t mode, int32_t max_systems) {
   NBG_ARG(max_systems >= 0, "max_systems must be >= 0");
   ctx->coop_mode = mode;
   ctx->coop_max = max_systems;
+  return 0;
+}
+
+int nbg_set_systems_per_wave(nbg_ctx* ctx, int32_t k) {
+  if (!ctx) return -1;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(k == 1 || k == 2, "systems per wave must be 1 or 2");
+  ctx->systems_per_wave = k;
   return 0;
 }
 
@@ -1167,8 +1233,10 @@ namespace {
 // the per-step cost of the body count; the descriptors are uploaded again.
 int batch_order(nbg_batch* b, int64_t chunk) {
   nbg_ctx* ctx = b->ctx;
-  if (b->order_chunk == chunk) return 0;
+  const int pairs = ctx->systems_per_wave == 2 ? 1 : 0;
+  if (b->order_chunk == chunk && b->order_pairs == pairs) return 0;
   Layout& L = b->L;
+  L.pairs = pairs;
   L.cost.clear();
   if (chunk > 0) {
     L.cost.resize(L.desc.size());
@@ -1179,6 +1247,7 @@ int batch_order(nbg_batch* b, int64_t chunk) {
   for (size_t k = 0; k < L.order.size(); ++k) sorted[k] = L.desc[L.order[k]];
   NBG_HIP(hipMemcpy(b->S.desc.p, sorted.data(), sorted.size() * sizeof(nbg::SysDesc), hipMemcpyHostToDevice));
   b->order_chunk = chunk;
+  b->order_pairs = pairs;
   return 0;
 }
 
@@ -1267,16 +1336,33 @@ int nbg_batch_round(nbg_batch* b, int64_t steps, int32_t restart, nbg_batch_stat
 // so its launch classes, LPT order and two-wave selection scale with it.
 namespace {
 struct TuneCand {
-  int coop_mode, coop_max, groups_per_cu;
+  int coop_mode, coop_max, groups_per_cu, systems_per_wave;
 };
 const TuneCand kTune[] = {
-    {1, 128, 0},  // two-wave workgroups for the systems beyond the batch's throughput bound
-    {0, 0, 0},    // one wave per system throughout
-    {2, 16, 0},   // two-wave workgroups for the 16, 32, 64 longest systems
-    {2, 32, 0},
-    {2, 64, 0},
-    {1, 128, 3},  // three one-wave workgroups per CU instead of four
+    {1, 128, 0, 1},  // two-wave workgroups for the systems beyond the batch's throughput bound
+    {0, 0, 0, 1},    // one wave per system throughout
+    {2, 16, 0, 1},   // two-wave workgroups for the 16, 32, 64 longest systems
+    {2, 32, 0, 1},
+    {2, 64, 0, 1},
+    {1, 128, 3, 1},  // three one-wave workgroups per CU instead of four
+    {1, 128, 0, 2},  // two systems per wave, with and without two-wave workgroups
+    {0, 0, 0, 2},
 };
+
+struct TuneKeep {
+  int coop_mode, coop_max, groups_per_cu, systems_per_wave;
+  explicit TuneKeep(const nbg_ctx* c)
+      : coop_mode(c->coop_mode), coop_max(c->coop_max), groups_per_cu(c->max_groups_per_cu),
+        systems_per_wave(c->systems_per_wave) {}
+  void restore(nbg_ctx* c) const {
+    c->coop_mode = coop_mode; c->coop_max = coop_max; c->max_groups_per_cu = groups_per_cu;
+    c->systems_per_wave = systems_per_wave;
+  }
+};
+void tune_apply(nbg_ctx* c, const TuneCand& t) {
+  c->coop_mode = t.coop_mode; c->coop_max = t.coop_max; c->max_groups_per_cu = t.groups_per_cu;
+  c->systems_per_wave = t.systems_per_wave;
+}
 }  // namespace
 
 int nbg_batch_autotune(nbg_batch* b, double frac, int32_t* best, double* cand_ms, int32_t max_cand) {
@@ -1295,34 +1381,38 @@ int nbg_batch_autotune(nbg_batch* b, double frac, int32_t* best, double* cand_ms
   };
   if (int rc = store_alloc(ctx, T, Ls, false)) return done(rc);
   const int nc = (int)(sizeof(kTune) / sizeof(kTune[0]));
-  const int keep_mode = ctx->coop_mode, keep_max = ctx->coop_max, keep_g = ctx->max_groups_per_cu;
+  const TuneKeep keep(ctx);
   int bi = 0;
   double bms = 0.0;
   for (int c = 0; c < nc; ++c) {
-    hipError_t e = hipMemcpy(T.state.p, b->S.state0.p, Ls.n_state * sizeof(double), hipMemcpyDeviceToDevice);
+    hipError_t e = hipSuccess;
+    if (Ls.pairs != (kTune[c].systems_per_wave == 2 ? 1 : 0)) {  // queue order for this candidate
+      Ls.pairs = kTune[c].systems_per_wave == 2 ? 1 : 0;
+      group_layout(Ls);
+      std::vector<nbg::SysDesc> sorted(Ls.desc.size());
+      for (size_t k = 0; k < Ls.order.size(); ++k) sorted[k] = Ls.desc[Ls.order[k]];
+      e = hipMemcpy(T.desc.p, sorted.data(), sorted.size() * sizeof(nbg::SysDesc), hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) e = hipMemcpy(T.state.p, b->S.state0.p, Ls.n_state * sizeof(double), hipMemcpyDeviceToDevice);
     if (e == hipSuccess) e = hipMemcpy(T.jac.p, b->S.jac0.p, Ls.n_jac * sizeof(double), hipMemcpyDeviceToDevice);
     if (e == hipSuccess) e = hipMemcpy(T.pair.p, b->S.pair.p, std::max<int64_t>(Ls.n_pair, 1), hipMemcpyDeviceToDevice);
     if (e == hipSuccess) e = hipMemset(T.ints.p, 0, Ls.n_int * sizeof(int64_t));
     if (e != hipSuccess) {
       ctx->err = std::string("autotune: ") + hipGetErrorString(e);
-      ctx->coop_mode = keep_mode; ctx->coop_max = keep_max; ctx->max_groups_per_cu = keep_g;
+      keep.restore(ctx);
       return done(-2);
     }
-    ctx->coop_mode = kTune[c].coop_mode;
-    ctx->coop_max = kTune[c].coop_max;
-    ctx->max_groups_per_cu = kTune[c].groups_per_cu;
+    tune_apply(ctx, kTune[c]);
     double ms = 0.0;
     const int rc = run_groups(ctx, T, Ls, 0, &ms);
     if (rc) {
-      ctx->coop_mode = keep_mode; ctx->coop_max = keep_max; ctx->max_groups_per_cu = keep_g;
+      keep.restore(ctx);
       return done(rc);
     }
     if (cand_ms && c < max_cand) cand_ms[c] = ms;
     if (c == 0 || ms < bms) { bms = ms; bi = c; }
   }
-  ctx->coop_mode = kTune[bi].coop_mode;
-  ctx->coop_max = kTune[bi].coop_max;
-  ctx->max_groups_per_cu = kTune[bi].groups_per_cu;
+  tune_apply(ctx, kTune[bi]);
   if (best) *best = bi;
   return done(0);
 }
@@ -1333,26 +1423,22 @@ int nbg_batch_autotune_round(nbg_batch* b, int64_t steps, int32_t* best, double*
   std::lock_guard<std::mutex> lk(ctx->mu);
   NBG_ARG(steps > 0, "steps must be positive");
   const int nc = (int)(sizeof(kTune) / sizeof(kTune[0]));
-  const int keep_mode = ctx->coop_mode, keep_max = ctx->coop_max, keep_g = ctx->max_groups_per_cu;
+  const TuneKeep keep(ctx);
   int bi = 0;
   double bms = 0.0;
   for (int c = 0; c < nc; ++c) {
     int rc = nbg_batch_reset(b);
-    ctx->coop_mode = kTune[c].coop_mode;
-    ctx->coop_max = kTune[c].coop_max;
-    ctx->max_groups_per_cu = kTune[c].groups_per_cu;
+    tune_apply(ctx, kTune[c]);
     nbg_batch_stats st;
     if (rc == 0) rc = batch_round(b, steps, false, &st);
     if (rc) {
-      ctx->coop_mode = keep_mode; ctx->coop_max = keep_max; ctx->max_groups_per_cu = keep_g;
+      keep.restore(ctx);
       return rc;
     }
     if (cand_ms && c < max_cand) cand_ms[c] = st.kernel_ms;
     if (c == 0 || st.kernel_ms < bms) { bms = st.kernel_ms; bi = c; }
   }
-  ctx->coop_mode = kTune[bi].coop_mode;
-  ctx->coop_max = kTune[bi].coop_max;
-  ctx->max_groups_per_cu = kTune[bi].groups_per_cu;
+  tune_apply(ctx, kTune[bi]);
   if (best) *best = bi;
   return nbg_batch_reset(b);
 }
@@ -1364,6 +1450,11 @@ int nbg_batch_autotune_candidate(int32_t index, int32_t* coop_mode, int32_t* coo
   if (coop_max) *coop_max = kTune[index].coop_max;
   if (groups_per_cu) *groups_per_cu = kTune[index].groups_per_cu;
   return nc;
+}
+
+int32_t nbg_batch_autotune_systems_per_wave(int32_t index) {
+  const int nc = (int)(sizeof(kTune) / sizeof(kTune[0]));
+  return (index < 0 || index >= nc) ? -1 : kTune[index].systems_per_wave;
 }
 
 int nbg_batch_download(nbg_batch* b, double* tt, double* dtdq0, double* dtdelements, int64_t* count, double* x,

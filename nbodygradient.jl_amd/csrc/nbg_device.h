@@ -47,7 +47,7 @@ struct PairSlot {
   double mc[24];  // jac_ij's mass columns 6 (rows 0-5, 7-12) and 13 (same rows)
 };
 
-template <int NB> struct Smem {
+template <int NB, bool G = (NB >= JGLOBAL_MIN)> struct Smem {
   static constexpr int N3 = 3 * NB, N4 = 4 * NB, N6 = 6 * NB, N7 = 7 * NB;
   static constexpr int JOFF = N6 * N7;
   // jac_step without mass rows (row 6b+k <-> Julia row 7b+k), then jac_error
@@ -55,25 +55,10 @@ template <int NB> struct Smem {
   // From JGLOBAL_MIN bodies on, these live in an HBM work area (jg) and the
   // LDS keeps everything else: four such systems fit a CU instead of two.
   // Every access to them is made by the lane that owns the column.
-  static constexpr bool JG = NB >= JGLOBAL_MIN;
+  static constexpr bool JG = G;
   double JJ[JG ? 1 : 2 * N6 * N7];
   double* jg;
   static constexpr int NP = NB * (NB - 1) / 2;  // pairs per sweep
-  static constexpr int PQ = 17;  // per-pair factors of phisalpha!, see phi_build
-  // pair slots share the space of the force-gradient arrays (union below)
-  static constexpr int CH_FIT = (int)(((2 * N3 * N4 + (NP > 0 ? NP : 1) * PQ) * sizeof(double)) / sizeof(PairSlot));
-  static constexpr int CH_MIN = CH_FIT < 4 ? 4 : CH_FIT;
-  static constexpr int CH = NP < CH_MIN ? NP : (CH_MIN > 64 ? 64 : CH_MIN);  // pairs per sweep chunk
-  // Lifetimes are disjoint: jac_phi/dadq live in the force-gradient stage
-  // (dadq also holds jac_kick around the kicks), the pair slots in the sweeps.
-  union {
-    struct {
-      double phi[N3 * N4];   // jac_phi block: row 3b+k (velocity k of body b), col 3d+p (x_p of d) or N3+d (m_d)
-      double dadq[N3 * N4];  // d a / d q: row 3b+k, same columns
-      double pq[(NP > 0 ? NP : 1) * PQ];  // per pair (lexicographic index)
-    } f;
-    PairSlot slot[CH];
-  } u;
   // Kepler pairs (i, j) of the forward / backward sweep grouped by level:
   // a pair's level is one more than the latest earlier pair sharing a body,
   // so the pairs of a level are disjoint; lvw holds the level widths.
@@ -96,6 +81,35 @@ template <int NB> struct Smem {
   uint8_t pair[NB * NB];
 };
 
+// The work area of a step's stages (LDS).  Lifetimes are disjoint: jac_phi
+// and dadq live in the force-gradient stage (dadq also holds jac_kick around
+// the kicks), the pair slots in the sweeps; each stage builds what it reads.
+// One system's layout (Solo) keeps it beside the system's record; a pair of
+// systems run by one wave (Duo) shares one, the slots split between them.
+template <int NB> struct Swork {
+  static constexpr int N3 = 3 * NB, N4 = 4 * NB;
+  static constexpr int NP = NB * (NB - 1) / 2;
+  static constexpr int PQ = 17;  // per-pair factors of phisalpha!, see phi_build
+  // pair slots share the space of the force-gradient arrays
+  static constexpr int CH_FIT = (int)(((2 * N3 * N4 + (NP > 0 ? NP : 1) * PQ) * sizeof(double)) / sizeof(PairSlot));
+  static constexpr int CH_MIN = CH_FIT < 4 ? 4 : CH_FIT;
+  static constexpr int CH = NP < CH_MIN ? NP : (CH_MIN > 64 ? 64 : CH_MIN);  // pairs per sweep chunk
+  union {
+    struct {
+      double phi[N3 * N4];   // jac_phi block: row 3b+k (velocity k of body b), col 3d+p (x_p of d) or N3+d (m_d)
+      double dadq[N3 * N4];  // d a / d q: row 3b+k, same columns
+      double pq[(NP > 0 ? NP : 1) * PQ];  // per pair (lexicographic index)
+    } f;
+    PairSlot slot[CH];
+  };
+};
+
+// LDS of one system run by one wave: its record and its work area.
+template <int NB> struct Solo {
+  Smem<NB> s;
+  Swork<NB> w;
+};
+
 // The Jacobian store of a system: an LDS array, or (from JGLOBAL_MIN bodies
 // on) the HBM work area, addressed as global memory from a wave-uniform
 // base so that accesses are global loads/stores with scalar bases rather
@@ -107,12 +121,12 @@ __device__ __forceinline__ gdouble* uniform_global(const double* p) {
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
   return (gdouble*)((((uint64_t)hi) << 32) | lo);
 }
-template <int NB> __device__ __forceinline__ auto jj(Smem<NB>& S) {
-  if constexpr (Smem<NB>::JG) return uniform_global(S.jg);
+template <int NB, bool G> __device__ __forceinline__ auto jj(Smem<NB, G>& S) {
+  if constexpr (G) return uniform_global(S.jg);
   else return static_cast<double*>(S.JJ);
 }
-template <int NB> __device__ __forceinline__ auto jj(const Smem<NB>& S) {
-  if constexpr (Smem<NB>::JG) return (const gdouble*)uniform_global(S.jg);
+template <int NB, bool G> __device__ __forceinline__ auto jj(const Smem<NB, G>& S) {
+  if constexpr (G) return (const gdouble*)uniform_global(S.jg);
   else return static_cast<const double*>(S.JJ);
 }
 
@@ -213,11 +227,11 @@ template <int NB> __device__ inline void col_dp(int cc, int& d, int& p) {
 // ---------------------------------------------------------------------------
 // kickfast! with Jacobian (ahl21.jl:337-386): impulses for kicked pairs.
 // v (Kahan) and dqdt_kick are uniform (lane 0 stores); jac_kick is built with
-// one lane per column into S.u.f.dadq.
+// one lane per column into W.f.dadq.
 // ---------------------------------------------------------------------------
-template <int NB> __device__ void kick_build(Smem<NB>& S, double h, bool needQ, int lane) {
+template <int NB, bool G> __device__ void kick_build(Smem<NB, G>& S, Swork<NB>& W, double h, bool needQ, int lane) {
   constexpr int N3 = 3 * NB, N4 = 4 * NB;
-  double* K = S.u.f.dadq;
+  double* K = W.f.dadq;
   if (lane < N4)
     for (int r = 0; r < N3; ++r) K[r * N4 + lane] = 0.0;
   int myd = -1, myp = -1;
@@ -274,13 +288,13 @@ template <int NB> __device__ void kick_build(Smem<NB>& S, double h, bool needQ, 
 // Kepler pairs, alpha = 2) with Jacobian.  Phi and dadq columns are owned by
 // lanes; a, v and dqdt_phi are uniform (lane 0 stores).
 // ---------------------------------------------------------------------------
-template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick, bool needQ, int lane) {
+template <int NB, bool G> __device__ void phi_build(Smem<NB, G>& S, Swork<NB>& W, double h, bool anyKick, bool needQ, int lane) {
   constexpr int N3 = 3 * NB, N4 = 4 * NB;
   const bool own = lane < N4;
   int myd = -1, myp = -1;
   if (own) col_dp<NB>(lane, myd, myp);
-  double* PHI = S.u.f.phi;
-  double* D = S.u.f.dadq;
+  double* PHI = W.f.phi;
+  double* D = W.f.dadq;
   if (anyKick) {
     // phic!: a, dadq, jac_phi set to zero (ahl21.jl:393-399)
     if (own)
@@ -427,8 +441,8 @@ template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick,
   // order; looping over the partner c in increasing order therefore repeats
   // every accumulation in the reference's order.  a and dadq start at zero
   // (ahl21.jl:559-563); jac_phi keeps phic!'s terms.
-  constexpr int NP = Smem<NB>::NP, PQ = Smem<NB>::PQ;
-  double* pq = S.u.f.pq;
+  constexpr int NP = Swork<NB>::NP, PQ = Swork<NB>::PQ;
+  double* pq = W.f.pq;
   auto pidx = [](int i, int j) { return i * (2 * NB - i - 1) / 2 + (j - i - 1); };
   const double coeff = (((2.0 * (h * h * h)) / 96.0) * 2.0) * NBG_GNEWT;
   // loop 1 factors (ahl21.jl:569-575): fac2 = G/r^3, fac_k, f3_k = fac_k * 3/r^2
@@ -599,7 +613,7 @@ template <int NB> __device__ void phi_build(Smem<NB>& S, double h, bool anyKick,
 // Jacobian column kernels (lane owns column c).
 // ---------------------------------------------------------------------------
 // Position rows of column c (pre-update), the operand of K*J and Phi*J.
-template <int NB> __device__ inline void load_pos(const Smem<NB>& S, int c, double* P) {
+template <int NB, bool G> __device__ inline void load_pos(const Smem<NB, G>& S, int c, double* P) {
   const auto J_ = jj<NB>(S);
   constexpr int N7 = 7 * NB;
 #pragma unroll
@@ -611,7 +625,7 @@ template <int NB> __device__ inline void load_pos(const Smem<NB>& S, int c, doub
 // J[velocity rows, c] += (B * J)[velocity rows, c] with Kahan compensation,
 // B = K or Phi (3n x 4n block); then the zero-addend Kahan pass over the
 // position rows that comp_sum_matrix! applies to the rest of the matrix.
-template <int NB> __device__ inline void vel_product_add(Smem<NB>& S, const double* B, int c, const double* P) {
+template <int NB, bool G> __device__ inline void vel_product_add(Smem<NB, G>& S, const double* B, int c, const double* P) {
   const auto J_ = jj<NB>(S);
   constexpr int N3 = 3 * NB, N4 = 4 * NB, N7 = 7 * NB;
 #pragma unroll 1
@@ -631,7 +645,7 @@ template <int NB> __device__ inline void vel_product_add(Smem<NB>& S, const doub
   }
 }
 
-template <int NB> __device__ inline void zero_add_rows(Smem<NB>& S, int c, bool pos, bool vel) {
+template <int NB, bool G> __device__ inline void zero_add_rows(Smem<NB, G>& S, int c, bool pos, bool vel) {
   const auto J_ = jj<NB>(S);
   constexpr int N7 = 7 * NB;
 #pragma unroll 1
@@ -649,7 +663,7 @@ template <int NB> __device__ inline void zero_add_rows(Smem<NB>& S, int c, bool 
 }
 
 // drift_grad! Jacobian part (ahl21.jl:326-328)
-template <int NB> __device__ inline void drift_cols(Smem<NB>& S, int c, double hh) {
+template <int NB, bool G> __device__ inline void drift_cols(Smem<NB, G>& S, int c, double hh) {
   const auto J_ = jj<NB>(S);
   constexpr int N7 = 7 * NB;
 #pragma unroll 1
@@ -668,7 +682,7 @@ template <int NB> __device__ inline void drift_cols(Smem<NB>& S, int c, double h
 // running sum starts at +0 and can never become -0, so the sign of a zero
 // coefficient cannot change it), and the mass columns 6 and 13 take their
 // coefficients from the slot, where pair_jacobian stored them.
-template <int NB> __device__ inline void pair_cols(Smem<NB>& S, const PairSlot& sl, int i, int j, int c) {
+template <int NB, bool G> __device__ inline void pair_cols(Smem<NB, G>& S, const PairSlot& sl, int i, int j, int c) {
   const auto J_ = jj<NB>(S);
   constexpr int N7 = 7 * NB;
   const int ri = 6 * i, rj = 6 * j;
@@ -736,10 +750,128 @@ template <int NB> __device__ inline void pair_cols(Smem<NB>& S, const PairSlot& 
 // state, so running all state updates of a chunk before its Jacobian updates
 // gives the reference's results exactly.
 // ---------------------------------------------------------------------------
-template <int NB>
-__device__ __attribute__((always_inline)) inline void sweep(Smem<NB>& S, int dir, double hh, bool needJ, bool needQ, int lane) {
+// Phase A for one lane: the Kepler drift of pair (i, j) of system S into
+// slot sl and the Kahan update of the two bodies' state.
+template <int NB, bool G>
+__device__ __attribute__((always_inline)) inline void pair_level_lane(Smem<NB, G>& S, PairSlot& sl, int i, int j,
+                                                                      double hh, bool drift_first) {
+  // every LDS operand is read into registers before anything is written
+  // back: the compiler cannot tell the slot from the state arrays, so
+  // interleaved reads and writes would each wait for the previous write (a
+  // dozen LDS round trips in a row per level)
+  {
+    double xi[3], xj[3], vi[3], vj[3];
+    for (int k = 0; k < 3; ++k) {
+      xi[k] = S.x[3 * i + k];
+      xj[k] = S.x[3 * j + k];
+      vi[k] = S.v[3 * i + k];
+      vj[k] = S.v[3 * j + k];
+    }
+    for (int k = 0; k < 3; ++k) {
+      sl.x0[k] = xi[k] - xj[k];
+      sl.v0[k] = vi[k] - vj[k];
+    }
+  }
+  const double m_i = S.m[i], m_j = S.m[j];
+  const double gm = NBG_GNEWT * (m_i + m_j);
+  if (gm != 0.0) {
+    pair_drift(&sl, gm, hh, drift_first);
+    const double mijinv = 1.0 / (m_i + m_j);
+    const double mi = m_i * mijinv;
+    const double mj = m_j * mijinv;
+    double d[6], xi[3], xj[3], vi[3], vj[3], exi[3], exj[3], evi[3], evj[3];
+    for (int k = 0; k < 6; ++k) d[k] = sl.delxv[k];
+    for (int k = 0; k < 3; ++k) {
+      xi[k] = S.x[3 * i + k];
+      exi[k] = S.xe[3 * i + k];
+      xj[k] = S.x[3 * j + k];
+      exj[k] = S.xe[3 * j + k];
+      vi[k] = S.v[3 * i + k];
+      evi[k] = S.ve[3 * i + k];
+      vj[k] = S.v[3 * j + k];
+      evj[k] = S.ve[3 * j + k];
+    }
+    for (int k = 0; k < 3; ++k) {
+      kahan(xi[k], exi[k], mj * d[k]);
+      kahan(xj[k], exj[k], (-mi) * d[k]);
+    }
+    for (int k = 0; k < 3; ++k) {
+      kahan(vi[k], evi[k], mj * d[3 + k]);
+      kahan(vj[k], evj[k], (-mi) * d[3 + k]);
+    }
+    for (int k = 0; k < 3; ++k) {
+      S.x[3 * i + k] = xi[k];
+      S.xe[3 * i + k] = exi[k];
+      S.x[3 * j + k] = xj[k];
+      S.xe[3 * j + k] = exj[k];
+      S.v[3 * i + k] = vi[k];
+      S.ve[3 * i + k] = evi[k];
+      S.v[3 * j + k] = vj[k];
+      S.ve[3 * j + k] = evj[k];
+    }
+    sl.mi = mi; sl.mj = mj; sl.mijinv = mijinv; sl.m_i = m_i; sl.m_j = m_j;
+  }
+  sl.valid = (gm != 0.0); sl.i = i; sl.j = j;
+}
+
+// Phase C of one chunk of `cnt` pairs (slots[0..cnt)) of system S.
+template <int NB, bool G>
+__device__ __attribute__((always_inline)) inline void sweep_rows(Smem<NB, G>& S, const PairSlot* slots, int cnt,
+                                                                 bool needJ, bool needQ, int lane) {
   constexpr int N7 = 7 * NB;
-  constexpr int CH = Smem<NB>::CH;
+  int lastq = -1;  // last slot of this chunk whose pair has mass
+#pragma unroll 1
+  for (int q = 0; q < cnt; ++q) {
+    const PairSlot& sl = slots[q];
+    const int i = sl.i, j = sl.j;
+    if (sl.valid) {
+      lastq = q;
+      if (needQ && lane < 6) {  // dqdt_ij (ahl21.jl:743-748)
+        S.dqij[lane] = sl.mj * sl.kep[lane + 42];
+        S.dqij[7 + lane] = (-sl.mi) * sl.kep[lane + 42];
+      }
+    }
+    // (gm == 0: the reference returns before touching jac_ij / dqdt_ij and
+    // the caller reuses the previous values: the last pair with mass, or
+    // jlast when that lies in an earlier chunk; dqij keeps its values.)
+    const PairSlot& src = lastq >= 0 ? slots[lastq] : S.jlast;
+    if (needJ)
+      for (int c = lane; c < N7; c += WAVE) pair_cols<NB>(S, src, i, j, c);
+    if (needQ) {
+      double q1[14];
+      for (int k = 0; k < 7; ++k) {
+        q1[k] = S.dqdt[7 * i + k];
+        q1[7 + k] = S.dqdt[7 * j + k];
+      }
+      if (lane < 14) {
+        double s2 = 0.0;
+        if (lane != 6 && lane != 13) {
+          const bool rowi = lane < 7;
+          const int rk = rowi ? lane : lane - 7;
+          for (int k = 0; k < 14; ++k) s2 = fmad(jij_coef(src, rowi, rk, k), q1[k], s2);
+        } else {
+          for (int k = 0; k < 14; ++k) s2 = fmad(0.0, q1[k], s2);
+        }
+        const double nd = (S.dqij[lane] * 0.5) + (q1[lane] + s2);
+        S.dqij[lane] = nd;
+        S.dqdt[lane < 7 ? 7 * i + lane : 7 * j + lane - 7] = nd;
+      }
+    }
+    wsync();
+  }
+  // keep the last pair with mass for a massless pair at the start of a later chunk
+  if (S.seq && lastq >= 0) {
+    const double* from = reinterpret_cast<const double*>(&slots[lastq]);
+    double* to = reinterpret_cast<double*>(&S.jlast);
+    for (int w = lane; w < (int)(sizeof(PairSlot) / sizeof(double)); w += WAVE) to[w] = from[w];
+    wsync();
+  }
+}
+
+template <int NB, bool G>
+__device__ __attribute__((always_inline)) inline void sweep(Smem<NB, G>& S, Swork<NB>& W, int dir, double hh, bool needJ,
+                                                            bool needQ, int lane) {
+  constexpr int CH = Swork<NB>::CH;
   const bool drift_first = (dir == 0);
   const int nlev = S.nlv[dir];
   const uint8_t* pl = S.plist[dir];
@@ -755,68 +887,9 @@ __device__ __attribute__((always_inline)) inline void sweep(Smem<NB>& S, int dir
 #pragma unroll 1
     for (int L = lev; L < lev_end; ++L) {
       const int w = lw[L];
-      const bool mine = lane < w;
-      if (mine) {
+      if (lane < w) {
         const int q = q0 + lane;
-        const int i = pl[2 * (c0 + q)], j = pl[2 * (c0 + q) + 1];
-        PairSlot& sl = S.u.slot[q];
-        // every LDS operand is read into registers before anything is
-        // written back: the compiler cannot tell the slot from the state
-        // arrays, so interleaved reads and writes would each wait for the
-        // previous write (a dozen LDS round trips in a row per level)
-        {
-          double xi[3], xj[3], vi[3], vj[3];
-          for (int k = 0; k < 3; ++k) {
-            xi[k] = S.x[3 * i + k];
-            xj[k] = S.x[3 * j + k];
-            vi[k] = S.v[3 * i + k];
-            vj[k] = S.v[3 * j + k];
-          }
-          for (int k = 0; k < 3; ++k) {
-            sl.x0[k] = xi[k] - xj[k];
-            sl.v0[k] = vi[k] - vj[k];
-          }
-        }
-        const double m_i = S.m[i], m_j = S.m[j];
-        const double gm = NBG_GNEWT * (m_i + m_j);
-        if (gm != 0.0) {
-          pair_drift(&sl, gm, hh, drift_first);
-          const double mijinv = 1.0 / (m_i + m_j);
-          const double mi = m_i * mijinv;
-          const double mj = m_j * mijinv;
-          double d[6], xi[3], xj[3], vi[3], vj[3], exi[3], exj[3], evi[3], evj[3];
-          for (int k = 0; k < 6; ++k) d[k] = sl.delxv[k];
-          for (int k = 0; k < 3; ++k) {
-            xi[k] = S.x[3 * i + k];
-            exi[k] = S.xe[3 * i + k];
-            xj[k] = S.x[3 * j + k];
-            exj[k] = S.xe[3 * j + k];
-            vi[k] = S.v[3 * i + k];
-            evi[k] = S.ve[3 * i + k];
-            vj[k] = S.v[3 * j + k];
-            evj[k] = S.ve[3 * j + k];
-          }
-          for (int k = 0; k < 3; ++k) {
-            kahan(xi[k], exi[k], mj * d[k]);
-            kahan(xj[k], exj[k], (-mi) * d[k]);
-          }
-          for (int k = 0; k < 3; ++k) {
-            kahan(vi[k], evi[k], mj * d[3 + k]);
-            kahan(vj[k], evj[k], (-mi) * d[3 + k]);
-          }
-          for (int k = 0; k < 3; ++k) {
-            S.x[3 * i + k] = xi[k];
-            S.xe[3 * i + k] = exi[k];
-            S.x[3 * j + k] = xj[k];
-            S.xe[3 * j + k] = exj[k];
-            S.v[3 * i + k] = vi[k];
-            S.ve[3 * i + k] = evi[k];
-            S.v[3 * j + k] = vj[k];
-            S.ve[3 * j + k] = evj[k];
-          }
-          sl.mi = mi; sl.mj = mj; sl.mijinv = mijinv; sl.m_i = m_i; sl.m_j = m_j;
-        }
-        sl.valid = (gm != 0.0); sl.i = i; sl.j = j;
+        pair_level_lane<NB>(S, W.slot[q], pl[2 * (c0 + q)], pl[2 * (c0 + q) + 1], hh, drift_first);
       }
       q0 += w;
       wsync();
@@ -825,56 +898,58 @@ __device__ __attribute__((always_inline)) inline void sweep(Smem<NB>& S, int dir
     c0 += cnt;
     if (!(needJ || needQ)) continue;
     // ---- B: analytic Kepler-drift Jacobians, one lane per pair
-    if (lane < cnt && S.u.slot[lane].valid) pair_jacobian(&S.u.slot[lane], drift_first);
+    if (lane < cnt && W.slot[lane].valid) pair_jacobian(&W.slot[lane], drift_first);
     wsync();
     // ---- C: jac_ij, Jacobian rows and dq/dt, in sweep order
-    int lastq = -1;  // last slot of this chunk whose pair has mass
+    sweep_rows<NB>(S, W.slot, cnt, needJ, needQ, lane);
+  }
+}
+
+// The same sweep for two systems of one wave (Duo) with the same level
+// schedule (same body count and pair kinds, no massless pair): phases A and
+// B serve both systems' pairs at once -- lanes [0, w) the first system's
+// pairs of a level, [w, 2w) the second's; phase C runs for each in turn.
+// Each system's slots are its own (slots[0, ch) and slots[ch, 2 ch)), and
+// every operation of a system is the one-system sweep's, so each system's
+// results are exactly its one-system results.
+template <int NB, bool G>
+__device__ __attribute__((always_inline)) inline void sweep2(Smem<NB, G>& S0, Smem<NB, G>& S1, PairSlot* slots, int ch,
+                                                             int dir, double hh0, double hh1, bool nJ0, bool nQ0,
+                                                             bool nJ1, bool nQ1, int lane) {
+  const bool drift_first = (dir == 0);
+  const int nlev = S0.nlv[dir];
+  const uint8_t* pl = S0.plist[dir];
+  const uint8_t* lw = S0.lvw[dir];
+  const bool pj0 = nJ0 || nQ0, pj1 = nJ1 || nQ1;
+  int lev = 0, c0 = 0;
+  while (lev < nlev) {
+    int cnt = 0, lev_end = lev;
+    while (lev_end < nlev && cnt + lw[lev_end] <= ch) cnt += lw[lev_end++];
+    int q0 = 0;
 #pragma unroll 1
-    for (int q = 0; q < cnt; ++q) {
-      const PairSlot& sl = S.u.slot[q];
-      const int i = sl.i, j = sl.j;
-      if (sl.valid) {
-        lastq = q;
-        if (needQ && lane < 6) {  // dqdt_ij (ahl21.jl:743-748)
-          S.dqij[lane] = sl.mj * sl.kep[lane + 42];
-          S.dqij[7 + lane] = (-sl.mi) * sl.kep[lane + 42];
-        }
+    for (int L = lev; L < lev_end; ++L) {
+      const int w = lw[L];
+      if (lane < 2 * w) {
+        const int sys = lane >= w ? 1 : 0;
+        const int q = q0 + lane - sys * w;
+        Smem<NB, G>& S = sys ? S1 : S0;
+        pair_level_lane<NB>(S, slots[sys * ch + q], pl[2 * (c0 + q)], pl[2 * (c0 + q) + 1], sys ? hh1 : hh0,
+                            drift_first);
       }
-      // (gm == 0: the reference returns before touching jac_ij / dqdt_ij and
-      // the caller reuses the previous values: the last pair with mass, or
-      // jlast when that lies in an earlier chunk; dqij keeps its values.)
-      const PairSlot& src = lastq >= 0 ? S.u.slot[lastq] : S.jlast;
-      if (needJ)
-        for (int c = lane; c < N7; c += WAVE) pair_cols<NB>(S, src, i, j, c);
-      if (needQ) {
-        double q1[14];
-        for (int k = 0; k < 7; ++k) {
-          q1[k] = S.dqdt[7 * i + k];
-          q1[7 + k] = S.dqdt[7 * j + k];
-        }
-        if (lane < 14) {
-          double s2 = 0.0;
-          if (lane != 6 && lane != 13) {
-            const bool rowi = lane < 7;
-            const int rk = rowi ? lane : lane - 7;
-            for (int k = 0; k < 14; ++k) s2 = fmad(jij_coef(src, rowi, rk, k), q1[k], s2);
-          } else {
-            for (int k = 0; k < 14; ++k) s2 = fmad(0.0, q1[k], s2);
-          }
-          const double nd = (S.dqij[lane] * 0.5) + (q1[lane] + s2);
-          S.dqij[lane] = nd;
-          S.dqdt[lane < 7 ? 7 * i + lane : 7 * j + lane - 7] = nd;
-        }
-      }
+      q0 += w;
       wsync();
     }
-    // keep the last pair with mass for a massless pair at the start of a later chunk
-    if (S.seq && lastq >= 0) {
-      const double* from = reinterpret_cast<const double*>(&S.u.slot[lastq]);
-      double* to = reinterpret_cast<double*>(&S.jlast);
-      for (int w = lane; w < (int)(sizeof(PairSlot) / sizeof(double)); w += WAVE) to[w] = from[w];
-      wsync();
+    lev = lev_end;
+    c0 += cnt;
+    if (!(pj0 || pj1)) continue;
+    if (lane < 2 * cnt) {
+      const int sys = lane >= cnt ? 1 : 0;
+      PairSlot* sl = slots + sys * ch + (lane - sys * cnt);
+      if ((sys ? pj1 : pj0) && sl->valid) pair_jacobian(sl, drift_first);
     }
+    wsync();
+    if (pj0) sweep_rows<NB>(S0, slots, cnt, nJ0, nQ0, lane);
+    if (pj1) sweep_rows<NB>(S1, slots + ch, cnt, nJ1, nQ1, lane);
   }
 }
 
@@ -884,7 +959,11 @@ __device__ __attribute__((always_inline)) inline void sweep(Smem<NB>& S, int dir
 // every flag combination (so grad and no-grad runs agree bit for bit, as the
 // reference's no-grad map ahl21_no_grad.jl:6-18 does).
 // ---------------------------------------------------------------------------
-template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, bool needQ, bool anyKick, int lane) {
+// The stages of one step between the pair sweeps: before the forward sweep
+// (kickfast! and the first drift), between the sweeps (phic!/phisalpha!),
+// after the backward sweep (the second drift and kickfast!).
+template <int NB, bool G>
+__device__ __attribute__((always_inline)) inline void step_pre(Smem<NB, G>& S, Swork<NB>& W, double h, bool needJ, bool needQ, bool anyKick, int lane) {
   constexpr int N3 = 3 * NB, N4 = 4 * NB, N7 = 7 * NB;
   const double h2 = 0.5 * h;
   const double h6 = h / 6.0;
@@ -901,7 +980,7 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
     }
   wsync();
   // ---- kickfast!(s, d, h/6)
-  if (anyKick) kick_build<NB>(S, h6, needQ, lane);
+  if (anyKick) kick_build<NB>(S, W, h6, needQ, lane);
   wsync();
   if (needQ) {
     // dqdt_kick ./= 6; tmp7n = K*dqdt (dqdt = 0 here); dqdt .+= dqdt_kick .+ tmp7n
@@ -910,7 +989,7 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
       if (anyKick) {
         const int b = r / 7, k = r - 7 * b;
         if (k >= 3 && k < 6) {
-          const double* Kr = S.u.f.dadq + (3 * b + k - 3) * N4;
+          const double* Kr = W.f.dadq + (3 * b + k - 3) * N4;
           for (int d = 0; d < NB; ++d) {
             tmp = fmad(Kr[3 * d + 0], S.dqdt[7 * d + 0], tmp);
             tmp = fmad(Kr[3 * d + 1], S.dqdt[7 * d + 1], tmp);
@@ -930,7 +1009,7 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
       load_pos<NB>(S, c, P);
       drift_cols<NB>(S, c, h2);
       zero_add_rows<NB>(S, c, true, !anyKick);
-      if (anyKick) vel_product_add<NB>(S, S.u.f.dadq, c, P);
+      if (anyKick) vel_product_add<NB>(S, W.f.dadq, c, P);
     }
   // ---- drift of the state (drift_grad! / drift!): entries are independent
   for (int q = lane; q < N3; q += WAVE) {
@@ -947,16 +1026,19 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
     }
   }
   wsync();
-  // ---- forward pair sweep (drift first)
-  sweep<NB>(S, 0, h2, needJ, needQ, lane);
+}
+
+template <int NB, bool G>
+__device__ __attribute__((always_inline)) inline void step_mid(Smem<NB, G>& S, Swork<NB>& W, double h, bool needJ, bool needQ, bool anyKick, int lane) {
+  constexpr int N3 = 3 * NB, N4 = 4 * NB, N7 = 7 * NB;
   // ---- phic!, phisalpha!
-  phi_build<NB>(S, h, anyKick, needQ, lane);
+  phi_build<NB>(S, W, h, anyKick, needQ, lane);
   wsync();
   if (needJ)
     for (int c = lane; c < N7; c += WAVE) {
       double P[N3];
       load_pos<NB>(S, c, P);
-      vel_product_add<NB>(S, S.u.f.phi, c, P);
+      vel_product_add<NB>(S, W.f.phi, c, P);
       zero_add_rows<NB>(S, c, true, false);
     }
   if (needQ) {
@@ -967,7 +1049,7 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
       double tmp = 0.0;
       const int b = r / 7, k = r - 7 * b;
       if (k >= 3 && k < 6) {
-        const double* Pr = S.u.f.phi + (3 * b + k - 3) * N4;
+        const double* Pr = W.f.phi + (3 * b + k - 3) * N4;
         for (int d = 0; d < NB; ++d) {
           tmp = fmad(Pr[3 * d + 0], S.dqdt[7 * d + 0], tmp);
           tmp = fmad(Pr[3 * d + 1], S.dqdt[7 * d + 1], tmp);
@@ -982,8 +1064,13 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
     for (int r = lane; r < N7; r += WAVE, ++nr) S.dqdt[r] = nv[nr];
   }
   wsync();
-  // ---- backward pair sweep (drift after)
-  sweep<NB>(S, 1, h2, needJ, needQ, lane);
+}
+
+template <int NB, bool G>
+__device__ __attribute__((always_inline)) inline void step_post(Smem<NB, G>& S, Swork<NB>& W, double h, bool needJ, bool needQ, bool anyKick, int lane) {
+  constexpr int N3 = 3 * NB, N4 = 4 * NB, N7 = 7 * NB;
+  const double h2 = 0.5 * h;
+  const double h6 = h / 6.0;
   // ---- drift
   if (needJ)
     for (int c = lane; c < N7; c += WAVE) drift_cols<NB>(S, c, h2);
@@ -1003,7 +1090,7 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
   }
   wsync();
   // ---- kickfast!(s, d, h/6)
-  if (anyKick) kick_build<NB>(S, h6, needQ, lane);
+  if (anyKick) kick_build<NB>(S, W, h6, needQ, lane);
   wsync();
   if (needQ) {
     double nv[2];
@@ -1013,7 +1100,7 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
       if (anyKick) {
         const int b = r / 7, k = r - 7 * b;
         if (k >= 3 && k < 6) {
-          const double* Kr = S.u.f.dadq + (3 * b + k - 3) * N4;
+          const double* Kr = W.f.dadq + (3 * b + k - 3) * N4;
           for (int d = 0; d < NB; ++d) {
             tmp = fmad(Kr[3 * d + 0], S.dqdt[7 * d + 0], tmp);
             tmp = fmad(Kr[3 * d + 1], S.dqdt[7 * d + 1], tmp);
@@ -1034,7 +1121,7 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
       if (anyKick) {
         double P[N3];
         load_pos<NB>(S, c, P);
-        vel_product_add<NB>(S, S.u.f.dadq, c, P);
+        vel_product_add<NB>(S, W.f.dadq, c, P);
         zero_add_rows<NB>(S, c, true, false);
       } else {
         zero_add_rows<NB>(S, c, true, true);
@@ -1043,21 +1130,47 @@ template <int NB> __device__ void ahl21_step(Smem<NB>& S, double h, bool needJ, 
   wsync();
 }
 
+template <int NB, bool G>
+__device__ void ahl21_step(Smem<NB, G>& S, Swork<NB>& W, double h, bool needJ, bool needQ, bool anyKick, int lane) {
+  step_pre<NB>(S, W, h, needJ, needQ, anyKick, lane);
+  sweep<NB>(S, W, 0, 0.5 * h, needJ, needQ, lane);  // forward pair sweep (drift first)
+  step_mid<NB>(S, W, h, needJ, needQ, anyKick, lane);
+  sweep<NB>(S, W, 1, 0.5 * h, needJ, needQ, lane);  // backward pair sweep (drift after)
+  step_post<NB>(S, W, h, needJ, needQ, anyKick, lane);
+}
+
+// One step of each of the two systems of a Duo, in lockstep: the stages
+// outside the sweeps run for one system, then the other (they share the
+// work area), the sweeps for both at once (sweep2).  The stage loops are
+// kept rolled so that their code exists once.
+template <int NB, bool G>
+__device__ void ahl21_step2(Smem<NB, G>& S0, Smem<NB, G>& S1, Swork<NB>& W, int ch, double h0, double h1,
+                            bool nJ0, bool nQ0, bool nJ1, bool nQ1, bool k0, bool k1, int lane) {
+#pragma unroll 1
+  for (int s = 0; s < 2; ++s) step_pre<NB>(s ? S1 : S0, W, s ? h1 : h0, s ? nJ1 : nJ0, s ? nQ1 : nQ0, s ? k1 : k0, lane);
+  sweep2<NB>(S0, S1, W.slot, ch, 0, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
+#pragma unroll 1
+  for (int s = 0; s < 2; ++s) step_mid<NB>(s ? S1 : S0, W, s ? h1 : h0, s ? nJ1 : nJ0, s ? nQ1 : nQ0, s ? k1 : k0, lane);
+  sweep2<NB>(S0, S1, W.slot, ch, 1, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
+#pragma unroll 1
+  for (int s = 0; s < 2; ++s) step_post<NB>(s ? S1 : S0, W, s ? h1 : h0, s ? nJ1 : nJ0, s ? nQ1 : nQ0, s ? k1 : k0, lane);
+}
+
 // ---------------------------------------------------------------------------
 // Transit search (timing.jl:3-110) and element gradients (timing.jl:112-138)
 // ---------------------------------------------------------------------------
-template <int NB> __device__ inline double gsky(const Smem<NB>& S, int i, int j) {
+template <int NB, bool G> __device__ inline double gsky(const Smem<NB, G>& S, int i, int j) {
   return (S.x[3 * j] - S.x[3 * i]) * (S.v[3 * j] - S.v[3 * i]) +
          (S.x[3 * j + 1] - S.x[3 * i + 1]) * (S.v[3 * j + 1] - S.v[3 * i + 1]);
 }
-template <int NB> __device__ inline double gdot(const Smem<NB>& S, int i, int j) {
+template <int NB, bool G> __device__ inline double gdot(const Smem<NB, G>& S, int i, int j) {
   return (((S.x[3 * j] - S.x[3 * i]) * (S.dqdt[7 * j + 3] - S.dqdt[7 * i + 3]) +
            (S.x[3 * j + 1] - S.x[3 * i + 1]) * (S.dqdt[7 * j + 4] - S.dqdt[7 * i + 4])) +
           (S.v[3 * j] - S.v[3 * i]) * (S.dqdt[7 * j + 0] - S.dqdt[7 * i + 0])) +
          (S.v[3 * j + 1] - S.v[3 * i + 1]) * (S.dqdt[7 * j + 1] - S.dqdt[7 * i + 1]);
 }
 
-template <int NB> __device__ inline void save_prior(Smem<NB>& S, int lane) {
+template <int NB, bool G> __device__ inline void save_prior(Smem<NB, G>& S, int lane) {
   constexpr int N3 = 3 * NB, N7 = 7 * NB;
   for (int q = lane; q < N3; q += WAVE) {
     S.px[q] = S.x[q]; S.pv[q] = S.v[q]; S.pxe[q] = S.xe[q]; S.pve[q] = S.ve[q];
@@ -1066,7 +1179,7 @@ template <int NB> __device__ inline void save_prior(Smem<NB>& S, int lane) {
   if (lane == 0) S.pt = S.t;
   wsync();
 }
-template <int NB> __device__ inline void restore_prior(Smem<NB>& S, int lane) {
+template <int NB, bool G> __device__ inline void restore_prior(Smem<NB, G>& S, int lane) {
   constexpr int N3 = 3 * NB, N7 = 7 * NB;
   for (int q = lane; q < N3; q += WAVE) {
     S.x[q] = S.px[q]; S.v[q] = S.pv[q]; S.xe[q] = S.pxe[q]; S.ve[q] = S.pve[q];
@@ -1076,8 +1189,8 @@ template <int NB> __device__ inline void restore_prior(Smem<NB>& S, int lane) {
   wsync();
 }
 
-template <int NB>
-__device__ void findtransit(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int ti, int jb, double dt0, int lane,
+template <int NB, bool G>
+__device__ void findtransit(Smem<NB, G>& S, Swork<NB>& W, const SysDesc& D, const SysBufs& B, int ti, int jb, double dt0, int lane,
                             double* ws) {
   const auto J_ = jj<NB>(S);
   constexpr int N6 = 6 * NB, N7 = 7 * NB;
@@ -1090,7 +1203,7 @@ __device__ void findtransit(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int
     tt2 = tt1;
     tt1 = dt0;
     restore_prior<NB>(S, lane);
-    ahl21_step<NB>(S, dt0, false, true, D.any_kick != 0, lane);
+    ahl21_step<NB>(S, W, dt0, false, true, D.any_kick != 0, lane);
     const double gs = gsky<NB>(S, ti, jb);
     const double gd = gdot<NB>(S, ti, jb);
     const double dt = -gs / gd;
@@ -1109,7 +1222,7 @@ __device__ void findtransit(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int
         ws[r * N7 + col] = J_[(unsigned)(r * N7 + col)];
         ws[N6 * N7 + r * N7 + col] = J_[(unsigned)(Smem<NB>::JOFF + r * N7 + col)];
       }
-    ahl21_step<NB>(S, dt0, true, true, D.any_kick != 0, lane);
+    ahl21_step<NB>(S, W, dt0, true, true, D.any_kick != 0, lane);
     const double gd = gdot<NB>(S, ti, jb);
     const double dx = S.x[3 * jb] - S.x[3 * ti], dy = S.x[3 * jb + 1] - S.x[3 * ti + 1];
     const double dvx = S.v[3 * jb] - S.v[3 * ti], dvy = S.v[3 * jb + 1] - S.v[3 * ti + 1];
@@ -1164,8 +1277,8 @@ __device__ void findtransit(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int
   wsync();
 }
 
-template <int NB>
-__device__ void detect_transits(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int lane, double* ws) {
+template <int NB, bool G>
+__device__ void detect_transits(Smem<NB, G>& S, Swork<NB>& W, const SysDesc& D, const SysBufs& B, int lane, double* ws) {
   // set_state!(s_prior, s) (timing.jl:6) is taken lazily, before the first
   // search of this step; a search leaves the state modified and the prior
   // state is restored right after it (timing.jl:19-20), so bodies without a
@@ -1187,7 +1300,7 @@ __device__ void detect_transits(Smem<NB>& S, const SysDesc& D, const SysBufs& B,
           saved = true;
         }
         const double dt0 = ((-gi) * D.hstep) / (gi - S.gsave[i]);
-        findtransit<NB>(S, D, B, ti, i, dt0, lane, ws);
+        findtransit<NB>(S, W, D, B, ti, i, dt0, lane, ws);
         restore_prior<NB>(S, lane);
       }
     }
@@ -1199,7 +1312,7 @@ __device__ void detect_transits(Smem<NB>& S, const SysDesc& D, const SysBufs& B,
 
 // calc_dtdelements! (timing.jl:112-138) for one system; jac_init is staged
 // transposed in the Jacobian's LDS space (the Jacobian is already stored).
-template <int NB> __device__ void dtdelements(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int lane) {
+template <int NB, bool G> __device__ void dtdelements(Smem<NB, G>& S, const SysDesc& D, const SysBufs& B, int lane) {
   const auto J_ = jj<NB>(S);
   constexpr int N7 = 7 * NB;
   const int n = NB;
@@ -1230,10 +1343,10 @@ template <int NB> __device__ void dtdelements(Smem<NB>& S, const SysDesc& D, con
 // ---------------------------------------------------------------------------
 // Load / store of a system between its HBM record and LDS.
 // ---------------------------------------------------------------------------
-template <int NB> __device__ void load_system(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int lane,
+template <int NB, bool G> __device__ void load_system(Smem<NB, G>& S, const SysDesc& D, const SysBufs& B, int lane,
                                               bool loadJ = true) {
   double* J_ = S.JJ;
-  if constexpr (Smem<NB>::JG) {
+  if constexpr (G) {
     J_ = B.ws + D.off_ws + 2 * (int64_t)(6 * NB) * (7 * NB);  // after the transit-step copy
     if (lane == 0) S.jg = J_;
   }
@@ -1315,7 +1428,7 @@ template <int NB> __device__ void load_system(Smem<NB>& S, const SysDesc& D, con
   wsync();
 }
 
-template <int NB> __device__ void store_system(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int lane) {
+template <int NB, bool G> __device__ void store_system(Smem<NB, G>& S, const SysDesc& D, const SysBufs& B, int lane) {
   const auto J_ = jj<NB>(S);
   constexpr int N3 = 3 * NB, N7 = 7 * NB;
   wsync();
@@ -1350,8 +1463,8 @@ template <int NB> __device__ void store_system(Smem<NB>& S, const SysDesc& D, co
 // xerror/verror, jac_step, jac_error (the mass rows, which no step changes:
 // unit rows of jac_step, zero rows of jac_error).  Out of line, so that the
 // step loop's register allocation is the same as without it.
-template <int NB>
-__device__ __attribute__((noinline)) void write_snapshot(const Smem<NB>& S, const SysDesc& D, const SysBufs& B,
+template <int NB, bool G>
+__device__ __attribute__((noinline)) void write_snapshot(const Smem<NB, G>& S, const SysDesc& D, const SysBufs& B,
                                                          int64_t k, int lane) {
   constexpr int N3 = 3 * NB, N7 = 7 * NB;
   const int64_t ns = D.nsteps;
@@ -1402,8 +1515,8 @@ __device__ inline double check_step_dev(double t0, double tmax) {
 // of calc_dbvdelements! (snapshot.jl:44-54), which the reference runs with
 // or without grad.  Out of line, so the step loop's register allocation
 // stays as it is.
-template <int NB>
-__device__ __attribute__((noinline)) void run_snapshot(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int lane) {
+template <int NB, bool G>
+__device__ __attribute__((noinline)) void run_snapshot(Smem<NB, G>& S, Swork<NB>& W, const SysDesc& D, const SysBufs& B, int lane) {
   constexpr int N7 = 7 * NB;
   const int n = NB;
   const int64_t nt = D.nsteps;
@@ -1423,8 +1536,8 @@ __device__ __attribute__((noinline)) void run_snapshot(Smem<NB>& S, const SysDes
     const double tmax = t0 + (hs * (double)ns);
     const bool land = (tmax != time);
 #pragma unroll 1
-    for (int64_t k = 0; k < ns; ++k) ahl21_step<NB>(S, hs, grad, grad && k + 1 == ns && !land, anyKick, lane);
-    if (land) ahl21_step<NB>(S, time - tmax, grad, grad, anyKick, lane);
+    for (int64_t k = 0; k < ns; ++k) ahl21_step<NB>(S, W, hs, grad, grad && k + 1 == ns && !land, anyKick, lane);
+    if (land) ahl21_step<NB>(S, W, time - tmax, grad, grad, anyKick, lane);
     if (lane == 0) S.t = time;
     wsync();
     const auto J_ = jj<NB>(S);
@@ -1478,14 +1591,14 @@ __device__ __attribute__((noinline)) void run_snapshot(Smem<NB>& S, const SysDes
 
 // Run (or resume) one system for at most `chunk` steps.  Returns true when
 // the system has finished.
-template <int NB>
-__device__ void run_system(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int64_t chunk, int lane) {
+template <int NB, bool G>
+__device__ void run_system(Smem<NB, G>& S, Swork<NB>& W, const SysDesc& D, const SysBufs& B, int64_t chunk, int lane) {
   load_system<NB>(S, D, B, lane);
   int64_t* ints = B.ints + D.off_int;
   const int64_t istep0 = ints[NB];
   if (istep0 > D.nsteps) return;  // finished earlier
   if (D.mode == MODE_SNAPSHOT) {
-    run_snapshot<NB>(S, D, B, lane);
+    run_snapshot<NB>(S, W, D, B, lane);
     return;
   }
   const int64_t iend = (chunk > 0 && istep0 + chunk < D.nsteps) ? istep0 + chunk : D.nsteps;
@@ -1508,15 +1621,15 @@ __device__ void run_system(Smem<NB>& S, const SysDesc& D, const SysBufs& B, int6
     // (set_state!, timing.jl:25-26)
     const bool last = (is + 1 == D.nsteps);
     const bool needQ = grad && last && (D.mode == MODE_TRANSITS || D.hlast == 0.0);
-    ahl21_step<NB>(S, D.h, grad, needQ, anyKick, lane);
+    ahl21_step<NB>(S, W, D.h, grad, needQ, anyKick, lane);
     if (D.mode == MODE_TRANSITS) {
       if (lane == 0) S.t = D.t0 + ((double)(is + 1) * D.h);
       wsync();
-      detect_transits<NB>(S, D, B, lane, ws);
+      detect_transits<NB>(S, W, D, B, lane, ws);
     }
   }
   const bool done = (iend == D.nsteps);
-  if (done && D.mode == MODE_STEPS && D.hlast != 0.0) ahl21_step<NB>(S, D.hlast, grad, grad, anyKick, lane);
+  if (done && D.mode == MODE_STEPS && D.hlast != 0.0) ahl21_step<NB>(S, W, D.hlast, grad, grad, anyKick, lane);
   store_system<NB>(S, D, B, lane);
   if (lane == 0) ints[NB] = done ? D.nsteps + 1 : iend;
   if (done && D.mode == MODE_TRANSITS && grad) dtdelements<NB>(S, D, B, lane);
@@ -1529,7 +1642,8 @@ __device__ __forceinline__ void dispatch_n(unsigned char* raw, const SysDesc& D,
                                            int lane) {
   if constexpr (N <= HI) {
     if (D.n == N) {
-      run_system<N>(*reinterpret_cast<Smem<N>*>(raw), D, B, chunk, lane);
+      Solo<N>& L = *reinterpret_cast<Solo<N>*>(raw);
+      run_system<N>(L.s, L.w, D, B, chunk, lane);
       return;
     }
     dispatch_n<N + 1, HI>(raw, D, B, chunk, lane);
@@ -1539,8 +1653,148 @@ __device__ __forceinline__ void dispatch_n(unsigned char* raw, const SysDesc& D,
 
 // LDS of a launch that may run any body count up to HI.
 template <int N> constexpr size_t lds_bytes_upto() {
-  if constexpr (N <= 2) return sizeof(Smem<2>);
-  else return sizeof(Smem<N>) > lds_bytes_upto<N - 1>() ? sizeof(Smem<N>) : lds_bytes_upto<N - 1>();
+  if constexpr (N <= 2) return sizeof(Solo<2>);
+  else return sizeof(Solo<N>) > lds_bytes_upto<N - 1>() ? sizeof(Solo<N>) : lds_bytes_upto<N - 1>();
+}
+
+// ---------------------------------------------------------------------------
+// Two systems per wave (a Duo), for transit runs of equal body count.  A
+// wave's lanes are mostly idle while it solves the Kepler pairs of a level
+// (at most n/2 of them), so one wave steps two systems in lockstep: their
+// Kepler drifts and pair Jacobians share the lanes (sweep2), the rest of each
+// step runs for one system, then the other.  The two records keep their
+// Jacobians in HBM (at every n) and share one work area, which takes the rest
+// of the launch's LDS and whose pair slots are split in halves.  A body count
+// is paired only when its Duo fits in the LDS of the launch's one-system
+// layouts (duo_fits), so the launch keeps its occupancy; other systems, and
+// pairs that cannot share a schedule, run one after the other as in nbg_run.
+// ---------------------------------------------------------------------------
+template <int N, int HI> struct Duo {
+  static constexpr size_t REC = sizeof(Smem<N, true>);
+  static constexpr size_t FREE = lds_bytes_upto<HI>() > 2 * REC ? lds_bytes_upto<HI>() - 2 * REC : 0;
+  // the work area takes what the launch's LDS leaves (at least one Swork<N>)
+  static constexpr size_t WBYTES = (FREE > sizeof(Swork<N>) ? FREE : sizeof(Swork<N>)) / 8 * 8;
+  Smem<N, true> s0, s1;  // Jacobians in HBM, whatever N
+  alignas(8) unsigned char w[WBYTES];
+};
+template <int N, int HI> constexpr bool duo_fits() { return sizeof(Duo<N, HI>) <= lds_bytes_upto<HI>(); }
+// slots per system in a Duo's shared work area
+template <int N, int HI> constexpr int duo_ch() { return (int)(Duo<N, HI>::WBYTES / sizeof(PairSlot)) / 2; }
+
+// The two systems' sweeps follow the same level schedule (and neither has a
+// massless pair, which keeps the sequential order).  Wave-uniform.
+template <int NB, bool G> __device__ inline bool same_schedule(const Smem<NB, G>& a, const Smem<NB, G>& b) {
+  bool same = a.seq == 0 && b.seq == 0;
+  for (int dir = 0; dir < 2 && same; ++dir) {
+    same = a.nlv[dir] == b.nlv[dir];
+    int np = 0;
+    for (int l = 0; l < a.nlv[dir] && same; ++l) {
+      same = a.lvw[dir][l] == b.lvw[dir][l];
+      np += a.lvw[dir][l];
+    }
+    for (int q = 0; q < 2 * np && same; ++q) same = a.plist[dir][q] == b.plist[dir][q];
+  }
+  return __builtin_amdgcn_readfirstlane(same ? 1 : 0) != 0;
+}
+
+// The step loop of run_system for MODE_TRANSITS, steps [is, iend).
+template <int NB, bool G>
+__device__ void transit_steps(Smem<NB, G>& S, Swork<NB>& W, const SysDesc& D, const SysBufs& B, int64_t is,
+                              int64_t iend, int lane) {
+  const bool anyKick = D.any_kick != 0;
+  const bool grad = D.grad != 0;
+  double* ws = B.ws + D.off_ws;
+#pragma unroll 1
+  for (; is < iend; ++is) {
+    const bool last = (is + 1 == D.nsteps);
+    ahl21_step<NB>(S, W, D.h, grad, grad && last, anyKick, lane);
+    if (lane == 0) S.t = D.t0 + ((double)(is + 1) * D.h);
+    wsync();
+    detect_transits<NB>(S, W, D, B, lane, ws);
+  }
+}
+
+// Two MODE_TRANSITS systems of NB bodies (or resume them for at most their
+// chunks of steps): in lockstep while both have steps left in this visit,
+// then the rest of the longer one alone; each ends as run_system ends it.
+// Every system's results are exactly its run_system results: the shared
+// stages run the same operations on its own record, the transit searches run
+// one system at a time, each on the whole work area.
+template <int NB, int HI>
+__device__ void run_pair(Duo<NB, HI>& P, const SysDesc& D0, const SysDesc& D1, const SysBufs& B, int64_t chunk0,
+                         int64_t chunk1, int lane) {
+  static_assert(duo_ch<NB, HI>() >= NB / 2, "a level of pairs must fit a system's slots");
+  Smem<NB, true>& S0 = P.s0;
+  Smem<NB, true>& S1 = P.s1;
+  Swork<NB>& W = *reinterpret_cast<Swork<NB>*>(P.w);
+  load_system<NB>(S0, D0, B, lane);
+  load_system<NB>(S1, D1, B, lane);
+  int64_t* ints0 = B.ints + D0.off_int;
+  int64_t* ints1 = B.ints + D1.off_int;
+  const int64_t st0 = ints0[NB], st1 = ints1[NB];
+  const bool on0 = st0 <= D0.nsteps, on1 = st1 <= D1.nsteps;  // not finished earlier
+  const int64_t e0 = !on0 ? st0 : (chunk0 > 0 && st0 + chunk0 < D0.nsteps) ? st0 + chunk0 : D0.nsteps;
+  const int64_t e1 = !on1 ? st1 : (chunk1 > 0 && st1 + chunk1 < D1.nsteps) ? st1 + chunk1 : D1.nsteps;
+  // gsave[i] = g!(i, ti, x, v) before the loop (Transits.jl:146-149)
+  if (lane == 0) {
+    if (on0 && st0 == 0)
+      for (int i = 0; i < NB; ++i)
+        if (i != D0.ti) S0.gsave[i] = gsky<NB>(S0, i, D0.ti);
+    if (on1 && st1 == 0)
+      for (int i = 0; i < NB; ++i)
+        if (i != D1.ti) S1.gsave[i] = gsky<NB>(S1, i, D1.ti);
+  }
+  wsync();
+  int64_t is0 = st0, is1 = st1;
+  if (on0 && on1 && same_schedule<NB>(S0, S1)) {
+    const bool k0 = D0.any_kick != 0, k1 = D1.any_kick != 0;
+    const bool g0 = D0.grad != 0, g1 = D1.grad != 0;
+    double* ws0 = B.ws + D0.off_ws;
+    double* ws1 = B.ws + D1.off_ws;
+#pragma unroll 1
+    for (; is0 < e0 && is1 < e1; ++is0, ++is1) {
+      ahl21_step2<NB>(S0, S1, W, duo_ch<NB, HI>(), D0.h, D1.h, g0, g0 && is0 + 1 == D0.nsteps, g1,
+                      g1 && is1 + 1 == D1.nsteps, k0, k1, lane);
+      if (lane == 0) {
+        S0.t = D0.t0 + ((double)(is0 + 1) * D0.h);
+        S1.t = D1.t0 + ((double)(is1 + 1) * D1.h);
+      }
+      wsync();
+      detect_transits<NB>(S0, W, D0, B, lane, ws0);
+      detect_transits<NB>(S1, W, D1, B, lane, ws1);
+    }
+  }
+  transit_steps<NB>(S0, W, D0, B, is0, e0, lane);
+  transit_steps<NB>(S1, W, D1, B, is1, e1, lane);
+  if (on0) {
+    store_system<NB>(S0, D0, B, lane);
+    if (lane == 0) ints0[NB] = (e0 == D0.nsteps) ? D0.nsteps + 1 : e0;
+    if (e0 == D0.nsteps && D0.grad) dtdelements<NB>(S0, D0, B, lane);
+  }
+  if (on1) {
+    store_system<NB>(S1, D1, B, lane);
+    if (lane == 0) ints1[NB] = (e1 == D1.nsteps) ? D1.nsteps + 1 : e1;
+    if (e1 == D1.nsteps && D1.grad) dtdelements<NB>(S1, D1, B, lane);
+  }
+}
+
+// Run D0 and D1 (equal body count) as a Duo; false when that count is not
+// paired in a launch of class HI.
+template <int N, int HI>
+__device__ __forceinline__ bool dispatch_pair(unsigned char* raw, const SysDesc& D0, const SysDesc& D1,
+                                              const SysBufs& B, int64_t chunk0, int64_t chunk1, int lane) {
+  if constexpr (N <= HI) {
+    if (D0.n == N) {
+      if constexpr (duo_fits<N, HI>()) {
+        run_pair<N, HI>(*reinterpret_cast<Duo<N, HI>*>(raw), D0, D1, B, chunk0, chunk1, lane);
+        return true;
+      } else {
+        return false;
+      }
+    }
+    return dispatch_pair<N + 1, HI>(raw, D0, D1, B, chunk0, chunk1, lane);
+  }
+  return false;
 }
 
 // ---------------------------------------------------------------------------
@@ -1578,8 +1832,8 @@ __device__ __forceinline__ void ring_store(int* p, int v) {
 }
 
 // Wave 0: hand one transit search to wave 1.
-template <int NB, int HI>
-__device__ void push_transit(Smem<NB>& S, const SysDesc& D, TransitRing<HI>& R, double* jobJ, int jb, double dt0,
+template <int NB, int HI, bool G>
+__device__ void push_transit(Smem<NB, G>& S, const SysDesc& D, TransitRing<HI>& R, double* jobJ, int jb, double dt0,
                              int64_t c, int lane, unsigned long long* err, int64_t spin_slot) {
   constexpr int N3 = 3 * NB, N6 = 6 * NB, N7 = 7 * NB;
   const int tail = R.tail;  // only this wave writes it
@@ -1625,8 +1879,8 @@ __device__ void push_transit(Smem<NB>& S, const SysDesc& D, TransitRing<HI>& R, 
 // most `chunk` steps of it) with the transit searches handed to wave 1
 // (detect_transits' tests and counting).  The kernel sets `done` after it
 // returns, so the searches of the visit are all finished when it ends.
-template <int NB, int HI>
-__device__ void coop_integrate(Smem<NB>& S, const SysDesc& D, const SysBufs& B, TransitRing<HI>& R, double* jobJ,
+template <int NB, int HI, bool G>
+__device__ void coop_integrate(Smem<NB, G>& S, Swork<NB>& W, const SysDesc& D, const SysBufs& B, TransitRing<HI>& R, double* jobJ,
                                int64_t chunk, int lane, unsigned long long* err, int64_t spin_slot) {
   load_system<NB>(S, D, B, lane);
   int64_t* ints = B.ints + D.off_int;
@@ -1644,7 +1898,7 @@ __device__ void coop_integrate(Smem<NB>& S, const SysDesc& D, const SysBufs& B, 
 #pragma unroll 1
     for (int64_t is = istep0; is < iend; ++is) {
       const bool last = (is + 1 == D.nsteps);
-      ahl21_step<NB>(S, D.h, grad, grad && last, anyKick, lane);  // dq/dt as run_system forms it
+      ahl21_step<NB>(S, W, D.h, grad, grad && last, anyKick, lane);  // dq/dt as run_system forms it
       if (lane == 0) S.t = D.t0 + ((double)(is + 1) * D.h);
       wsync();
       const int ti = D.ti;
@@ -1676,8 +1930,8 @@ __device__ void coop_integrate(Smem<NB>& S, const SysDesc& D, const SysBufs& B, 
 // Wave 1: run the pushed transit searches until wave 0 is done and the ring
 // is empty.  Its layout holds the system's masses, pair mask and level
 // schedule; state and J come from each job.
-template <int NB, int HI>
-__device__ void coop_search(Smem<NB>& S, const SysDesc& D, const SysBufs& B, TransitRing<HI>& R, double* jobJ,
+template <int NB, int HI, bool G>
+__device__ void coop_search(Smem<NB, G>& S, Swork<NB>& W, const SysDesc& D, const SysBufs& B, TransitRing<HI>& R, double* jobJ,
                             int lane, unsigned long long* err, int64_t spin_job) {
   constexpr int N3 = 3 * NB, N6 = 6 * NB, N7 = 7 * NB;
   load_system<NB>(S, D, B, lane, false);
@@ -1720,7 +1974,7 @@ __device__ void coop_search(Smem<NB>& S, const SysDesc& D, const SysBufs& B, Tra
       S.count[jb] = c + 1;
     }
     double* src = jobJ + (int64_t)slot * 2 * N6 * N7;
-    if constexpr (Smem<NB>::JG) {
+    if constexpr (G) {
       if (lane == 0) S.jg = src;  // the snapshot is this search's Jacobian store
     } else if (D.grad) {
       for (int col = lane; col < N7; col += WAVE)
@@ -1730,7 +1984,7 @@ __device__ void coop_search(Smem<NB>& S, const SysDesc& D, const SysBufs& B, Tra
         }
     }
     wsync();
-    findtransit<NB>(S, D, B, D.ti, jb, dt0, lane, ws);
+    findtransit<NB>(S, W, D, B, D.ti, jb, dt0, lane, ws);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     if (lane == 0) ring_store(&R.head, head + 1);
     head += 1;
@@ -1744,9 +1998,9 @@ __device__ __forceinline__ void dispatch_coop(unsigned char* raw, int wave, cons
                                               unsigned long long* err, const CoopBufs& C) {
   if constexpr (N <= HI) {
     if (D.n == N) {
-      Smem<N>& S = *reinterpret_cast<Smem<N>*>(raw);
-      if (wave == 0) coop_integrate<N, HI>(S, D, B, R, jobJ, chunk, lane, err, C.spin_slot);
-      else coop_search<N, HI>(S, D, B, R, jobJ, lane, err, C.spin_job);
+      Solo<N>& L = *reinterpret_cast<Solo<N>*>(raw);
+      if (wave == 0) coop_integrate<N, HI>(L.s, L.w, D, B, R, jobJ, chunk, lane, err, C.spin_slot);
+      else coop_search<N, HI>(L.s, L.w, D, B, R, jobJ, lane, err, C.spin_job);
       return;
     }
     dispatch_coop<N + 1, HI>(raw, wave, D, B, R, jobJ, chunk, lane, err, C);
@@ -1757,7 +2011,7 @@ template <int N, int HI>
 __device__ __forceinline__ void dispatch_dtdel(unsigned char* raw, const SysDesc& D, const SysBufs& B, int lane) {
   if constexpr (N <= HI) {
     if (D.n == N) {
-      dtdelements<N>(*reinterpret_cast<Smem<N>*>(raw), D, B, lane);
+      dtdelements<N>(reinterpret_cast<Solo<N>*>(raw)->s, D, B, lane);
       return;
     }
     dispatch_dtdel<N + 1, HI>(raw, D, B, lane);
@@ -1865,6 +2119,47 @@ __global__ __launch_bounds__(WAVE) void nbg_run(GroupList gl, SysBufs B, int64_t
     dispatch_n<2, HI>(raw, D, B, D.chunk > 0 ? D.chunk : chunk, lane);
     wsync();
     if (lane == 0) visit_done(D, ints, before, tr0, it0, B, units);
+  }
+}
+
+// Persistent kernel of launch class HI with two systems per wave: every
+// workgroup takes two consecutive systems of its queue at a time (the host
+// orders the queues so that systems of a paired body count come in pairs)
+// and runs them as a Duo when they qualify, else one after the other as
+// nbg_run does.  LDS and the work accounting are nbg_run's.
+template <int HI>
+__global__ __launch_bounds__(WAVE) void nbg_run_duo(GroupList gl, SysBufs B, int64_t chunk,
+                                                    unsigned long long* __restrict__ units) {
+  __shared__ __attribute__((aligned(16))) unsigned char raw[lds_bytes_upto<HI>()];
+  const int lane = threadIdx.x;
+  int g = 0;
+  while (g < gl.ng) {
+    int idx = 0;
+    if (lane == 0) idx = atomicAdd(gl.queue[g], 2);
+    idx = __builtin_amdgcn_readfirstlane(idx);  // wave-uniform, as in nbg_run
+    if (idx >= gl.nsys[g]) {
+      ++g;
+      continue;
+    }
+    const bool two = idx + 1 < gl.nsys[g];
+    const SysDesc D0 = gl.descs[g][idx];
+    const SysDesc D1 = gl.descs[g][two ? idx + 1 : idx];
+    const int64_t* ints0 = B.ints + D0.off_int;
+    const int64_t* ints1 = B.ints + D1.off_int;
+    const int64_t before0 = ints0[D0.n], tr00 = ints0[D0.n + 1], it00 = ints0[D0.n + 2];
+    const int64_t before1 = ints1[D1.n], tr01 = ints1[D1.n + 1], it01 = ints1[D1.n + 2];
+    const int64_t c0 = D0.chunk > 0 ? D0.chunk : chunk, c1 = D1.chunk > 0 ? D1.chunk : chunk;
+    const bool pairable = two && D0.n == D1.n && D0.mode == MODE_TRANSITS && D1.mode == MODE_TRANSITS;
+    if (!(pairable && dispatch_pair<2, HI>(raw, D0, D1, B, c0, c1, lane))) {
+      dispatch_n<2, HI>(raw, D0, B, c0, lane);
+      wsync();
+      if (two) dispatch_n<2, HI>(raw, D1, B, c1, lane);
+    }
+    wsync();
+    if (lane == 0) {
+      visit_done(D0, ints0, before0, tr00, it00, B, units);
+      if (two) visit_done(D1, ints1, before1, tr01, it01, B, units);
+    }
   }
 }
 

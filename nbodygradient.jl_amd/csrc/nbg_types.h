@@ -8,8 +8,9 @@
 //                   t  (scalars at the end, STATE_T0_SLOT..)
 //  jac    (double): jac_step[7n*7n] jac_error[7n*7n] jac_init[7n*7n]
 //  ws     (double): two compressed 6n x 7n blocks (Jacobian copy during the
-//                   final gradient step of a transit); for n >= JGLOBAL_MIN two
-//                   more: the Jacobian and its error during the run
+//                   final gradient step of a transit); for n >= JGLOBAL_MIN, and
+//                   in every batch record, two more: the Jacobian and its error
+//                   during the run when they are kept in HBM
 //  tt     (double): ntbv x n x ntt   (TransitTiming.tt / TransitParameters.ttbv)
 //  dq     (double): ntbv x n x ntt x 7 x n (dtdq0); del (double) same (dtdelements,
 //                   at off_del)
@@ -117,6 +118,9 @@ NBG_TD int64_t jac_len(int n) { return 3 * 49 * (int64_t)n * n; }
 // error) in an HBM work area at the end of their ws record instead of LDS.
 constexpr int JGLOBAL_MIN = 7;
 NBG_TD int64_t ws_len(int n) { return (n >= JGLOBAL_MIN ? 4 : 2) * 42 * (int64_t)n * n; }
+// Batch records: room for the Jacobian in HBM at every n (two systems per
+// wave keep it there; nbg_run_duo).
+NBG_TD int64_t ws_len_batch(int n) { return 4 * 42 * (int64_t)n * n; }
 NBG_TD int64_t int_len(int n) { return (int64_t)n + 3; }
 
 // Trajectory record of a MODE_CARTESIAN run of ns steps: the state before

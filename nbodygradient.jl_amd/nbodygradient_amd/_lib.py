@@ -88,6 +88,9 @@ def lib():
     L.nbg_server_footprint.argtypes = [C.c_int32, C.c_int32, C.c_int64]
     L.nbg_server_footprint.restype = C.c_int64
     L.nbg_copy_calibrate.argtypes = [VP, C.c_int64, C.c_int32, D]
+    L.nbg_set_systems_per_wave.argtypes = [VP, C.c_int32]
+    L.nbg_batch_autotune_systems_per_wave.argtypes = [C.c_int32]
+    L.nbg_batch_autotune_systems_per_wave.restype = C.c_int32
     L.nbg_probe_row_update.argtypes = [VP, C.c_int32, C.c_int32, C.c_int32, D, D]
     L.nbg_libm_eval_device.argtypes = [VP, C.c_int32, C.c_int64, D, D]
     L.nbg_selftest_pair_host.argtypes = [D, D, C.c_double, C.c_double, C.c_int32, D]
@@ -155,6 +158,13 @@ class Context:
 
     def set_grid_cap(self, grid_cap=0):
         self.check(lib().nbg_set_grid_cap(self._h, int(grid_cap)), "nbg_set_grid_cap")
+
+    def set_systems_per_wave(self, k=1):
+        """Batch runs and rounds: 1 system per wave, or 2 (transit runs of
+        equal body count, n <= 9) in lockstep with their Kepler pairs sharing
+        the wave's lanes; results are the same bits either way (C ABI:
+        nbg_set_systems_per_wave)."""
+        self.check(lib().nbg_set_systems_per_wave(self._h, int(k)), "nbg_set_systems_per_wave")
 
     def set_coop(self, mode=1, max_systems=128):
         """Two-wave workgroups for the longest transit-search systems of batch

@@ -95,8 +95,8 @@ class Batch:
         """nbg_batch_autotune: time a copy of this batch with every step count
         scaled by frac under each candidate launch configuration (two-wave
         workgroups off / automatic / for the 16, 32, 64 longest systems;
-        four or three one-wave workgroups per CU) and keep the fastest in the
-        context.  Returns the candidates with their times and the choice."""
+        four or three one-wave workgroups per CU; one or two systems per
+        wave) and keep the fastest in the context.  Returns the candidates with their times and the choice."""
         L = _lib.lib()
         nc = L.nbg_batch_autotune_candidate(-1, None, None, None)
         ms = np.zeros(nc)
@@ -107,7 +107,8 @@ class Batch:
         for k in range(nc):
             a, b_, g = C.c_int32(), C.c_int32(), C.c_int32()
             L.nbg_batch_autotune_candidate(k, C.byref(a), C.byref(b_), C.byref(g))
-            cands.append({"coop_mode": a.value, "coop_max": b_.value, "groups_per_cu": g.value, "ms": float(ms[k])})
+            cands.append({"coop_mode": a.value, "coop_max": b_.value, "groups_per_cu": g.value,
+                          "systems_per_wave": L.nbg_batch_autotune_systems_per_wave(k), "ms": float(ms[k])})
         return {"frac": frac, "best": best.value, "candidates": cands}
 
     def autotune_round(self, steps):
@@ -124,7 +125,8 @@ class Batch:
         for k in range(nc):
             a, b_, g = C.c_int32(), C.c_int32(), C.c_int32()
             L.nbg_batch_autotune_candidate(k, C.byref(a), C.byref(b_), C.byref(g))
-            cands.append({"coop_mode": a.value, "coop_max": b_.value, "groups_per_cu": g.value, "ms": float(ms[k])})
+            cands.append({"coop_mode": a.value, "coop_max": b_.value, "groups_per_cu": g.value,
+                          "systems_per_wave": L.nbg_batch_autotune_systems_per_wave(k), "ms": float(ms[k])})
         return {"round_steps": int(steps), "best": best.value, "candidates": cands}
 
     def _offsets(self):

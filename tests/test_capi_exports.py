@@ -79,7 +79,9 @@ def test_autotune_candidates_start_with_the_default():
     for k in range(nc):
         a, b, g = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         assert L.nbg_batch_autotune_candidate(k, ctypes.byref(a), ctypes.byref(b), ctypes.byref(g)) == nc
-        cands.append((a.value, b.value, g.value))
-    assert cands[0] == (1, 128, 0)
+        cands.append((a.value, b.value, g.value, L.nbg_batch_autotune_systems_per_wave(k)))
+    assert cands[0] == (1, 128, 0, 1)
     assert any(c[0] == 0 for c in cands)
+    assert {c[3] for c in cands} == {1, 2}  # one and two systems per wave are both tried
+    assert L.nbg_batch_autotune_systems_per_wave(nc) == -1
     assert len(set(cands)) == nc

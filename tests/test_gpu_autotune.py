@@ -42,3 +42,4 @@ def test_autotune_picks_a_candidate_and_results_do_not_change():
     finally:
         ctx.set_coop(1, 128)
         ctx.set_launch(0)
+        ctx.set_systems_per_wave(1)

@@ -114,4 +114,5 @@ def test_autotune_round_keeps_results():
     ctx = _lib.default_context()
     ctx.set_coop(1, 128)
     ctx.set_launch(0)
+    ctx.set_systems_per_wave(1)
     b.close()
