@@ -45,11 +45,12 @@ sys.path.insert(0, os.path.join(ROOT, "nbodygradient.jl_amd"))
 import numpy as np  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
-# HBM bytes per system-step of nbg_run<9> in rounds of the 4096-system batch,
-# from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the same counters'
-# reading of nbg_copy_kernel of known size (the figures and the commands are in
-# profiles/nbg_run_c2_round_counters.md)
-HBM_BYTES_PER_UNIT = 373.3e3
+# HBM bytes per system-step of the dominant launch in rounds of the 4096-system
+# batch, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the same
+# counters' reading of nbg_copy_kernel of known size (the figures and the
+# commands are in profiles/nbg_run_c2_round_counters.md and
+# profiles/nbg_run_duo_c2_round_counters.md)
+HBM_BYTES_PER_UNIT = {"nbg_run<9>": 373.3e3, "nbg_run_duo<9>": 514.6e3}
 METRIC = "system-steps/s w/ Jacobian, batched vs plain GPU; p50/p99 request latency"
 T_START = time.perf_counter()
 
@@ -237,6 +238,8 @@ def main():
                     help="wall-clock budget of the whole run: a side leg that would not fit is dropped")
     ap.add_argument("--no-side", action="store_true", help="headline only")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--systems-per-wave", type=int, default=0, choices=[0, 1, 2],
+                    help="0: the autotuner's choice (1 without it); 1 or 2: forced")
     ap.add_argument("--no-makespan", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -283,7 +286,9 @@ def main():
         c = tune["candidates"][tune["best"]]
         log("autotune (one round of %d steps per candidate, %.1f s): %s ms -> coop mode %d max %d, groups/CU %d" %
             (K, time.perf_counter() - t, ", ".join("%.0f" % q["ms"] for q in tune["candidates"]),
-             c["coop_mode"], c["coop_max"], c["groups_per_cu"]))
+             c["coop_mode"], c["coop_max"], c["groups_per_cu"]) + ", systems per wave %d" % c["systems_per_wave"])
+    spw = args.systems_per_wave or (tune["candidates"][tune["best"]]["systems_per_wave"] if tune else 1)
+    ctx.set_systems_per_wave(spw)
     for w in range(args.warmup):
         st = batch.round(K)
         log("warmup round %d: %.1f ms device, %d system-steps" % (w, st["kernel_ms"], st["system_steps"]))
@@ -313,7 +318,7 @@ def main():
     lrep = {}
     for cls, L in launches.items():
         flops = sum(W(n) * u for n, u in L["units"].items())
-        kname = "nbg_run_coop<10>" if cls == 1 else "nbg_run<%d>" % cls
+        kname = "nbg_run_coop<10>" if cls == 1 else ("nbg_run_duo<%d>" if spw == 2 else "nbg_run<%d>") % cls
         lrep[str(cls)] = {"kernel": kname, "ms_total": L["ms"], "ms_per_round": L["ms"] / args.steps,
                           "grid": L["grid"], "units": sum(L["units"].values()),
                           "units_by_n": {str(n): u for n, u in sorted(L["units"].items())}, "flops": flops,
@@ -326,8 +331,9 @@ def main():
         log("launch %s: grid %d, %.1f ms per round, %d units %s, %.3f TFLOP/s" %
             (L["kernel"], L["grid"], L["ms_per_round"], L["units"], L["units_by_n"], L["tflops"]))
     traffic = None
-    if HBM_BYTES_PER_UNIT is not None:
-        traffic = HBM_BYTES_PER_UNIT * D["units"] / args.steps
+    bytes_unit = HBM_BYTES_PER_UNIT.get(D["kernel"])
+    if bytes_unit is not None:
+        traffic = bytes_unit * D["units"] / args.steps
     out = {
         "metric": METRIC,
         "value": value,
@@ -346,15 +352,19 @@ def main():
                                "system advances %d integrator steps (finished systems restart)" % K,
                    "systems": len(specs), "round_steps": K, "system_steps_timed": units,
                    "transits_timed": trans, "newton_iters_timed": newton,
-                   "parallelism": "1 GPU, one persistent launch per body-count class (n <= 9 in one)",
+                   "parallelism": "1 GPU, one persistent launch per body-count class (n <= 9 in one), "
+                                  "%d system%s per wave" % (spw, "s" if spw > 1 else ""),
+                   "systems_per_wave": spw,
                    "autotune": tune},
         "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": D["kernel"], "launch_ms": D["ms_per_round"], "units_per_launch": D["units"] / args.steps,
                      "note": "FP64 vector (VALU) peak 78.6 TFLOP/s; no MFMA in the kernel (a measured pair-update "
                              "probe ran 28 % slower on the FP64 matrix cores); W(n) = 2016 n^2 (n-1) + 126 n^3 flops "
-                             "per system-step; traffic = 373.3 KB per system-step (calibrated rocprofv3 FETCH_SIZE + "
-                             "WRITE_SIZE of these rounds) x the launch's system-steps per round"},
+                             "per system-step; traffic = " +
+                             ("%.1f KB per system-step (calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE of these rounds) "
+                              "x the launch's system-steps per round" % (bytes_unit / 1e3) if bytes_unit else
+                              "not measured for this kernel")},
         "launches": lrep,
         "transits_with_gradients_per_s": trans / t_timed,
     }

@@ -146,6 +146,7 @@ typedef struct {
   double launch_ms[13];
   int64_t launch_units[13][13];
   int32_t launch_grid[13];  /* workgroups (one wave each) of that launch */
+  int32_t paired_systems;   /* systems the queues paired two per wave (nbg_set_systems_per_wave) */
 } nbg_batch_stats;
 
 int nbg_batch_create(nbg_ctx* ctx, int32_t nsys, const int32_t* n, const double* h, const double* tmax,

@@ -80,6 +80,7 @@ struct Store {
   DevVec<int64_t> ints;
   DevVec<uint8_t> pair;
   DevVec<nbg::SysDesc> desc;
+  DevVec<int> ustart;  // units of the two-per-wave queues (Layout::ustart)
   DevVec<int> queue;
   DevVec<unsigned long long> units;  // work units completed, [launch][n]
   DevVec<double> coop;               // Jacobian snapshot slots of the cooperative workgroups
@@ -87,7 +88,7 @@ struct Store {
   void release() {
     state.release(); state0.release(); jac.release(); jac0.release(); ws.release();
     tt.release(); dq.release(); del.release(); ints.release(); pair.release(); desc.release(); queue.release();
-    units.release(); coop.release(); traj.release();
+    units.release(); coop.release(); traj.release(); ustart.release();
   }
 };
 
@@ -101,6 +102,10 @@ struct Layout {
   std::vector<double> gcost;
   std::vector<double> cost;  // optional per-system cost (default: from nsteps)
   int pairs = 0;             // queues ordered for two systems per wave (nbg_run_duo)
+  int wave_slots = 0;        // one-wave workgroups the GPU holds at once (pairing keeps at least as many units)
+  // units of the two-per-wave queues: for group g, ustart[uoff[g] ..] are the
+  // offsets (from the group's start) of its units, then the group's count
+  std::vector<int> ustart, uoff;
 };
 
 double check_step(double t0, double tmax) {  // Integrator.jl:249-259
@@ -173,50 +178,61 @@ void group_layout(Layout& L) {
     L.gcnt.back() += 1;
     L.gcost.back() += cost(L.order[k]);
   }
+  L.ustart.clear();
+  L.uoff.clear();
   if (!L.pairs) return;
-  // Two systems per wave: a launch takes queue positions 2k and 2k + 1
-  // together.  Within each group, systems of a body count that runs in pairs
-  // are matched with the next system of the same count (longest with
-  // longest), every other system with the next such unmatched one; the
-  // units keep the longest-first order of their first system, and a unit of
-  // one system (at most one per group) goes last, so that every pair starts
-  // at an even position.
-  for (size_t g = 0; g < L.gn.size(); ++g) {
+  // Two systems per wave: the queues hand out units of one or two systems.
+  // Pairing trades latency for throughput -- a pair's two runs take about
+  // 0.9 x the sum of their one-wave times, each run that much longer -- so
+  // only systems of the same body count whose pair stays below the batch's
+  // work spread over every wave slot are paired, the shortest first, and only
+  // as many pairs as leave at least one unit per wave slot.  Units keep the
+  // longest-first order of their first system.
+  double total = 0.0;
+  for (int i = 0; i < ns; ++i) total += cost(i);
+  const double floor_cost = L.wave_slots > 0 ? total / L.wave_slots : total;
+  int64_t budget = L.wave_slots > 0 ? std::max<int64_t>(0, (int64_t)ns - L.wave_slots) : ns / 2;
+  std::vector<int> partner(ns, -1);  // by system index
+  for (size_t g = 0; g < L.gn.size() && budget > 0; ++g) {
     const int off = L.goff[g], cnt = L.gcnt[g], cls = L.gn[g];
-    std::vector<int> sub(L.order.begin() + off, L.order.begin() + off + cnt);
-    std::vector<std::vector<int>> pos(NB_MAX + 1);  // positions in sub by body count
-    for (int k = 0; k < cnt; ++k) pos[L.desc[sub[k]].n].push_back(k);
-    std::vector<size_t> cur(NB_MAX + 1, 0);
-    std::vector<char> used(cnt, 0);
-    std::vector<std::vector<int>> units;
-    int single = -1;  // unit holding one unmatched system, waiting for a second
-    for (int k = 0; k < cnt; ++k) {
-      if (used[k]) continue;
-      used[k] = 1;
-      const int n = L.desc[sub[k]].n;
-      if (nbg_duo_pairs(cls, n)) {
-        auto& pn = pos[n];
-        while (cur[n] < pn.size() && used[pn[cur[n]]]) ++cur[n];
-        if (cur[n] < pn.size()) {
-          const int j = pn[cur[n]++];
-          used[j] = 1;
-          units.push_back({sub[k], sub[j]});
-          continue;
-        }
+    std::vector<std::vector<int>> pos(NB_MAX + 1);  // systems by body count, longest first
+    for (int k = 0; k < cnt; ++k) pos[L.desc[L.order[off + k]].n].push_back(L.order[off + k]);
+    // the shortest systems first, across body counts
+    std::vector<int> at(NB_MAX + 1);
+    for (int n = NB_MIN; n <= NB_MAX; ++n) at[n] = (int)pos[n].size() - 1;
+    while (budget > 0) {
+      int bn = -1;
+      double bc = 0.0;
+      for (int n = NB_MIN; n <= NB_MAX; ++n) {
+        if (at[n] < 1 || !nbg_duo_pairs(cls, n)) continue;
+        const double c = cost(pos[n][at[n] - 1]);
+        if (bn < 0 || c < bc) { bn = n; bc = c; }
       }
-      if (single >= 0) {
-        units[single].push_back(sub[k]);
-        single = -1;
-      } else {
-        units.push_back({sub[k]});
-        single = (int)units.size() - 1;
+      if (bn < 0 || 1.8 * bc > floor_cost) break;
+      const int a = pos[bn][at[bn]], b = pos[bn][at[bn] - 1];
+      partner[a] = b;
+      partner[b] = a;
+      at[bn] -= 2;
+      --budget;
+    }
+  }
+  for (size_t g = 0; g < L.gn.size(); ++g) {
+    const int off = L.goff[g], cnt = L.gcnt[g];
+    std::vector<int> sub(L.order.begin() + off, L.order.begin() + off + cnt);
+    std::vector<char> placed(ns, 0);
+    L.uoff.push_back((int)L.ustart.size());
+    int at = off;
+    for (int i : sub) {
+      if (placed[i]) continue;
+      L.ustart.push_back(at - off);
+      L.order[at++] = i;
+      placed[i] = 1;
+      if (partner[i] >= 0) {
+        L.order[at++] = partner[i];
+        placed[partner[i]] = 1;
       }
     }
-    int at = off;
-    for (size_t u = 0; u < units.size(); ++u)
-      if ((int)u != single)
-        for (int i : units[u]) L.order[at++] = i;
-    if (single >= 0) L.order[at++] = units[single][0];
+    L.ustart.push_back(cnt);
   }
 }
 
@@ -261,6 +277,7 @@ struct nbg_batch {
   int64_t steps_main = 0;
   int64_t order_chunk = 0;  // the visit length the device queue order is for (0: whole runs)
   int order_pairs = 0;      // ... and whether it pairs systems (two per wave)
+  int order_slots = 0;      // ... and for how many wave slots
 };
 
 #define NBG_HIP(expr)                                                                 \
@@ -400,9 +417,14 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
         ++kg[g];
         ++K;
       }
-      if (L.pairs && (kg[g] & 1)) {  // keep the pairs of the one-wave queue at even positions
-        --kg[g];
-        --K;
+      if (L.pairs) {  // the cooperative systems must end on a unit boundary of the two-per-wave queue
+        const int* us = L.ustart.data() + L.uoff[g];
+        int u = 0;
+        while (us[u] < kg[g]) ++u;
+        if (us[u] != kg[g]) {
+          --kg[g];
+          --K;
+        }
       }
     }
   }
@@ -449,6 +471,15 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
       gl.nsys[gl.ng] = L.gcnt[h] - skip;
       gl.descs[gl.ng] = S.desc.p + L.goff[h] + skip;
       gl.queue[gl.ng] = S.queue.p + h;
+      if (L.pairs) {  // units from the first one after the cooperative systems; offsets from the group's start
+        const int* us = L.ustart.data() + L.uoff[h];
+        const int nu = (int)((h + 1 < G ? L.uoff[h + 1] : (int)L.ustart.size()) - L.uoff[h]) - 1;
+        int u0 = 0;
+        while (us[u0] < skip) ++u0;
+        gl.descs[gl.ng] = S.desc.p + L.goff[h];
+        gl.nunits[gl.ng] = nu - u0;
+        gl.ustart[gl.ng] = S.ustart.p + L.uoff[h] + u0;
+      }
       gl.ng += 1;
     }
     const int grid = grids[g];
@@ -1228,26 +1259,51 @@ int nbg_batch_reset(nbg_batch* b) {
 
 namespace {
 
+// One-wave workgroups the GPU holds at once for a batch's largest launch
+// class (as grid_for sizes its grid).
+int wave_slots(nbg_ctx* ctx, const Layout& L) {
+  int nmax = NB_MIN;
+  for (const auto& d : L.desc) nmax = std::max(nmax, (int)d.n);
+  int per_cu = (int)(163840 / std::max<size_t>(nbg_lds_bytes(nbg_class_of(nmax)), 1));
+  per_cu = std::max(1, std::min(per_cu, 8));
+  if (ctx->max_groups_per_cu > 0) per_cu = std::min(per_cu, ctx->max_groups_per_cu);
+  return ctx->grid_cap > 0 ? std::min(ctx->grid_cap, ctx->cu_count * per_cu) : ctx->cu_count * per_cu;
+}
+
+// Upload a layout's queue order (descriptors, and the units of two-per-wave
+// queues) to its store.
+int upload_order(nbg_ctx* ctx, Store& S, const Layout& L) {
+  std::vector<nbg::SysDesc> sorted(L.desc.size());
+  for (size_t k = 0; k < L.order.size(); ++k) sorted[k] = L.desc[L.order[k]];
+  NBG_HIP(hipMemcpy(S.desc.p, sorted.data(), sorted.size() * sizeof(nbg::SysDesc), hipMemcpyHostToDevice));
+  if (!L.ustart.empty()) {
+    NBG_HIP(S.ustart.ensure(L.ustart.size()));
+    NBG_HIP(hipMemcpy(S.ustart.p, L.ustart.data(), L.ustart.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  return 0;
+}
+
 // Order a batch's device queues for visits of `chunk` steps (0: whole runs):
 // longest first by the steps one visit takes, min(chunk, nsteps + 1), times
 // the per-step cost of the body count; the descriptors are uploaded again.
 int batch_order(nbg_batch* b, int64_t chunk) {
   nbg_ctx* ctx = b->ctx;
   const int pairs = ctx->systems_per_wave == 2 ? 1 : 0;
-  if (b->order_chunk == chunk && b->order_pairs == pairs) return 0;
   Layout& L = b->L;
+  const int slots = pairs ? wave_slots(ctx, L) : 0;
+  if (b->order_chunk == chunk && b->order_pairs == pairs && b->order_slots == slots) return 0;
   L.pairs = pairs;
   L.cost.clear();
   if (chunk > 0) {
     L.cost.resize(L.desc.size());
     for (size_t i = 0; i < L.desc.size(); ++i) L.cost[i] = (double)std::min<int64_t>(chunk, L.desc[i].nsteps + 1);
   }
+  L.wave_slots = slots;
   group_layout(L);
-  std::vector<nbg::SysDesc> sorted(L.desc.size());
-  for (size_t k = 0; k < L.order.size(); ++k) sorted[k] = L.desc[L.order[k]];
-  NBG_HIP(hipMemcpy(b->S.desc.p, sorted.data(), sorted.size() * sizeof(nbg::SysDesc), hipMemcpyHostToDevice));
+  if (int rc = upload_order(ctx, b->S, L)) return rc;
   b->order_chunk = chunk;
   b->order_pairs = pairs;
+  b->order_slots = slots;
   return 0;
 }
 
@@ -1281,6 +1337,12 @@ int fill_stats(nbg_batch* b, double ms, const std::vector<float>& gms, const std
     stats->system_steps = b->steps_main + (b->L.desc.empty() || !b->L.desc[0].grad ? 0 : tr);
   }
   stats->kernel_ms = ms;
+  if (b->L.pairs)
+    for (size_t g = 0; g < G; ++g) {
+      const int* us = b->L.ustart.data() + b->L.uoff[g];
+      for (int u = 0; us[u] < b->L.gcnt[g]; ++u)
+        if (us[u + 1] - us[u] == 2) stats->paired_systems += 2;
+    }
   for (size_t g = 0; g < G; ++g) {
     const int c = b->L.gn[g];
     stats->launch_ms[c] = gms[g];
@@ -1388,10 +1450,12 @@ int nbg_batch_autotune(nbg_batch* b, double frac, int32_t* best, double* cand_ms
     hipError_t e = hipSuccess;
     if (Ls.pairs != (kTune[c].systems_per_wave == 2 ? 1 : 0)) {  // queue order for this candidate
       Ls.pairs = kTune[c].systems_per_wave == 2 ? 1 : 0;
+      Ls.wave_slots = wave_slots(ctx, Ls);
       group_layout(Ls);
-      std::vector<nbg::SysDesc> sorted(Ls.desc.size());
-      for (size_t k = 0; k < Ls.order.size(); ++k) sorted[k] = Ls.desc[Ls.order[k]];
-      e = hipMemcpy(T.desc.p, sorted.data(), sorted.size() * sizeof(nbg::SysDesc), hipMemcpyHostToDevice);
+      if (int rc = upload_order(ctx, T, Ls)) {
+        keep.restore(ctx);
+        return done(rc);
+      }
     }
     if (e == hipSuccess) e = hipMemcpy(T.state.p, b->S.state0.p, Ls.n_state * sizeof(double), hipMemcpyDeviceToDevice);
     if (e == hipSuccess) e = hipMemcpy(T.jac.p, b->S.jac0.p, Ls.n_jac * sizeof(double), hipMemcpyDeviceToDevice);

@@ -2123,10 +2123,10 @@ __global__ __launch_bounds__(WAVE) void nbg_run(GroupList gl, SysBufs B, int64_t
 }
 
 // Persistent kernel of launch class HI with two systems per wave: every
-// workgroup takes two consecutive systems of its queue at a time (the host
-// orders the queues so that systems of a paired body count come in pairs)
-// and runs them as a Duo when they qualify, else one after the other as
-// nbg_run does.  LDS and the work accounting are nbg_run's.
+// workgroup takes one unit of its queue at a time -- one system, or two
+// systems the host paired (equal body count) -- and runs a pair as a Duo when
+// it qualifies, else one system after the other as nbg_run does.  LDS and the
+// work accounting are nbg_run's.
 template <int HI>
 __global__ __launch_bounds__(WAVE) void nbg_run_duo(GroupList gl, SysBufs B, int64_t chunk,
                                                     unsigned long long* __restrict__ units) {
@@ -2134,14 +2134,15 @@ __global__ __launch_bounds__(WAVE) void nbg_run_duo(GroupList gl, SysBufs B, int
   const int lane = threadIdx.x;
   int g = 0;
   while (g < gl.ng) {
-    int idx = 0;
-    if (lane == 0) idx = atomicAdd(gl.queue[g], 2);
-    idx = __builtin_amdgcn_readfirstlane(idx);  // wave-uniform, as in nbg_run
-    if (idx >= gl.nsys[g]) {
+    int u = 0;
+    if (lane == 0) u = atomicAdd(gl.queue[g], 1);
+    u = __builtin_amdgcn_readfirstlane(u);  // wave-uniform, as in nbg_run
+    if (u >= gl.nunits[g]) {
       ++g;
       continue;
     }
-    const bool two = idx + 1 < gl.nsys[g];
+    const int idx = __builtin_amdgcn_readfirstlane(gl.ustart[g][u]);
+    const bool two = __builtin_amdgcn_readfirstlane(gl.ustart[g][u + 1]) - idx == 2;
     const SysDesc D0 = gl.descs[g][idx];
     const SysDesc D1 = gl.descs[g][two ? idx + 1 : idx];
     const int64_t* ints0 = B.ints + D0.off_int;

@@ -65,6 +65,10 @@ struct GroupList {
   int32_t nsys[MAX_GROUPS];
   const SysDesc* descs[MAX_GROUPS];
   int* queue[MAX_GROUPS];
+  // two systems per wave (nbg_run_duo): the queue hands out units of one or
+  // two systems; unit u is descs[g][ustart[g][u] .. ustart[g][u + 1])
+  int32_t nunits[MAX_GROUPS];
+  const int* ustart[MAX_GROUPS];
 };
 
 struct SysBufs {

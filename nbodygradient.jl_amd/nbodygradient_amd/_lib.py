@@ -26,7 +26,7 @@ class NbgError(RuntimeError):
 class BatchStats(C.Structure):
     _fields_ = [("system_steps", C.c_int64), ("transits", C.c_int64), ("newton_iters", C.c_int64),
                 ("kernel_ms", C.c_double), ("launch_ms", C.c_double * 13), ("launch_units", (C.c_int64 * 13) * 13),
-                ("launch_grid", C.c_int32 * 13)]
+                ("launch_grid", C.c_int32 * 13), ("paired_systems", C.c_int32)]
 
 
 _lib = None

@@ -142,7 +142,7 @@ class Batch:
     @staticmethod
     def _stats(st):
         return {"system_steps": st.system_steps, "transits": st.transits, "newton_iters": st.newton_iters,
-                "kernel_ms": st.kernel_ms,
+                "kernel_ms": st.kernel_ms, "paired_systems": st.paired_systems,
                 "launches": {c: {"ms": st.launch_ms[c], "grid": st.launch_grid[c],
                                  "units": {n: st.launch_units[c][n] for n in range(13) if st.launch_units[c][n] > 0}}
                              for c in range(13) if st.launch_ms[c] > 0}}

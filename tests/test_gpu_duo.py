@@ -32,9 +32,16 @@ def _specs(ns, seed, steps_lo=60, steps_hi=420):
     return out
 
 
-def _run(specs, k, rounds=None, derivatives=True, states=None, **kw):
+# Pairing keeps at least one unit per wave slot and pairs only runs short
+# against the batch's work per slot, so batches this small pair systems only
+# on a small grid: every run here (both forms) uses `grid` workgroups.
+GRID = 2
+
+
+def _run(specs, k, rounds=None, derivatives=True, states=None, grid=GRID, **kw):
     ctx = nb.default_context()
     ctx.set_systems_per_wave(k)
+    ctx.set_grid_cap(grid)
     try:
         b = Batch(specs, states if states is not None else W.initial_states(specs, processes=1), **kw)
         if rounds is None:
@@ -42,11 +49,17 @@ def _run(specs, k, rounds=None, derivatives=True, states=None, **kw):
         else:
             st = None
             for K, restart in rounds:
-                b.round(K, restart=restart)
+                s_ = b.round(K, restart=restart)
+                st = s_ if st is None else st
         res = b.download(derivatives=derivatives)
         b.close()
     finally:
         ctx.set_systems_per_wave(1)
+        ctx.set_grid_cap(0)
+    if k == 2:
+        assert st["paired_systems"] > 0
+    else:
+        assert st["paired_systems"] == 0
     return st, res
 
 
@@ -76,7 +89,7 @@ def test_duo_paired_systems_match_oracle():
     print("gpu: duo vs oracle", flush=True)
     specs = _specs([9, 9, 7, 7, 2, 2], seed=42, steps_lo=150, steps_hi=300)
     states = W.initial_states(specs, processes=1)
-    _, res = _run(specs, 2)
+    _, res = _run(specs, 2, grid=1)
     for s, (x, v, jinit), r in zip(specs, states, res):
         os_ = O.OState(x, v, s.elements[:, 0], t=s.t0, jac_init=jinit)
         ott, odq, odel, oc = O.integrate_transits(os_, s.h, s.tmax, s.ntt(), grad=True)
@@ -137,6 +150,7 @@ def test_autotune_round_tries_two_systems_per_wave():
     specs = _specs(MIX, seed=46)
     states = W.initial_states(specs, processes=1)
     ctx = nb.default_context()
+    ctx.set_grid_cap(GRID)
     try:
         b = Batch(specs, states)
         b.run()
@@ -151,3 +165,24 @@ def test_autotune_round_tries_two_systems_per_wave():
         ctx.set_coop(1, 128)
         ctx.set_launch(0)
         ctx.set_systems_per_wave(1)
+        ctx.set_grid_cap(0)
+
+
+def test_pairing_keeps_a_unit_per_wave_slot():
+    """Without a grid cap a batch smaller than the GPU's wave slots is not
+    paired (pairs would leave slots idle); results are the same either way."""
+    print("gpu: duo pairing policy", flush=True)
+    specs = _specs([5] * 40, seed=47)
+    states = W.initial_states(specs, processes=1)
+    ctx = nb.default_context()
+    ctx.set_systems_per_wave(2)
+    try:
+        b = Batch(specs, states)
+        st = b.run()
+        assert st["paired_systems"] == 0
+        r2 = b.download()
+        b.close()
+    finally:
+        ctx.set_systems_per_wave(1)
+    _, r1 = _run(specs, 1, states=states)
+    _same(r1, r2)
