@@ -283,6 +283,198 @@ template <int NB, bool G> __device__ void kick_build(Smem<NB, G>& S, Swork<NB>& 
   }
 }
 
+// phisalpha! (ahl21.jl:558-700, alpha = 2) for one system, on the lanes
+// whose column index within the system is col = 0, 1, ..., its pair-factor
+// loops striding by STEP lanes: one system on a whole wave (STEP = WAVE,
+// col = lane) or two on its halves (STEP = 32, phi_build2).  col 0 writes
+// the system's uniform values (a, v, dqdt_phi).
+template <int NB, bool G, int STEP>
+__device__ __attribute__((always_inline)) inline void phisalpha_cols(Smem<NB, G>& S, Swork<NB>& W, double h, bool anyKick,
+                                                                     bool needQ, int col) {
+  constexpr int N3 = 3 * NB, N4 = 4 * NB;
+  const bool own = col < N4;
+  int myd = -1, myp = -1;
+  if (own) col_dp<NB>(col, myd, myp);
+  double* PHI = W.f.phi;
+  double* D = W.f.dadq;
+  // phisalpha! (ahl21.jl:558-700, alpha = 2) for the Kepler pairs.
+  // First one lane per pair evaluates the pair's factors (the divisions and
+  // square roots) into pq; then, one row block (the three rows of body b) at
+  // a time, a lane keeps the block's rows of its dadq and jac_phi columns in
+  // registers and accumulates them from pq.  Body b's rows receive
+  // contributions only from the pairs that contain b, and in the reference's
+  // lexicographic pair order those come partner by partner in increasing
+  // order; looping over the partner c in increasing order therefore repeats
+  // every accumulation in the reference's order.  a and dadq start at zero
+  // (ahl21.jl:559-563); jac_phi keeps phic!'s terms.
+  constexpr int NP = Swork<NB>::NP, PQ = Swork<NB>::PQ;
+  double* pq = W.f.pq;
+  auto pidx = [](int i, int j) { return i * (2 * NB - i - 1) / 2 + (j - i - 1); };
+  const double coeff = (((2.0 * (h * h * h)) / 96.0) * 2.0) * NBG_GNEWT;
+  // loop 1 factors (ahl21.jl:569-575): fac2 = G/r^3, fac_k, f3_k = fac_k * 3/r^2
+  for (int p = col; p < NP; p += STEP) {
+    int i = 0, q = p;
+    while (q >= NB - 1 - i) { q -= NB - 1 - i; ++i; }
+    const int j = i + 1 + q;
+    double rij[3];
+    for (int k = 0; k < 3; ++k) rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
+    const double r2 = dot3(rij);
+    const double r3 = r2 * __builtin_sqrt(r2);
+    const double fac2 = NBG_GNEWT / r3;
+    double* o = pq + p * PQ;
+    o[0] = fac2;
+    for (int k = 0; k < 3; ++k) {
+      const double fac = fac2 * rij[k];
+      o[1 + k] = fac;
+      o[4 + k] = fac * (3.0 / r2);
+    }
+  }
+  wsync();
+  // loop 1: accelerations and dadq (ahl21.jl:567-602).  For the lane's
+  // column (body myd, coordinate myp; myp = 3 for the mass) the reference's
+  // conditional updates of pair (i, j) reduce, per role of myd in the pair,
+  // to at most two additions in a fixed order: t1 then t2, each -0.0 (which
+  // leaves any sum unchanged) when it does not apply.  "self" is the body b
+  // of the row block, "other" its partner c.
+  const bool mpos = myp >= 0 && myp < 3, mmass = myp == 3;
+  const int ry = mpos ? myp : 0;
+#pragma unroll 1
+  for (int b = 0; b < NB; ++b) {
+    double ab[3] = {0.0, 0.0, 0.0};
+    double db[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+    for (int c = 0; c < NB; ++c) {
+      if (c == b) continue;
+      const int i = c < b ? c : b, j = c < b ? b : c;
+      if (anyKick && S.pair[i + NB * j]) continue;
+      const double* o = pq + pidx(i, j) * PQ;
+      const double fac2 = o[0];
+      const double rmy = S.x[3 * i + ry] - S.x[3 * j + ry];
+      const bool bi = (b == i);
+      const double m = bi ? S.m[j] : S.m[i];  // the partner's mass
+      const bool self_pos = mpos && myd == b, other_pos = mpos && myd == c;
+      const bool other_mass = mmass && myd == c;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double fac = o[1 + k];
+        const double f3 = o[4 + k];
+        const double mf = m * fac;
+        ab[k] += bi ? -mf : mf;
+        const double x3 = (f3 * m) * rmy;
+        const double f2m = fac2 * m;
+        const bool pk = myp == k;
+        const double t1 = other_mass ? (bi ? -fac : fac) : (self_pos ? x3 : (other_pos ? -x3 : -0.0));
+        const double t2 = (pk && self_pos) ? -f2m : ((pk && other_pos) ? f2m : -0.0);
+        db[k] = (db[k] + t1) + t2;
+      }
+    }
+    if (own)
+      for (int k = 0; k < 3; ++k) D[(3 * b + k) * N4 + col] = db[k];
+    if (col == 0)
+      for (int k = 0; k < 3; ++k) S.a[3 * b + k] = ab[k];
+  }
+  wsync();
+  // loop 2 factors (ahl21.jl:609-640), needing the accelerations above
+  for (int p = col; p < NP; p += STEP) {
+    int i = 0, q = p;
+    while (q >= NB - 1 - i) { q -= NB - 1 - i; ++i; }
+    const int j = i + 1 + q;
+    double rij[3], aij[3];
+    for (int k = 0; k < 3; ++k) {
+      aij[k] = S.a[3 * i + k] - S.a[3 * j + k];
+      rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
+    }
+    const double r2 = dot3(rij);
+    const double r1 = __builtin_sqrt(r2);
+    const double ardot = dot3(aij, rij);
+    const double fac1 = coeff / ((r2 * r2) * r1);
+    const double mi = S.m[i], mj = S.m[j];
+    const double fac2 = ((2.0 * NBG_GNEWT) * (mi + mj)) / r1 + 3.0 * ardot;
+    double* o = pq + p * PQ;
+    for (int k = 0; k < 3; ++k) {
+      const double fac = fac1 * (rij[k] * fac2 - r2 * aij[k]);
+      o[k] = fac;                                                  // fac
+      o[3 + k] = fac * (5.0 / r2);                                 // f5
+      o[6 + k] = (((2.0 * NBG_GNEWT) * fac1) * rij[k]) / r1;       // fm
+      o[9 + k] = ((-2.0) * fac1) * ((((rij[k] * NBG_GNEWT) * (mi + mj)) / (r2 * r1)) + aij[k]);  // fr
+      // (fac1 * 3) * rij[k] multiplies aij[p] in fac3 and is ff = (3 * fac1) * rij[k]
+      // as well (IEEE multiplication commutes)
+      o[12 + k] = (fac1 * 3.0) * rij[k];
+    }
+    o[15] = fac1 * fac2;   // fd
+    o[16] = (-fac1) * r2;  // fa
+  }
+  wsync();
+  // loop 2: velocity corrections, dqdt_phi and jac_phi (ahl21.jl:607-698),
+  // with the same role reduction as loop 1: per k, at most three
+  // role-dependent additions (t1, t2, t3), then the dadq terms.
+  const double q3h = 3.0 / h;
+#pragma unroll 1
+  for (int b = 0; b < NB; ++b) {
+    double vb[3], veb[3], qb[3], pb[3];
+    for (int k = 0; k < 3; ++k) {
+      vb[k] = S.v[3 * b + k];
+      veb[k] = S.ve[3 * b + k];
+      qb[k] = needQ ? S.dqdt_phi[7 * b + 3 + k] : 0.0;
+      pb[k] = (anyKick && own) ? PHI[(3 * b + k) * N4 + col] : 0.0;
+    }
+#pragma unroll 1
+    for (int c = 0; c < NB; ++c) {
+      if (c == b) continue;
+      const int i = c < b ? c : b, j = c < b ? b : c;
+      if (anyKick && S.pair[i + NB * j]) continue;
+      const double* o = pq + pidx(i, j) * PQ;
+      double rij[3], Di[3], Dj[3];
+      for (int k = 0; k < 3; ++k) {
+        rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
+        Di[k] = own ? D[(3 * i + k) * N4 + col] : 0.0;
+        Dj[k] = own ? D[(3 * j + k) * N4 + col] : 0.0;
+      }
+      double dot = 0.0;
+      if (own)
+        for (int k = 0; k < 3; ++k) dot += rij[k] * (Di[k] - Dj[k]);
+      const double rmy = ry == 0 ? rij[0] : (ry == 1 ? rij[1] : rij[2]);
+      const double amy = S.a[3 * i + ry] - S.a[3 * j + ry];
+      const bool bi = (b == i);
+      const double m = bi ? S.m[j] : S.m[i];  // the partner's mass
+      const bool self_pos = mpos && myd == b, other_pos = mpos && myd == c;
+      const bool self_mass = mmass && myd == b, other_mass = mmass && myd == c;
+      const double fd = o[15], fa = o[16];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const double fac = o[k], f5 = o[3 + k], fm = o[6 + k], fr = o[9 + k], ff = o[12 + k];
+        const double mf = m * fac;
+        kahan(vb[k], veb[k], bi ? mf : -mf);
+        if (needQ) {
+          const double qm = (q3h * m) * fac;
+          qb[k] += bi ? qm : -qm;
+        }
+        const double diff = Di[k] - Dj[k];
+        const double x5 = (f5 * m) * rmy;
+        const double y = fm * m;
+        const double sy = bi ? y : -y, sfac = bi ? fac : -fac;
+        const double fdm = fd * m;
+        const double fac3 = fr * rmy + o[12 + k] * amy;
+        const double mf3 = m * fac3;
+        const bool pk = myp == k;
+        const double t1 = self_pos ? -x5 : (other_pos ? x5 : (self_mass ? sy : (other_mass ? sfac : -0.0)));
+        const double t2 = (pk && self_pos) ? fdm : ((pk && other_pos) ? -fdm : (other_mass ? sy : -0.0));
+        const double t3 = self_pos ? mf3 : (other_pos ? -mf3 : -0.0);
+        const double td = (fa * m) * diff, tf = (ff * m) * dot;
+        pb[k] = (((((pb[k] + t1) + t2) + t3) + (bi ? td : -td)) + (bi ? tf : -tf));
+      }
+    }
+    if (own)
+      for (int k = 0; k < 3; ++k) PHI[(3 * b + k) * N4 + col] = pb[k];
+    if (col == 0)
+      for (int k = 0; k < 3; ++k) {
+        S.v[3 * b + k] = vb[k];
+        S.ve[3 * b + k] = veb[k];
+        if (needQ) S.dqdt_phi[7 * b + 3 + k] = qb[k];
+      }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // phic! (ahl21.jl:392-552, kicked pairs) and phisalpha! (ahl21.jl:558-700,
 // Kepler pairs, alpha = 2) with Jacobian.  Phi and dadq columns are owned by
@@ -431,182 +623,7 @@ template <int NB, bool G> __device__ void phi_build(Smem<NB, G>& S, Swork<NB>& W
         wsync();
       }
   }
-  // phisalpha! (ahl21.jl:558-700, alpha = 2) for the Kepler pairs.
-  // First one lane per pair evaluates the pair's factors (the divisions and
-  // square roots) into pq; then, one row block (the three rows of body b) at
-  // a time, a lane keeps the block's rows of its dadq and jac_phi columns in
-  // registers and accumulates them from pq.  Body b's rows receive
-  // contributions only from the pairs that contain b, and in the reference's
-  // lexicographic pair order those come partner by partner in increasing
-  // order; looping over the partner c in increasing order therefore repeats
-  // every accumulation in the reference's order.  a and dadq start at zero
-  // (ahl21.jl:559-563); jac_phi keeps phic!'s terms.
-  constexpr int NP = Swork<NB>::NP, PQ = Swork<NB>::PQ;
-  double* pq = W.f.pq;
-  auto pidx = [](int i, int j) { return i * (2 * NB - i - 1) / 2 + (j - i - 1); };
-  const double coeff = (((2.0 * (h * h * h)) / 96.0) * 2.0) * NBG_GNEWT;
-  // loop 1 factors (ahl21.jl:569-575): fac2 = G/r^3, fac_k, f3_k = fac_k * 3/r^2
-  for (int p = lane; p < NP; p += WAVE) {
-    int i = 0, q = p;
-    while (q >= NB - 1 - i) { q -= NB - 1 - i; ++i; }
-    const int j = i + 1 + q;
-    double rij[3];
-    for (int k = 0; k < 3; ++k) rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
-    const double r2 = dot3(rij);
-    const double r3 = r2 * __builtin_sqrt(r2);
-    const double fac2 = NBG_GNEWT / r3;
-    double* o = pq + p * PQ;
-    o[0] = fac2;
-    for (int k = 0; k < 3; ++k) {
-      const double fac = fac2 * rij[k];
-      o[1 + k] = fac;
-      o[4 + k] = fac * (3.0 / r2);
-    }
-  }
-  wsync();
-  // loop 1: accelerations and dadq (ahl21.jl:567-602).  For the lane's
-  // column (body myd, coordinate myp; myp = 3 for the mass) the reference's
-  // conditional updates of pair (i, j) reduce, per role of myd in the pair,
-  // to at most two additions in a fixed order: t1 then t2, each -0.0 (which
-  // leaves any sum unchanged) when it does not apply.  "self" is the body b
-  // of the row block, "other" its partner c.
-  const bool mpos = myp >= 0 && myp < 3, mmass = myp == 3;
-  const int ry = mpos ? myp : 0;
-#pragma unroll 1
-  for (int b = 0; b < NB; ++b) {
-    double ab[3] = {0.0, 0.0, 0.0};
-    double db[3] = {0.0, 0.0, 0.0};
-#pragma unroll 1
-    for (int c = 0; c < NB; ++c) {
-      if (c == b) continue;
-      const int i = c < b ? c : b, j = c < b ? b : c;
-      if (anyKick && S.pair[i + NB * j]) continue;
-      const double* o = pq + pidx(i, j) * PQ;
-      const double fac2 = o[0];
-      const double rmy = S.x[3 * i + ry] - S.x[3 * j + ry];
-      const bool bi = (b == i);
-      const double m = bi ? S.m[j] : S.m[i];  // the partner's mass
-      const bool self_pos = mpos && myd == b, other_pos = mpos && myd == c;
-      const bool other_mass = mmass && myd == c;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const double fac = o[1 + k];
-        const double f3 = o[4 + k];
-        const double mf = m * fac;
-        ab[k] += bi ? -mf : mf;
-        const double x3 = (f3 * m) * rmy;
-        const double f2m = fac2 * m;
-        const bool pk = myp == k;
-        const double t1 = other_mass ? (bi ? -fac : fac) : (self_pos ? x3 : (other_pos ? -x3 : -0.0));
-        const double t2 = (pk && self_pos) ? -f2m : ((pk && other_pos) ? f2m : -0.0);
-        db[k] = (db[k] + t1) + t2;
-      }
-    }
-    if (own)
-      for (int k = 0; k < 3; ++k) D[(3 * b + k) * N4 + lane] = db[k];
-    if (lane == 0)
-      for (int k = 0; k < 3; ++k) S.a[3 * b + k] = ab[k];
-  }
-  wsync();
-  // loop 2 factors (ahl21.jl:609-640), needing the accelerations above
-  for (int p = lane; p < NP; p += WAVE) {
-    int i = 0, q = p;
-    while (q >= NB - 1 - i) { q -= NB - 1 - i; ++i; }
-    const int j = i + 1 + q;
-    double rij[3], aij[3];
-    for (int k = 0; k < 3; ++k) {
-      aij[k] = S.a[3 * i + k] - S.a[3 * j + k];
-      rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
-    }
-    const double r2 = dot3(rij);
-    const double r1 = __builtin_sqrt(r2);
-    const double ardot = dot3(aij, rij);
-    const double fac1 = coeff / ((r2 * r2) * r1);
-    const double mi = S.m[i], mj = S.m[j];
-    const double fac2 = ((2.0 * NBG_GNEWT) * (mi + mj)) / r1 + 3.0 * ardot;
-    double* o = pq + p * PQ;
-    for (int k = 0; k < 3; ++k) {
-      const double fac = fac1 * (rij[k] * fac2 - r2 * aij[k]);
-      o[k] = fac;                                                  // fac
-      o[3 + k] = fac * (5.0 / r2);                                 // f5
-      o[6 + k] = (((2.0 * NBG_GNEWT) * fac1) * rij[k]) / r1;       // fm
-      o[9 + k] = ((-2.0) * fac1) * ((((rij[k] * NBG_GNEWT) * (mi + mj)) / (r2 * r1)) + aij[k]);  // fr
-      // (fac1 * 3) * rij[k] multiplies aij[p] in fac3 and is ff = (3 * fac1) * rij[k]
-      // as well (IEEE multiplication commutes)
-      o[12 + k] = (fac1 * 3.0) * rij[k];
-    }
-    o[15] = fac1 * fac2;   // fd
-    o[16] = (-fac1) * r2;  // fa
-  }
-  wsync();
-  // loop 2: velocity corrections, dqdt_phi and jac_phi (ahl21.jl:607-698),
-  // with the same role reduction as loop 1: per k, at most three
-  // role-dependent additions (t1, t2, t3), then the dadq terms.
-  const double q3h = 3.0 / h;
-#pragma unroll 1
-  for (int b = 0; b < NB; ++b) {
-    double vb[3], veb[3], qb[3], pb[3];
-    for (int k = 0; k < 3; ++k) {
-      vb[k] = S.v[3 * b + k];
-      veb[k] = S.ve[3 * b + k];
-      qb[k] = needQ ? S.dqdt_phi[7 * b + 3 + k] : 0.0;
-      pb[k] = (anyKick && own) ? PHI[(3 * b + k) * N4 + lane] : 0.0;
-    }
-#pragma unroll 1
-    for (int c = 0; c < NB; ++c) {
-      if (c == b) continue;
-      const int i = c < b ? c : b, j = c < b ? b : c;
-      if (anyKick && S.pair[i + NB * j]) continue;
-      const double* o = pq + pidx(i, j) * PQ;
-      double rij[3], Di[3], Dj[3];
-      for (int k = 0; k < 3; ++k) {
-        rij[k] = S.x[3 * i + k] - S.x[3 * j + k];
-        Di[k] = own ? D[(3 * i + k) * N4 + lane] : 0.0;
-        Dj[k] = own ? D[(3 * j + k) * N4 + lane] : 0.0;
-      }
-      double dot = 0.0;
-      if (own)
-        for (int k = 0; k < 3; ++k) dot += rij[k] * (Di[k] - Dj[k]);
-      const double rmy = ry == 0 ? rij[0] : (ry == 1 ? rij[1] : rij[2]);
-      const double amy = S.a[3 * i + ry] - S.a[3 * j + ry];
-      const bool bi = (b == i);
-      const double m = bi ? S.m[j] : S.m[i];  // the partner's mass
-      const bool self_pos = mpos && myd == b, other_pos = mpos && myd == c;
-      const bool self_mass = mmass && myd == b, other_mass = mmass && myd == c;
-      const double fd = o[15], fa = o[16];
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        const double fac = o[k], f5 = o[3 + k], fm = o[6 + k], fr = o[9 + k], ff = o[12 + k];
-        const double mf = m * fac;
-        kahan(vb[k], veb[k], bi ? mf : -mf);
-        if (needQ) {
-          const double qm = (q3h * m) * fac;
-          qb[k] += bi ? qm : -qm;
-        }
-        const double diff = Di[k] - Dj[k];
-        const double x5 = (f5 * m) * rmy;
-        const double y = fm * m;
-        const double sy = bi ? y : -y, sfac = bi ? fac : -fac;
-        const double fdm = fd * m;
-        const double fac3 = fr * rmy + o[12 + k] * amy;
-        const double mf3 = m * fac3;
-        const bool pk = myp == k;
-        const double t1 = self_pos ? -x5 : (other_pos ? x5 : (self_mass ? sy : (other_mass ? sfac : -0.0)));
-        const double t2 = (pk && self_pos) ? fdm : ((pk && other_pos) ? -fdm : (other_mass ? sy : -0.0));
-        const double t3 = self_pos ? mf3 : (other_pos ? -mf3 : -0.0);
-        const double td = (fa * m) * diff, tf = (ff * m) * dot;
-        pb[k] = (((((pb[k] + t1) + t2) + t3) + (bi ? td : -td)) + (bi ? tf : -tf));
-      }
-    }
-    if (own)
-      for (int k = 0; k < 3; ++k) PHI[(3 * b + k) * N4 + lane] = pb[k];
-    if (lane == 0)
-      for (int k = 0; k < 3; ++k) {
-        S.v[3 * b + k] = vb[k];
-        S.ve[3 * b + k] = veb[k];
-        if (needQ) S.dqdt_phi[7 * b + 3 + k] = qb[k];
-      }
-  }
+  phisalpha_cols<NB, G, WAVE>(S, W, h, anyKick, needQ, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -1029,11 +1046,23 @@ __device__ __attribute__((always_inline)) inline void step_pre(Smem<NB, G>& S, S
 }
 
 template <int NB, bool G>
+__device__ __attribute__((always_inline)) inline void step_mid_rest(Smem<NB, G>& S, Swork<NB>& W, bool needJ, bool needQ,
+                                                                    int lane);
+
+template <int NB, bool G>
 __device__ __attribute__((always_inline)) inline void step_mid(Smem<NB, G>& S, Swork<NB>& W, double h, bool needJ, bool needQ, bool anyKick, int lane) {
-  constexpr int N3 = 3 * NB, N4 = 4 * NB, N7 = 7 * NB;
   // ---- phic!, phisalpha!
   phi_build<NB>(S, W, h, anyKick, needQ, lane);
   wsync();
+  step_mid_rest<NB>(S, W, needJ, needQ, lane);
+}
+
+// The rest of the stage between the sweeps, once jac_phi is built:
+// J_vel += jac_phi * J_pos, and dq/dt.
+template <int NB, bool G>
+__device__ __attribute__((always_inline)) inline void step_mid_rest(Smem<NB, G>& S, Swork<NB>& W, bool needJ, bool needQ,
+                                                                    int lane) {
+  constexpr int N3 = 3 * NB, N4 = 4 * NB, N7 = 7 * NB;
   if (needJ)
     for (int c = lane; c < N7; c += WAVE) {
       double P[N3];
@@ -1141,19 +1170,47 @@ __device__ void ahl21_step(Smem<NB, G>& S, Swork<NB>& W, double h, bool needJ, b
 
 // One step of each of the two systems of a Duo, in lockstep: the stages
 // outside the sweeps run for one system, then the other (they share the
-// work area), the sweeps for both at once (sweep2).  The stage loops are
-// kept rolled so that their code exists once.
+// work area), the sweeps for both at once (sweep2).  Each stage is written
+// out for each record (fixed LDS addresses): a loop over the two records
+// with a selected base took 1.5 % longer, its register pressure higher.
+// phisalpha! of both systems of a Duo at once, where the 4n columns and the
+// pairs of one system fit half a wave: lanes [0, 32) serve the first
+// system's columns and pair factors, [32, 64) the second's, each lane with
+// exactly the operations phi_build gives it; each system builds into its own
+// work area.  Systems without kicked pairs only.
 template <int NB, bool G>
+__device__ __attribute__((always_inline)) inline void phi_build2(Smem<NB, G>& S0, Smem<NB, G>& S1, Swork<NB>& W0,
+                                                                 Swork<NB>& W1, double h0, double h1, bool nQ0, bool nQ1,
+                                                                 int lane) {
+  static_assert(4 * NB <= 32 && Swork<NB>::NP <= 32, "a system's columns and pairs must fit half a wave");
+  const bool sys = lane >= 32;
+  phisalpha_cols<NB, G, 32>(sys ? S1 : S0, sys ? W1 : W0, sys ? h1 : h0, false, sys ? nQ1 : nQ0, lane & 31);
+}
+
+template <int NB, bool G, bool PHI2>
 __device__ void ahl21_step2(Smem<NB, G>& S0, Smem<NB, G>& S1, Swork<NB>& W, int ch, double h0, double h1,
                             bool nJ0, bool nQ0, bool nJ1, bool nQ1, bool k0, bool k1, int lane) {
-#pragma unroll 1
-  for (int s = 0; s < 2; ++s) step_pre<NB>(s ? S1 : S0, W, s ? h1 : h0, s ? nJ1 : nJ0, s ? nQ1 : nQ0, s ? k1 : k0, lane);
+  step_pre<NB>(S0, W, h0, nJ0, nQ0, k0, lane);
+  step_pre<NB>(S1, W, h1, nJ1, nQ1, k1, lane);
   sweep2<NB>(S0, S1, W.slot, ch, 0, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
-#pragma unroll 1
-  for (int s = 0; s < 2; ++s) step_mid<NB>(s ? S1 : S0, W, s ? h1 : h0, s ? nJ1 : nJ0, s ? nQ1 : nQ0, s ? k1 : k0, lane);
+  bool shared = false;
+  if constexpr (PHI2) {
+    if (!k0 && !k1) {  // the second work area follows the first
+      Swork<NB>& W1 = *reinterpret_cast<Swork<NB>*>(reinterpret_cast<unsigned char*>(&W) + sizeof(Swork<NB>));
+      phi_build2<NB>(S0, S1, W, W1, h0, h1, nQ0, nQ1, lane);
+      wsync();
+      step_mid_rest<NB>(S0, W, nJ0, nQ0, lane);
+      step_mid_rest<NB>(S1, W1, nJ1, nQ1, lane);
+      shared = true;
+    }
+  }
+  if (!shared) {
+    step_mid<NB>(S0, W, h0, nJ0, nQ0, k0, lane);
+    step_mid<NB>(S1, W, h1, nJ1, nQ1, k1, lane);
+  }
   sweep2<NB>(S0, S1, W.slot, ch, 1, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
-#pragma unroll 1
-  for (int s = 0; s < 2; ++s) step_post<NB>(s ? S1 : S0, W, s ? h1 : h0, s ? nJ1 : nJ0, s ? nQ1 : nQ0, s ? k1 : k0, lane);
+  step_post<NB>(S0, W, h0, nJ0, nQ0, k0, lane);
+  step_post<NB>(S1, W, h1, nJ1, nQ1, k1, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -1680,6 +1737,11 @@ template <int N, int HI> struct Duo {
 template <int N, int HI> constexpr bool duo_fits() { return sizeof(Duo<N, HI>) <= lds_bytes_upto<HI>(); }
 // slots per system in a Duo's shared work area
 template <int N, int HI> constexpr int duo_ch() { return (int)(Duo<N, HI>::WBYTES / sizeof(PairSlot)) / 2; }
+// both systems' force-gradient stages at once (phi_build2): the columns and
+// pairs of one system fit half a wave and two work areas fit the Duo's
+template <int N, int HI> constexpr bool duo_phi2() {
+  return 4 * N <= 32 && Swork<N>::NP <= 32 && 2 * sizeof(Swork<N>) <= Duo<N, HI>::WBYTES;
+}
 
 // The two systems' sweeps follow the same level schedule (and neither has a
 // massless pair, which keeps the sequential order).  Wave-uniform.
@@ -1753,8 +1815,8 @@ __device__ void run_pair(Duo<NB, HI>& P, const SysDesc& D0, const SysDesc& D1, c
     double* ws1 = B.ws + D1.off_ws;
 #pragma unroll 1
     for (; is0 < e0 && is1 < e1; ++is0, ++is1) {
-      ahl21_step2<NB>(S0, S1, W, duo_ch<NB, HI>(), D0.h, D1.h, g0, g0 && is0 + 1 == D0.nsteps, g1,
-                      g1 && is1 + 1 == D1.nsteps, k0, k1, lane);
+      ahl21_step2<NB, true, duo_phi2<NB, HI>()>(S0, S1, W, duo_ch<NB, HI>(), D0.h, D1.h, g0, g0 && is0 + 1 == D0.nsteps,
+                                                 g1, g1 && is1 + 1 == D1.nsteps, k0, k1, lane);
       if (lane == 0) {
         S0.t = D0.t0 + ((double)(is0 + 1) * D0.h);
         S1.t = D1.t0 + ((double)(is1 + 1) * D1.h);
