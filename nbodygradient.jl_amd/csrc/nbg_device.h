@@ -832,9 +832,9 @@ __device__ __attribute__((always_inline)) inline void pair_level_lane(Smem<NB, G
 }
 
 // Phase C of one chunk of `cnt` pairs (slots[0..cnt)) of system S.
-template <int NB, bool G>
+template <int NB, bool G, int STEP = WAVE>
 __device__ __attribute__((always_inline)) inline void sweep_rows(Smem<NB, G>& S, const PairSlot* slots, int cnt,
-                                                                 bool needJ, bool needQ, int lane) {
+                                                                 bool needJ, bool needQ, int col) {
   constexpr int N7 = 7 * NB;
   int lastq = -1;  // last slot of this chunk whose pair has mass
 #pragma unroll 1
@@ -843,9 +843,9 @@ __device__ __attribute__((always_inline)) inline void sweep_rows(Smem<NB, G>& S,
     const int i = sl.i, j = sl.j;
     if (sl.valid) {
       lastq = q;
-      if (needQ && lane < 6) {  // dqdt_ij (ahl21.jl:743-748)
-        S.dqij[lane] = sl.mj * sl.kep[lane + 42];
-        S.dqij[7 + lane] = (-sl.mi) * sl.kep[lane + 42];
+      if (needQ && col < 6) {  // dqdt_ij (ahl21.jl:743-748)
+        S.dqij[col] = sl.mj * sl.kep[col + 42];
+        S.dqij[7 + col] = (-sl.mi) * sl.kep[col + 42];
       }
     }
     // (gm == 0: the reference returns before touching jac_ij / dqdt_ij and
@@ -853,25 +853,25 @@ __device__ __attribute__((always_inline)) inline void sweep_rows(Smem<NB, G>& S,
     // jlast when that lies in an earlier chunk; dqij keeps its values.)
     const PairSlot& src = lastq >= 0 ? slots[lastq] : S.jlast;
     if (needJ)
-      for (int c = lane; c < N7; c += WAVE) pair_cols<NB>(S, src, i, j, c);
+      for (int c = col; c < N7; c += STEP) pair_cols<NB>(S, src, i, j, c);
     if (needQ) {
       double q1[14];
       for (int k = 0; k < 7; ++k) {
         q1[k] = S.dqdt[7 * i + k];
         q1[7 + k] = S.dqdt[7 * j + k];
       }
-      if (lane < 14) {
+      if (col < 14) {
         double s2 = 0.0;
-        if (lane != 6 && lane != 13) {
-          const bool rowi = lane < 7;
-          const int rk = rowi ? lane : lane - 7;
+        if (col != 6 && col != 13) {
+          const bool rowi = col < 7;
+          const int rk = rowi ? col : col - 7;
           for (int k = 0; k < 14; ++k) s2 = fmad(jij_coef(src, rowi, rk, k), q1[k], s2);
         } else {
           for (int k = 0; k < 14; ++k) s2 = fmad(0.0, q1[k], s2);
         }
-        const double nd = (S.dqij[lane] * 0.5) + (q1[lane] + s2);
-        S.dqij[lane] = nd;
-        S.dqdt[lane < 7 ? 7 * i + lane : 7 * j + lane - 7] = nd;
+        const double nd = (S.dqij[col] * 0.5) + (q1[col] + s2);
+        S.dqij[col] = nd;
+        S.dqdt[col < 7 ? 7 * i + col : 7 * j + col - 7] = nd;
       }
     }
     wsync();
@@ -880,7 +880,7 @@ __device__ __attribute__((always_inline)) inline void sweep_rows(Smem<NB, G>& S,
   if (S.seq && lastq >= 0) {
     const double* from = reinterpret_cast<const double*>(&slots[lastq]);
     double* to = reinterpret_cast<double*>(&S.jlast);
-    for (int w = lane; w < (int)(sizeof(PairSlot) / sizeof(double)); w += WAVE) to[w] = from[w];
+    for (int w = col; w < (int)(sizeof(PairSlot) / sizeof(double)); w += STEP) to[w] = from[w];
     wsync();
   }
 }
@@ -929,7 +929,7 @@ __device__ __attribute__((always_inline)) inline void sweep(Smem<NB, G>& S, Swor
 // Each system's slots are its own (slots[0, ch) and slots[ch, 2 ch)), and
 // every operation of a system is the one-system sweep's, so each system's
 // results are exactly its one-system results.
-template <int NB, bool G>
+template <int NB, bool G, bool HALF = false>
 __device__ __attribute__((always_inline)) inline void sweep2(Smem<NB, G>& S0, Smem<NB, G>& S1, PairSlot* slots, int ch,
                                                              int dir, double hh0, double hh1, bool nJ0, bool nQ0,
                                                              bool nJ1, bool nQ1, int lane) {
@@ -965,8 +965,13 @@ __device__ __attribute__((always_inline)) inline void sweep2(Smem<NB, G>& S0, Sm
       if ((sys ? pj1 : pj0) && sl->valid) pair_jacobian(sl, drift_first);
     }
     wsync();
-    if (pj0) sweep_rows<NB>(S0, slots, cnt, nJ0, nQ0, lane);
-    if (pj1) sweep_rows<NB>(S1, slots + ch, cnt, nJ1, nQ1, lane);
+    if constexpr (HALF) {  // both systems' rows at once, one per half of the wave
+      const bool sys = lane >= 32;
+      sweep_rows<NB, G, 32>(sys ? S1 : S0, slots + (sys ? ch : 0), cnt, sys ? nJ1 : nJ0, sys ? nQ1 : nQ0, lane & 31);
+    } else {
+      if (pj0) sweep_rows<NB>(S0, slots, cnt, nJ0, nQ0, lane);
+      if (pj1) sweep_rows<NB>(S1, slots + ch, cnt, nJ1, nQ1, lane);
+    }
   }
 }
 
@@ -979,29 +984,31 @@ __device__ __attribute__((always_inline)) inline void sweep2(Smem<NB, G>& S0, Sm
 // The stages of one step between the pair sweeps: before the forward sweep
 // (kickfast! and the first drift), between the sweeps (phic!/phisalpha!),
 // after the backward sweep (the second drift and kickfast!).
-template <int NB, bool G>
-__device__ __attribute__((always_inline)) inline void step_pre(Smem<NB, G>& S, Swork<NB>& W, double h, bool needJ, bool needQ, bool anyKick, int lane) {
+template <int NB, bool G, int STEP = WAVE>
+__device__ __attribute__((always_inline)) inline void step_pre(Smem<NB, G>& S, Swork<NB>& W, double h, bool needJ, bool needQ, bool anyKick, int col) {
   constexpr int N3 = 3 * NB, N4 = 4 * NB, N7 = 7 * NB;
   const double h2 = 0.5 * h;
   const double h6 = h / 6.0;
   // zero_out!(d); fill!(s.dqdt, 0)
-  if (lane < 14) S.dqij[lane] = 0.0;
+  if (col < 14) S.dqij[col] = 0.0;
   if (S.seq)  // jac_ij = 0 at the start of the step (zero_out!, ahl21.jl:8)
-    for (int w = lane; w < (int)(sizeof(PairSlot) / sizeof(double)); w += WAVE)
+    for (int w = col; w < (int)(sizeof(PairSlot) / sizeof(double)); w += STEP)
       reinterpret_cast<double*>(&S.jlast)[w] = 0.0;
   if (needQ)
-    for (int r = lane; r < N7; r += WAVE) {
+    for (int r = col; r < N7; r += STEP) {
       S.dqdt[r] = 0.0;
       S.dqdt_phi[r] = 0.0;
       S.dqdt_kick[r] = 0.0;
     }
   wsync();
   // ---- kickfast!(s, d, h/6)
-  if (anyKick) kick_build<NB>(S, W, h6, needQ, lane);
+  if constexpr (STEP == WAVE) {  // kicked pairs: whole-wave form only
+    if (anyKick) kick_build<NB>(S, W, h6, needQ, col);
+  }
   wsync();
   if (needQ) {
     // dqdt_kick ./= 6; tmp7n = K*dqdt (dqdt = 0 here); dqdt .+= dqdt_kick .+ tmp7n
-    for (int r = lane; r < N7; r += WAVE) {
+    for (int r = col; r < N7; r += STEP) {
       double tmp = 0.0;
       if (anyKick) {
         const int b = r / 7, k = r - 7 * b;
@@ -1021,7 +1028,7 @@ __device__ __attribute__((always_inline)) inline void step_pre(Smem<NB, G>& S, S
   }
   // ---- Jacobian: jac_copy = K*J (pre-drift), drift_grad!, J += jac_copy
   if (needJ)
-    for (int c = lane; c < N7; c += WAVE) {
+    for (int c = col; c < N7; c += STEP) {
       double P[N3];
       load_pos<NB>(S, c, P);
       drift_cols<NB>(S, c, h2);
@@ -1029,7 +1036,7 @@ __device__ __attribute__((always_inline)) inline void step_pre(Smem<NB, G>& S, S
       if (anyKick) vel_product_add<NB>(S, W.f.dadq, c, P);
     }
   // ---- drift of the state (drift_grad! / drift!): entries are independent
-  for (int q = lane; q < N3; q += WAVE) {
+  for (int q = col; q < N3; q += STEP) {
     double xq = S.x[q], eq = S.xe[q];
     kahan(xq, eq, h2 * S.v[q]);
     S.x[q] = xq;
@@ -1037,7 +1044,7 @@ __device__ __attribute__((always_inline)) inline void step_pre(Smem<NB, G>& S, S
   }
   wsync();
   if (needQ) {
-    for (int q = lane; q < N3; q += WAVE) {
+    for (int q = col; q < N3; q += STEP) {
       const int b = q / 3, k = q - 3 * b;
       S.dqdt[7 * b + k] = 0.5 * S.v[q] + h2 * S.dqdt[7 * b + 3 + k];
     }
@@ -1045,9 +1052,9 @@ __device__ __attribute__((always_inline)) inline void step_pre(Smem<NB, G>& S, S
   wsync();
 }
 
-template <int NB, bool G>
+template <int NB, bool G, int STEP = WAVE>
 __device__ __attribute__((always_inline)) inline void step_mid_rest(Smem<NB, G>& S, Swork<NB>& W, bool needJ, bool needQ,
-                                                                    int lane);
+                                                                    int col);
 
 template <int NB, bool G>
 __device__ __attribute__((always_inline)) inline void step_mid(Smem<NB, G>& S, Swork<NB>& W, double h, bool needJ, bool needQ, bool anyKick, int lane) {
@@ -1059,12 +1066,12 @@ __device__ __attribute__((always_inline)) inline void step_mid(Smem<NB, G>& S, S
 
 // The rest of the stage between the sweeps, once jac_phi is built:
 // J_vel += jac_phi * J_pos, and dq/dt.
-template <int NB, bool G>
+template <int NB, bool G, int STEP>
 __device__ __attribute__((always_inline)) inline void step_mid_rest(Smem<NB, G>& S, Swork<NB>& W, bool needJ, bool needQ,
-                                                                    int lane) {
+                                                                    int col) {
   constexpr int N3 = 3 * NB, N4 = 4 * NB, N7 = 7 * NB;
   if (needJ)
-    for (int c = lane; c < N7; c += WAVE) {
+    for (int c = col; c < N7; c += STEP) {
       double P[N3];
       load_pos<NB>(S, c, P);
       vel_product_add<NB>(S, W.f.phi, c, P);
@@ -1074,7 +1081,7 @@ __device__ __attribute__((always_inline)) inline void step_mid_rest(Smem<NB, G>&
     // tmp7n = jac_phi*dqdt; dqdt .+= dqdt_phi .+ tmp7n
     double nv[2];
     int nr = 0;
-    for (int r = lane; r < N7; r += WAVE, ++nr) {
+    for (int r = col; r < N7; r += STEP, ++nr) {
       double tmp = 0.0;
       const int b = r / 7, k = r - 7 * b;
       if (k >= 3 && k < 6) {
@@ -1090,20 +1097,20 @@ __device__ __attribute__((always_inline)) inline void step_mid_rest(Smem<NB, G>&
     }
     wsync();
     nr = 0;
-    for (int r = lane; r < N7; r += WAVE, ++nr) S.dqdt[r] = nv[nr];
+    for (int r = col; r < N7; r += STEP, ++nr) S.dqdt[r] = nv[nr];
   }
   wsync();
 }
 
-template <int NB, bool G>
-__device__ __attribute__((always_inline)) inline void step_post(Smem<NB, G>& S, Swork<NB>& W, double h, bool needJ, bool needQ, bool anyKick, int lane) {
+template <int NB, bool G, int STEP = WAVE>
+__device__ __attribute__((always_inline)) inline void step_post(Smem<NB, G>& S, Swork<NB>& W, double h, bool needJ, bool needQ, bool anyKick, int col) {
   constexpr int N3 = 3 * NB, N4 = 4 * NB, N7 = 7 * NB;
   const double h2 = 0.5 * h;
   const double h6 = h / 6.0;
   // ---- drift
   if (needJ)
-    for (int c = lane; c < N7; c += WAVE) drift_cols<NB>(S, c, h2);
-  for (int q = lane; q < N3; q += WAVE) {
+    for (int c = col; c < N7; c += STEP) drift_cols<NB>(S, c, h2);
+  for (int q = col; q < N3; q += STEP) {
     double xq = S.x[q], eq = S.xe[q];
     kahan(xq, eq, h2 * S.v[q]);
     S.x[q] = xq;
@@ -1111,20 +1118,22 @@ __device__ __attribute__((always_inline)) inline void step_post(Smem<NB, G>& S, 
   }
   wsync();
   if (needQ) {
-    for (int q = lane; q < N3; q += WAVE) {
+    for (int q = col; q < N3; q += STEP) {
       const int b = q / 3, k = q - 3 * b;
       S.dqdt[7 * b + k] += 0.5 * S.v[q] + h2 * S.dqdt[7 * b + 3 + k];
     }
-    for (int r = lane; r < N7; r += WAVE) S.dqdt_kick[r] = 0.0;
+    for (int r = col; r < N7; r += STEP) S.dqdt_kick[r] = 0.0;
   }
   wsync();
   // ---- kickfast!(s, d, h/6)
-  if (anyKick) kick_build<NB>(S, W, h6, needQ, lane);
+  if constexpr (STEP == WAVE) {  // kicked pairs: whole-wave form only
+    if (anyKick) kick_build<NB>(S, W, h6, needQ, col);
+  }
   wsync();
   if (needQ) {
     double nv[2];
     int nr = 0;
-    for (int r = lane; r < N7; r += WAVE, ++nr) {
+    for (int r = col; r < N7; r += STEP, ++nr) {
       double tmp = 0.0;
       if (anyKick) {
         const int b = r / 7, k = r - 7 * b;
@@ -1143,10 +1152,10 @@ __device__ __attribute__((always_inline)) inline void step_post(Smem<NB, G>& S, 
     }
     wsync();
     nr = 0;
-    for (int r = lane; r < N7; r += WAVE, ++nr) S.dqdt[r] = nv[nr];
+    for (int r = col; r < N7; r += STEP, ++nr) S.dqdt[r] = nv[nr];
   }
   if (needJ)
-    for (int c = lane; c < N7; c += WAVE) {
+    for (int c = col; c < N7; c += STEP) {
       if (anyKick) {
         double P[N3];
         load_pos<NB>(S, c, P);
@@ -1211,6 +1220,30 @@ __device__ void ahl21_step2(Smem<NB, G>& S0, Smem<NB, G>& S1, Swork<NB>& W, int 
   sweep2<NB>(S0, S1, W.slot, ch, 1, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
   step_post<NB>(S0, W, h0, nJ0, nQ0, k0, lane);
   step_post<NB>(S1, W, h1, nJ1, nQ1, k1, lane);
+}
+
+// One step of both systems of a Duo entirely at once, for body counts whose
+// 7n columns fit half a wave (records with their Jacobians in LDS): lanes
+// [0, 32) do the first system's work of every stage, [32, 64) the second's,
+// each lane with the operations the one-system step gives it; the sweeps'
+// Kepler levels and pair Jacobians are shared as in sweep2.  No kicked pairs.
+template <int NB>
+__device__ void ahl21_step2h(Smem<NB, false>& S0, Smem<NB, false>& S1, Swork<NB>& W0, Swork<NB>& W1, PairSlot* slots,
+                             int ch, double h0, double h1, bool nJ0, bool nQ0, bool nJ1, bool nQ1, int lane) {
+  static_assert(7 * NB <= 32, "a system's columns must fit half a wave");
+  const bool sys = lane >= 32;
+  const int col = lane & 31;
+  Smem<NB, false>& S = sys ? S1 : S0;
+  Swork<NB>& W = sys ? W1 : W0;
+  const double h = sys ? h1 : h0;
+  const bool nJ = sys ? nJ1 : nJ0, nQ = sys ? nQ1 : nQ0;
+  step_pre<NB, false, 32>(S, W, h, nJ, nQ, false, col);
+  sweep2<NB, false, true>(S0, S1, slots, ch, 0, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
+  phisalpha_cols<NB, false, 32>(S, W, h, false, nQ, col);
+  wsync();
+  step_mid_rest<NB, false, 32>(S, W, nJ, nQ, col);
+  sweep2<NB, false, true>(S0, S1, slots, ch, 1, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
+  step_post<NB, false, 32>(S, W, h, nJ, nQ, false, col);
 }
 
 // ---------------------------------------------------------------------------
@@ -1726,21 +1759,32 @@ template <int N> constexpr size_t lds_bytes_upto() {
 // layouts (duo_fits), so the launch keeps its occupancy; other systems, and
 // pairs that cannot share a schedule, run one after the other as in nbg_run.
 // ---------------------------------------------------------------------------
-template <int N, int HI> struct Duo {
-  static constexpr size_t REC = sizeof(Smem<N, true>);
+template <int N, int HI, bool G = true> struct Duo {
+  static constexpr size_t REC = sizeof(Smem<N, G>);
   static constexpr size_t FREE = lds_bytes_upto<HI>() > 2 * REC ? lds_bytes_upto<HI>() - 2 * REC : 0;
   // the work area takes what the launch's LDS leaves (at least one Swork<N>)
   static constexpr size_t WBYTES = (FREE > sizeof(Swork<N>) ? FREE : sizeof(Swork<N>)) / 8 * 8;
-  Smem<N, true> s0, s1;  // Jacobians in HBM, whatever N
+  Smem<N, G> s0, s1;  // G: Jacobians in HBM
   alignas(8) unsigned char w[WBYTES];
 };
-template <int N, int HI> constexpr bool duo_fits() { return sizeof(Duo<N, HI>) <= lds_bytes_upto<HI>(); }
+template <int N, int HI, bool G = true> constexpr bool duo_fits() {
+  return sizeof(Duo<N, HI, G>) <= lds_bytes_upto<HI>();
+}
 // slots per system in a Duo's shared work area
-template <int N, int HI> constexpr int duo_ch() { return (int)(Duo<N, HI>::WBYTES / sizeof(PairSlot)) / 2; }
+template <int N, int HI, bool G = true> constexpr int duo_ch() {
+  return (int)(Duo<N, HI, G>::WBYTES / sizeof(PairSlot)) / 2;
+}
 // both systems' force-gradient stages at once (phi_build2): the columns and
 // pairs of one system fit half a wave and two work areas fit the Duo's
-template <int N, int HI> constexpr bool duo_phi2() {
-  return 4 * N <= 32 && Swork<N>::NP <= 32 && 2 * sizeof(Swork<N>) <= Duo<N, HI>::WBYTES;
+template <int N, int HI, bool G = true> constexpr bool duo_phi2() {
+  return 4 * N <= 32 && Swork<N>::NP <= 32 && 2 * sizeof(Swork<N>) <= Duo<N, HI, G>::WBYTES;
+}
+// the whole step of both systems at once (ahl21_step2h): every column loop
+// of one system (7n columns) fits half a wave, and two records with their
+// Jacobians in LDS fit beside two work areas
+template <int N, int HI> constexpr bool duo_half() {
+  return 7 * N <= 32 && duo_fits<N, HI, false>() && duo_phi2<N, HI, false>() && duo_ch<N, HI, false>() >= N / 2 &&
+         duo_ch<N, HI, false>() >= 1;
 }
 
 // The two systems' sweeps follow the same level schedule (and neither has a
@@ -1782,12 +1826,12 @@ __device__ void transit_steps(Smem<NB, G>& S, Swork<NB>& W, const SysDesc& D, co
 // Every system's results are exactly its run_system results: the shared
 // stages run the same operations on its own record, the transit searches run
 // one system at a time, each on the whole work area.
-template <int NB, int HI>
-__device__ void run_pair(Duo<NB, HI>& P, const SysDesc& D0, const SysDesc& D1, const SysBufs& B, int64_t chunk0,
+template <int NB, int HI, bool G>
+__device__ void run_pair(Duo<NB, HI, G>& P, const SysDesc& D0, const SysDesc& D1, const SysBufs& B, int64_t chunk0,
                          int64_t chunk1, int lane) {
-  static_assert(duo_ch<NB, HI>() >= NB / 2, "a level of pairs must fit a system's slots");
-  Smem<NB, true>& S0 = P.s0;
-  Smem<NB, true>& S1 = P.s1;
+  static_assert(duo_ch<NB, HI, G>() >= NB / 2, "a level of pairs must fit a system's slots");
+  Smem<NB, G>& S0 = P.s0;
+  Smem<NB, G>& S1 = P.s1;
   Swork<NB>& W = *reinterpret_cast<Swork<NB>*>(P.w);
   load_system<NB>(S0, D0, B, lane);
   load_system<NB>(S1, D1, B, lane);
@@ -1815,8 +1859,18 @@ __device__ void run_pair(Duo<NB, HI>& P, const SysDesc& D0, const SysDesc& D1, c
     double* ws1 = B.ws + D1.off_ws;
 #pragma unroll 1
     for (; is0 < e0 && is1 < e1; ++is0, ++is1) {
-      ahl21_step2<NB, true, duo_phi2<NB, HI>()>(S0, S1, W, duo_ch<NB, HI>(), D0.h, D1.h, g0, g0 && is0 + 1 == D0.nsteps,
-                                                 g1, g1 && is1 + 1 == D1.nsteps, k0, k1, lane);
+      const bool q0 = g0 && is0 + 1 == D0.nsteps, q1 = g1 && is1 + 1 == D1.nsteps;
+      bool done = false;
+      if constexpr (!G) {
+        if (!k0 && !k1) {
+          Swork<NB>& W1 = *reinterpret_cast<Swork<NB>*>(P.w + sizeof(Swork<NB>));
+          ahl21_step2h<NB>(S0, S1, W, W1, W.slot, duo_ch<NB, HI, G>(), D0.h, D1.h, g0, q0, g1, q1, lane);
+          done = true;
+        }
+      }
+      if (!done)
+        ahl21_step2<NB, G, duo_phi2<NB, HI, G>()>(S0, S1, W, duo_ch<NB, HI, G>(), D0.h, D1.h, g0, q0, g1, q1, k0, k1,
+                                                   lane);
       if (lane == 0) {
         S0.t = D0.t0 + ((double)(is0 + 1) * D0.h);
         S1.t = D1.t0 + ((double)(is1 + 1) * D1.h);
@@ -1847,8 +1901,11 @@ __device__ __forceinline__ bool dispatch_pair(unsigned char* raw, const SysDesc&
                                               const SysBufs& B, int64_t chunk0, int64_t chunk1, int lane) {
   if constexpr (N <= HI) {
     if (D0.n == N) {
-      if constexpr (duo_fits<N, HI>()) {
-        run_pair<N, HI>(*reinterpret_cast<Duo<N, HI>*>(raw), D0, D1, B, chunk0, chunk1, lane);
+      if constexpr (duo_half<N, HI>()) {  // Jacobians in LDS, every stage on half-waves
+        run_pair<N, HI, false>(*reinterpret_cast<Duo<N, HI, false>*>(raw), D0, D1, B, chunk0, chunk1, lane);
+        return true;
+      } else if constexpr (duo_fits<N, HI>()) {
+        run_pair<N, HI, true>(*reinterpret_cast<Duo<N, HI, true>*>(raw), D0, D1, B, chunk0, chunk1, lane);
         return true;
       } else {
         return false;
