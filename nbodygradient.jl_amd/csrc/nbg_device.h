@@ -1820,12 +1820,210 @@ __device__ void transit_steps(Smem<NB, G>& S, Swork<NB>& W, const SysDesc& D, co
   }
 }
 
+// A system's place in its run for the Duo's step-by-step loop: the next
+// step is a main step (ACT_MAIN), a Newton iteration of a transit search
+// (ACT_NEWTON) or the search's final gradient step (ACT_JSTEP); ACT_DONE once
+// the visit's main steps and their searches are done.  The loop walks
+// run_system's control flow -- the main step, detect_transits!'s body loop,
+// findtransit!'s iterations -- one step at a time, so that the steps of two
+// systems (of any kind) can share one combined step.
+enum : int { ACT_MAIN = 0, ACT_NEWTON = 1, ACT_JSTEP = 2, ACT_DONE = 3 };
+struct RunCursor {
+  int64_t is, iend;  // next main step, end of the visit
+  int act;
+  int body;          // next body of detect_transits' loop
+  int jb;            // body of the search in progress
+  int iter;          // its Newton iterations
+  int saved;         // s_prior taken after this main step
+  double gi, dt0, stmp, tt1, tt2;
+};
+
+// detect_transits! (timing.jl:3-29) from body c.body on, as detect_transits
+// runs it, until a transit starts a search (act = ACT_NEWTON) or the bodies
+// are done (act = the next main step, or ACT_DONE).
+template <int NB, bool G>
+__device__ void cursor_detect(Smem<NB, G>& S, const SysDesc& D, RunCursor& c, int lane) {
+  const int ti = D.ti;
+  int act = c.is < c.iend ? ACT_MAIN : ACT_DONE;
+  int body = NB;
+#pragma unroll 1
+  for (int i = c.body; i < NB; ++i) {
+    if (i == ti) continue;
+    const double gi = gsky<NB>(S, i, ti);
+    const double ri = __builtin_sqrt((S.x[3 * i] * S.x[3 * i] + S.x[3 * i + 1] * S.x[3 * i + 1]) + S.x[3 * i + 2] * S.x[3 * i + 2]);
+    if (gi > 0.0 && S.gsave[i] < 0.0 && -S.x[3 * i + 2] > 0.25 * ri && ri < 1e12) {
+      const int64_t cnt = S.count[i] + 1;
+      wsync();
+      if (lane == 0) S.count[i] = cnt;
+      wsync();
+      if (cnt >= 1 && cnt <= D.ntt) {
+        if (!c.saved) {
+          save_prior<NB>(S, lane);
+          c.saved = 1;
+        }
+        c.dt0 = ((-gi) * D.hstep) / (gi - S.gsave[i]);
+        c.gi = gi;
+        c.jb = i;
+        c.iter = 0;
+        c.stmp = 0.0;
+        c.tt1 = c.dt0 + 1.0;
+        c.tt2 = c.dt0 + 2.0;
+        act = ACT_NEWTON;
+        body = i;  // gsave[i] is written when the search ends
+        break;
+      }
+    }
+    wsync();
+    if (lane == 0) S.gsave[i] = gi;
+    wsync();
+  }
+  c.act = __builtin_amdgcn_readfirstlane(act);
+  c.body = __builtin_amdgcn_readfirstlane(body);
+}
+
+// The step of the cursor's next action: its size and flags, and the state it
+// starts from (findtransit! restores s_prior before each iteration, and keeps
+// the Jacobian for the final gradient step, timing.jl:45-60).
+template <int NB, bool G>
+__device__ void cursor_prepare(Smem<NB, G>& S, const SysDesc& D, const SysBufs& B, RunCursor& c, double& h, bool& nJ,
+                               bool& nQ, int lane) {
+  const bool grad = D.grad != 0;
+  if (c.act == ACT_MAIN) {
+    h = D.h;
+    nJ = grad;
+    nQ = grad && c.is + 1 == D.nsteps;  // dq/dt of the run's last step, as run_system forms it
+    return;
+  }
+  restore_prior<NB>(S, lane);
+  h = c.dt0;
+  nQ = true;
+  if (c.act == ACT_NEWTON) {
+    c.tt2 = c.tt1;
+    c.tt1 = c.dt0;
+    nJ = false;
+    return;
+  }
+  constexpr int N6 = 6 * NB, N7 = 7 * NB;
+  const auto J_ = jj<NB>(S);
+  double* ws = B.ws + D.off_ws;
+  for (int col = lane; col < N7; col += WAVE)
+    for (int r = 0; r < N6; ++r) {
+      ws[r * N7 + col] = J_[(unsigned)(r * N7 + col)];
+      ws[N6 * N7 + r * N7 + col] = J_[(unsigned)(Smem<NB, G>::JOFF + r * N7 + col)];
+    }
+  nJ = true;
+}
+
+// The transit's records once its search has converged (findtransit!'s end,
+// timing.jl:61-110): tt (and b^2, v_sky), with the gradient step just taken
+// the dt/dq rows, then the Jacobian restored; the search counters.
+template <int NB, bool G>
+__device__ void cursor_record(Smem<NB, G>& S, const SysDesc& D, const SysBufs& B, const RunCursor& c, int lane) {
+  constexpr int N6 = 6 * NB, N7 = 7 * NB;
+  const int n = NB, ti = D.ti, jb = c.jb, nb = D.ntbv;
+  const double dt0 = c.dt0;
+  const int64_t cc = S.count[jb] - 1;
+  double* tt = B.tt + D.off_tt;
+  if (D.grad) {
+    const auto J_ = jj<NB>(S);
+    double* ws = B.ws + D.off_ws;
+    const double gd = gdot<NB>(S, ti, jb);
+    const double dx = S.x[3 * jb] - S.x[3 * ti], dy = S.x[3 * jb + 1] - S.x[3 * ti + 1];
+    const double dvx = S.v[3 * jb] - S.v[3 * ti], dvy = S.v[3 * jb + 1] - S.v[3 * ti + 1];
+    double* dq = B.dq + D.off_dq;
+    const int64_t ntt = D.ntt;
+    for (int col = lane; col < N7; col += WAVE) {
+      const int rj = 6 * jb, ri = 6 * ti;
+      const double d0 = -((((J_[(unsigned)(rj * N7 + col)] - J_[(unsigned)(ri * N7 + col)]) * dvx +
+                            (J_[(unsigned)((rj + 1) * N7 + col)] - J_[(unsigned)((ri + 1) * N7 + col)]) * dvy) +
+                           (J_[(unsigned)((rj + 3) * N7 + col)] - J_[(unsigned)((ri + 3) * N7 + col)]) * dx) +
+                          (J_[(unsigned)((rj + 4) * N7 + col)] - J_[(unsigned)((ri + 4) * N7 + col)]) * dy) /
+                        gd;
+      const int64_t base = (int64_t)jb + n * (cc + ntt * (int64_t)col);
+      dq[0 + nb * base] = d0;
+      if (nb == 3) {
+        const double vsky = __builtin_sqrt(dvx * dvx + dvy * dvy);
+        const double dvdt = (dvx * (S.dqdt[7 * jb + 3] - S.dqdt[7 * ti + 3]) +
+                             dvy * (S.dqdt[7 * jb + 4] - S.dqdt[7 * ti + 4])) / vsky;
+        dq[1 + nb * base] = ((J_[(unsigned)((rj + 3) * N7 + col)] - J_[(unsigned)((ri + 3) * N7 + col)]) * dvx +
+                             (J_[(unsigned)((rj + 4) * N7 + col)] - J_[(unsigned)((ri + 4) * N7 + col)]) * dvy) / vsky +
+                            dvdt * d0;
+        dq[2 + nb * base] = 2.0 * ((J_[(unsigned)(rj * N7 + col)] - J_[(unsigned)(ri * N7 + col)]) * dx +
+                                   (J_[(unsigned)((rj + 1) * N7 + col)] - J_[(unsigned)((ri + 1) * N7 + col)]) * dy);
+      }
+    }
+    if (lane == 0) {
+      tt[0 + nb * (jb + n * cc)] = S.t + dt0;
+      if (nb == 3) {
+        tt[1 + nb * (jb + n * cc)] = __builtin_sqrt(dvx * dvx + dvy * dvy);
+        tt[2 + nb * (jb + n * cc)] = dx * dx + dy * dy;
+      }
+    }
+    for (int col = lane; col < N7; col += WAVE)
+      for (int r = 0; r < N6; ++r) {
+        J_[(unsigned)(r * N7 + col)] = ws[r * N7 + col];
+        J_[(unsigned)(Smem<NB, G>::JOFF + r * N7 + col)] = ws[N6 * N7 + r * N7 + col];
+      }
+  } else if (lane == 0) {
+    tt[0 + nb * (jb + n * cc)] = S.t + dt0;
+    if (nb == 3) {
+      const double dx = S.x[3 * jb] - S.x[3 * ti], dy = S.x[3 * jb + 1] - S.x[3 * ti + 1];
+      const double dvx = S.v[3 * jb] - S.v[3 * ti], dvy = S.v[3 * jb + 1] - S.v[3 * ti + 1];
+      tt[1 + nb * (jb + n * cc)] = __builtin_sqrt(dvx * dvx + dvy * dvy);
+      tt[2 + nb * (jb + n * cc)] = dx * dx + dy * dy;
+    }
+  }
+  if (lane == 0) {
+    int64_t* ints = B.ints + D.off_int;
+    ints[NB + 1] += 1;
+    ints[NB + 2] += c.iter;
+  }
+  wsync();
+}
+
+// After the cursor's step: the main step's time and transit tests, or the
+// Newton update (timing.jl:49-56) and, once converged, the transit's records,
+// s_prior restored and the next bodies' tests.
+template <int NB, bool G>
+__device__ void cursor_finish(Smem<NB, G>& S, const SysDesc& D, const SysBufs& B, RunCursor& c, int lane) {
+  if (c.act == ACT_MAIN) {
+    if (lane == 0) S.t = D.t0 + ((double)(c.is + 1) * D.h);
+    wsync();
+    c.is += 1;
+    c.saved = 0;
+    c.body = 0;
+    cursor_detect<NB>(S, D, c, lane);
+    return;
+  }
+  if (c.act == ACT_NEWTON) {
+    const double gs = gsky<NB>(S, D.ti, c.jb);
+    const double gd = gdot<NB>(S, D.ti, c.jb);
+    const double dt = -gs / gd;
+    kahan(c.dt0, c.stmp, dt);
+    c.iter += 1;
+    if (!(c.iter >= 20 || c.dt0 == c.tt1 || c.dt0 == c.tt2)) return;  // another iteration
+    if (D.grad) {
+      c.act = ACT_JSTEP;
+      return;
+    }
+  }
+  cursor_record<NB>(S, D, B, c, lane);
+  restore_prior<NB>(S, lane);
+  wsync();
+  if (lane == 0) S.gsave[c.jb] = c.gi;
+  wsync();
+  c.body = c.jb + 1;
+  cursor_detect<NB>(S, D, c, lane);
+}
+
 // Two MODE_TRANSITS systems of NB bodies (or resume them for at most their
-// chunks of steps): in lockstep while both have steps left in this visit,
-// then the rest of the longer one alone; each ends as run_system ends it.
-// Every system's results are exactly its run_system results: the shared
-// stages run the same operations on its own record, the transit searches run
-// one system at a time, each on the whole work area.
+// chunks of steps), step by step: every pass takes the next step of each
+// system that has one -- a main step, a Newton iteration of a transit search
+// or a search's gradient step -- both in one combined step when both have
+// one, so that one system's searches overlap the other's steps; each ends as
+// run_system ends it.  Every system's results are exactly its run_system
+// results: each step runs the same operations on its own record, and each
+// system's sequence of steps, tests and records is run_system's.
 template <int NB, int HI, bool G>
 __device__ void run_pair(Duo<NB, HI, G>& P, const SysDesc& D0, const SysDesc& D1, const SysBufs& B, int64_t chunk0,
                          int64_t chunk1, int lane) {
@@ -1851,37 +2049,45 @@ __device__ void run_pair(Duo<NB, HI, G>& P, const SysDesc& D0, const SysDesc& D1
         if (i != D1.ti) S1.gsave[i] = gsky<NB>(S1, i, D1.ti);
   }
   wsync();
-  int64_t is0 = st0, is1 = st1;
   if (on0 && on1 && same_schedule<NB>(S0, S1)) {
+    // step by step: each pass takes one step of each system with work left
+    // -- a main step, a Newton iteration or a transit's gradient step, of any
+    // mix -- as one combined step when both have one
     const bool k0 = D0.any_kick != 0, k1 = D1.any_kick != 0;
-    const bool g0 = D0.grad != 0, g1 = D1.grad != 0;
-    double* ws0 = B.ws + D0.off_ws;
-    double* ws1 = B.ws + D1.off_ws;
+    RunCursor c0, c1;
+    c0.is = st0; c0.iend = e0; c0.act = st0 < e0 ? ACT_MAIN : ACT_DONE; c0.body = NB; c0.saved = 0;
+    c1.is = st1; c1.iend = e1; c1.act = st1 < e1 ? ACT_MAIN : ACT_DONE; c1.body = NB; c1.saved = 0;
 #pragma unroll 1
-    for (; is0 < e0 && is1 < e1; ++is0, ++is1) {
-      const bool q0 = g0 && is0 + 1 == D0.nsteps, q1 = g1 && is1 + 1 == D1.nsteps;
-      bool done = false;
-      if constexpr (!G) {
-        if (!k0 && !k1) {
-          Swork<NB>& W1 = *reinterpret_cast<Swork<NB>*>(P.w + sizeof(Swork<NB>));
-          ahl21_step2h<NB>(S0, S1, W, W1, W.slot, duo_ch<NB, HI, G>(), D0.h, D1.h, g0, q0, g1, q1, lane);
-          done = true;
+    while (c0.act != ACT_DONE || c1.act != ACT_DONE) {
+      const bool a0 = c0.act != ACT_DONE, a1 = c1.act != ACT_DONE;
+      double h0 = 0.0, h1 = 0.0;
+      bool nJ0 = false, nQ0 = false, nJ1 = false, nQ1 = false;
+      if (a0) cursor_prepare<NB>(S0, D0, B, c0, h0, nJ0, nQ0, lane);
+      if (a1) cursor_prepare<NB>(S1, D1, B, c1, h1, nJ1, nQ1, lane);
+      if (a0 && a1) {
+        bool done = false;
+        if constexpr (!G) {
+          if (!k0 && !k1) {
+            Swork<NB>& W1 = *reinterpret_cast<Swork<NB>*>(P.w + sizeof(Swork<NB>));
+            ahl21_step2h<NB>(S0, S1, W, W1, W.slot, duo_ch<NB, HI, G>(), h0, h1, nJ0, nQ0, nJ1, nQ1, lane);
+            done = true;
+          }
         }
+        if (!done)
+          ahl21_step2<NB, G, duo_phi2<NB, HI, G>()>(S0, S1, W, duo_ch<NB, HI, G>(), h0, h1, nJ0, nQ0, nJ1, nQ1, k0,
+                                                     k1, lane);
+      } else if (a0) {
+        ahl21_step<NB>(S0, W, h0, nJ0, nQ0, k0, lane);
+      } else {
+        ahl21_step<NB>(S1, W, h1, nJ1, nQ1, k1, lane);
       }
-      if (!done)
-        ahl21_step2<NB, G, duo_phi2<NB, HI, G>()>(S0, S1, W, duo_ch<NB, HI, G>(), D0.h, D1.h, g0, q0, g1, q1, k0, k1,
-                                                   lane);
-      if (lane == 0) {
-        S0.t = D0.t0 + ((double)(is0 + 1) * D0.h);
-        S1.t = D1.t0 + ((double)(is1 + 1) * D1.h);
-      }
-      wsync();
-      detect_transits<NB>(S0, W, D0, B, lane, ws0);
-      detect_transits<NB>(S1, W, D1, B, lane, ws1);
+      if (a0) cursor_finish<NB>(S0, D0, B, c0, lane);
+      if (a1) cursor_finish<NB>(S1, D1, B, c1, lane);
     }
+  } else {
+    transit_steps<NB>(S0, W, D0, B, st0, e0, lane);
+    transit_steps<NB>(S1, W, D1, B, st1, e1, lane);
   }
-  transit_steps<NB>(S0, W, D0, B, is0, e0, lane);
-  transit_steps<NB>(S1, W, D1, B, is1, e1, lane);
   if (on0) {
     store_system<NB>(S0, D0, B, lane);
     if (lane == 0) ints0[NB] = (e0 == D0.nsteps) ? D0.nsteps + 1 : e0;
