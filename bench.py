@@ -50,7 +50,7 @@ FP64_PEAK_TFLOPS = 78.6
 # counters' reading of nbg_copy_kernel of known size (the figures and the
 # commands are in profiles/nbg_run_c2_round_counters.md and
 # profiles/nbg_run_duo_c2_round_counters.md)
-HBM_BYTES_PER_UNIT = {"nbg_run<9>": 373.3e3, "nbg_run_duo<9>": 514.6e3}
+HBM_BYTES_PER_UNIT = {"nbg_run<9>": 373.3e3, "nbg_run_duo<9>": 480.5e3}
 METRIC = "system-steps/s w/ Jacobian, batched vs plain GPU; p50/p99 request latency"
 T_START = time.perf_counter()
 
