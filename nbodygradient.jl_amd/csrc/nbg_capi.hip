@@ -263,6 +263,7 @@ struct nbg_ctx {
   int coop_mode = 1;   // 0 off, 1 systems longer than the batch's throughput bound, 2 the longest ones
   int coop_max = 128;  // at most this many cooperative systems per batch
   int systems_per_wave = 1;  // batches: 2 runs transit systems of equal body count two per wave (nbg_run_duo)
+  int row_update = 0;        // Jacobian row update: 0 the reference's order (bitwise), 1 factored
   int64_t coop_polls = 0;  // > 0: bound on every inter-wave wait (diagnostic); 0: the defaults
   int nstreams = 4;
   std::vector<hipStream_t> streams;
@@ -434,6 +435,7 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
   if (K > 0) {
     NBG_HIP(S.coop.ensure((int64_t)K * nbg_coop_stride()));
     nbg::GroupList cg{};
+    cg.rowupd = ctx->row_update;
     for (size_t g = 0; g < G; ++g) {
       if (kg[g] == 0) continue;
       cg.nsys[cg.ng] = kg[g];
@@ -468,6 +470,7 @@ int run_groups(nbg_ctx* ctx, Store& S, const Layout& L, int64_t chunk, double* k
   for (size_t g : gord) {
     if (g == 1 && chunk == 0) std::this_thread::sleep_for(std::chrono::microseconds(2000));
     nbg::GroupList gl{};
+    gl.rowupd = ctx->row_update;
     for (size_t h = g; h < G; ++h) {  // own class, then the smaller ones
       const int skip = kg[h];  // the cooperative systems
       gl.nsys[gl.ng] = L.gcnt[h] - skip;
@@ -782,6 +785,14 @@ int nbg_set_coop(nbg_ctx* ctx, int32_t mode, int32_t max_systems) {
   return 0;
 }
 
+int nbg_set_row_update(nbg_ctx* ctx, int32_t mode) {
+  if (!ctx) return -1;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  NBG_ARG(mode == 0 || mode == 1, "row update mode must be 0 (reference order) or 1 (factored)");
+  ctx->row_update = mode;
+  return 0;
+}
+
 int nbg_set_systems_per_wave(nbg_ctx* ctx, int32_t k) {
   if (!ctx) return -1;
   std::lock_guard<std::mutex> lk(ctx->mu);
@@ -981,6 +992,7 @@ int nbg_integrate_transits_noalloc(nbg_ctx* ctx, int32_t n, double h, double tma
   }
   // one persistent workgroup of the system's launch class takes the system
   nbg::GroupList gl{};
+  gl.rowupd = ctx->row_update;
   gl.ng = 1;
   gl.nsys[0] = 1;
   gl.descs[0] = reinterpret_cast<const nbg::SysDesc*>(dev(o_desc));

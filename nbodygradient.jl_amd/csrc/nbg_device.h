@@ -70,6 +70,7 @@ template <int NB, bool G = (NB >= JGLOBAL_MIN)> struct Smem {
   // when it lies in an earlier chunk, or zero at the start of a step.
   PairSlot jlast;
   int seq;  // the system has such a pair: sweeps keep the sequential order
+  int rowupd;  // SysDesc::rowupd: the form of the Jacobian row update (pair_cols)
   double dqij[14];       // dqdt_ij
   double x[N3], v[N3], xe[N3], ve[N3], m[NB];
   double px[N3], pv[N3], pxe[N3], pve[N3];  // s_prior
@@ -699,7 +700,8 @@ template <int NB, bool G> __device__ inline void drift_cols(Smem<NB, G>& S, int 
 // running sum starts at +0 and can never become -0, so the sign of a zero
 // coefficient cannot change it), and the mass columns 6 and 13 take their
 // coefficients from the slot, where pair_jacobian stored them.
-template <int NB, bool G> __device__ inline void pair_cols(Smem<NB, G>& S, const PairSlot& sl, int i, int j, int c) {
+template <int NB, bool G, bool FACT = false>
+__device__ inline void pair_cols(Smem<NB, G>& S, const PairSlot& sl, int i, int j, int c) {
   const auto J_ = jj<NB>(S);
   constexpr int N7 = 7 * NB;
   const int ri = 6 * i, rj = 6 * j;
@@ -715,6 +717,26 @@ template <int NB, bool G> __device__ inline void pair_cols(Smem<NB, G>& S, const
   const double a13 = (c == 7 * j + 6) ? 1.0 : 0.0;
   const double mi = sl.mi, mj = sl.mj;
   double o[12];
+  if constexpr (FACT) {
+    // Factored (rounds differently from the reference's mul!): the Kepler
+    // block of jac_ij gives the rows of i +mj*kep and those of j -mi*kep
+    // times the same difference a_i - a_j, so kep*(a_i - a_j) is formed once.
+    double d[6], sk[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) d[k] = a[k] - a[6 + k];
+#pragma unroll
+    for (int rk = 0; rk < 6; ++rk) {
+      double t = 0.0;
+#pragma unroll
+      for (int k = 0; k < 6; ++k) t = fmad(sl.kep[rk + 6 * k], d[k], t);
+      sk[rk] = t;
+    }
+#pragma unroll
+    for (int rk = 0; rk < 6; ++rk) {
+      o[rk] = fmad(sl.mc[12 + rk], a13, fmad(sl.mc[rk], a6, mj * sk[rk]));
+      o[6 + rk] = fmad(sl.mc[18 + rk], a13, fmad(sl.mc[6 + rk], a6, (-mi) * sk[rk]));
+    }
+  } else {
 #pragma unroll
   for (int q = 0; q < 12; ++q) o[q] = 0.0;
   // k = 0..5: rows of i get +mj*kep * a_i[k], rows of j -mi*kep * a_i[k]
@@ -739,6 +761,7 @@ template <int NB, bool G> __device__ inline void pair_cols(Smem<NB, G>& S, const
     }
 #pragma unroll
   for (int q = 0; q < 12; ++q) o[q] = fmad(sl.mc[12 + q], a13, o[q]);
+  }
 #pragma unroll
   for (int q = 0; q < 12; ++q) {
     double sv = a[q], se = e[q];
@@ -831,8 +854,9 @@ __device__ __attribute__((always_inline)) inline void pair_level_lane(Smem<NB, G
   sl.valid = (gm != 0.0); sl.i = i; sl.j = j;
 }
 
-// Phase C of one chunk of `cnt` pairs (slots[0..cnt)) of system S.
-template <int NB, bool G, int STEP = WAVE>
+// Phase C of one chunk of `cnt` pairs (slots[0..cnt)) of system S, with the
+// row update in the reference's order or (FACT) factored.
+template <int NB, bool G, int STEP = WAVE, bool FACT = false>
 __device__ __attribute__((always_inline)) inline void sweep_rows(Smem<NB, G>& S, const PairSlot* slots, int cnt,
                                                                  bool needJ, bool needQ, int col) {
   constexpr int N7 = 7 * NB;
@@ -853,7 +877,7 @@ __device__ __attribute__((always_inline)) inline void sweep_rows(Smem<NB, G>& S,
     // jlast when that lies in an earlier chunk; dqij keeps its values.)
     const PairSlot& src = lastq >= 0 ? slots[lastq] : S.jlast;
     if (needJ)
-      for (int c = col; c < N7; c += STEP) pair_cols<NB>(S, src, i, j, c);
+      for (int c = col; c < N7; c += STEP) pair_cols<NB, G, FACT>(S, src, i, j, c);
     if (needQ) {
       double q1[14];
       for (int k = 0; k < 7; ++k) {
@@ -885,7 +909,7 @@ __device__ __attribute__((always_inline)) inline void sweep_rows(Smem<NB, G>& S,
   }
 }
 
-template <int NB, bool G>
+template <int NB, bool G, bool FACT = false>
 __device__ __attribute__((always_inline)) inline void sweep(Smem<NB, G>& S, Swork<NB>& W, int dir, double hh, bool needJ,
                                                             bool needQ, int lane) {
   constexpr int CH = Swork<NB>::CH;
@@ -918,7 +942,7 @@ __device__ __attribute__((always_inline)) inline void sweep(Smem<NB, G>& S, Swor
     if (lane < cnt && W.slot[lane].valid) pair_jacobian(&W.slot[lane], drift_first);
     wsync();
     // ---- C: jac_ij, Jacobian rows and dq/dt, in sweep order
-    sweep_rows<NB>(S, W.slot, cnt, needJ, needQ, lane);
+    sweep_rows<NB, G, WAVE, FACT>(S, W.slot, cnt, needJ, needQ, lane);
   }
 }
 
@@ -929,7 +953,7 @@ __device__ __attribute__((always_inline)) inline void sweep(Smem<NB, G>& S, Swor
 // Each system's slots are its own (slots[0, ch) and slots[ch, 2 ch)), and
 // every operation of a system is the one-system sweep's, so each system's
 // results are exactly its one-system results.
-template <int NB, bool G, bool HALF = false>
+template <int NB, bool G, bool HALF = false, bool FACT = false>
 __device__ __attribute__((always_inline)) inline void sweep2(Smem<NB, G>& S0, Smem<NB, G>& S1, PairSlot* slots, int ch,
                                                              int dir, double hh0, double hh1, bool nJ0, bool nQ0,
                                                              bool nJ1, bool nQ1, int lane) {
@@ -967,10 +991,11 @@ __device__ __attribute__((always_inline)) inline void sweep2(Smem<NB, G>& S0, Sm
     wsync();
     if constexpr (HALF) {  // both systems' rows at once, one per half of the wave
       const bool sys = lane >= 32;
-      sweep_rows<NB, G, 32>(sys ? S1 : S0, slots + (sys ? ch : 0), cnt, sys ? nJ1 : nJ0, sys ? nQ1 : nQ0, lane & 31);
+      sweep_rows<NB, G, 32, FACT>(sys ? S1 : S0, slots + (sys ? ch : 0), cnt, sys ? nJ1 : nJ0, sys ? nQ1 : nQ0,
+                                  lane & 31);
     } else {
-      if (pj0) sweep_rows<NB>(S0, slots, cnt, nJ0, nQ0, lane);
-      if (pj1) sweep_rows<NB>(S1, slots + ch, cnt, nJ1, nQ1, lane);
+      if (pj0) sweep_rows<NB, G, WAVE, FACT>(S0, slots, cnt, nJ0, nQ0, lane);
+      if (pj1) sweep_rows<NB, G, WAVE, FACT>(S1, slots + ch, cnt, nJ1, nQ1, lane);
     }
   }
 }
@@ -1168,13 +1193,24 @@ __device__ __attribute__((always_inline)) inline void step_post(Smem<NB, G>& S, 
   wsync();
 }
 
+template <int NB, bool G, bool FACT>
+__device__ void ahl21_step_form(Smem<NB, G>& S, Swork<NB>& W, double h, bool needJ, bool needQ, bool anyKick,
+                                int lane) {
+  step_pre<NB>(S, W, h, needJ, needQ, anyKick, lane);
+  sweep<NB, G, FACT>(S, W, 0, 0.5 * h, needJ, needQ, lane);  // forward pair sweep (drift first)
+  step_mid<NB>(S, W, h, needJ, needQ, anyKick, lane);
+  sweep<NB, G, FACT>(S, W, 1, 0.5 * h, needJ, needQ, lane);  // backward pair sweep (drift after)
+  step_post<NB>(S, W, h, needJ, needQ, anyKick, lane);
+}
+
+// The step in the record's form of the Jacobian row update (SysDesc::rowupd):
+// two instantiations, so that neither form's code is in the other's loops.
 template <int NB, bool G>
 __device__ void ahl21_step(Smem<NB, G>& S, Swork<NB>& W, double h, bool needJ, bool needQ, bool anyKick, int lane) {
-  step_pre<NB>(S, W, h, needJ, needQ, anyKick, lane);
-  sweep<NB>(S, W, 0, 0.5 * h, needJ, needQ, lane);  // forward pair sweep (drift first)
-  step_mid<NB>(S, W, h, needJ, needQ, anyKick, lane);
-  sweep<NB>(S, W, 1, 0.5 * h, needJ, needQ, lane);  // backward pair sweep (drift after)
-  step_post<NB>(S, W, h, needJ, needQ, anyKick, lane);
+  if (__builtin_amdgcn_readfirstlane(S.rowupd))
+    ahl21_step_form<NB, G, true>(S, W, h, needJ, needQ, anyKick, lane);
+  else
+    ahl21_step_form<NB, G, false>(S, W, h, needJ, needQ, anyKick, lane);
 }
 
 // One step of each of the two systems of a Duo, in lockstep: the stages
@@ -1196,12 +1232,12 @@ __device__ __attribute__((always_inline)) inline void phi_build2(Smem<NB, G>& S0
   phisalpha_cols<NB, G, 32>(sys ? S1 : S0, sys ? W1 : W0, sys ? h1 : h0, false, sys ? nQ1 : nQ0, lane & 31);
 }
 
-template <int NB, bool G, bool PHI2>
+template <int NB, bool G, bool PHI2, bool FACT>
 __device__ void ahl21_step2(Smem<NB, G>& S0, Smem<NB, G>& S1, Swork<NB>& W, int ch, double h0, double h1,
                             bool nJ0, bool nQ0, bool nJ1, bool nQ1, bool k0, bool k1, int lane) {
   step_pre<NB>(S0, W, h0, nJ0, nQ0, k0, lane);
   step_pre<NB>(S1, W, h1, nJ1, nQ1, k1, lane);
-  sweep2<NB>(S0, S1, W.slot, ch, 0, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
+  sweep2<NB, G, false, FACT>(S0, S1, W.slot, ch, 0, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
   bool shared = false;
   if constexpr (PHI2) {
     if (!k0 && !k1) {  // the second work area follows the first
@@ -1217,7 +1253,7 @@ __device__ void ahl21_step2(Smem<NB, G>& S0, Smem<NB, G>& S1, Swork<NB>& W, int 
     step_mid<NB>(S0, W, h0, nJ0, nQ0, k0, lane);
     step_mid<NB>(S1, W, h1, nJ1, nQ1, k1, lane);
   }
-  sweep2<NB>(S0, S1, W.slot, ch, 1, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
+  sweep2<NB, G, false, FACT>(S0, S1, W.slot, ch, 1, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
   step_post<NB>(S0, W, h0, nJ0, nQ0, k0, lane);
   step_post<NB>(S1, W, h1, nJ1, nQ1, k1, lane);
 }
@@ -1227,7 +1263,7 @@ __device__ void ahl21_step2(Smem<NB, G>& S0, Smem<NB, G>& S1, Swork<NB>& W, int 
 // [0, 32) do the first system's work of every stage, [32, 64) the second's,
 // each lane with the operations the one-system step gives it; the sweeps'
 // Kepler levels and pair Jacobians are shared as in sweep2.  No kicked pairs.
-template <int NB>
+template <int NB, bool FACT>
 __device__ void ahl21_step2h(Smem<NB, false>& S0, Smem<NB, false>& S1, Swork<NB>& W0, Swork<NB>& W1, PairSlot* slots,
                              int ch, double h0, double h1, bool nJ0, bool nQ0, bool nJ1, bool nQ1, int lane) {
   static_assert(7 * NB <= 32, "a system's columns must fit half a wave");
@@ -1238,11 +1274,11 @@ __device__ void ahl21_step2h(Smem<NB, false>& S0, Smem<NB, false>& S1, Swork<NB>
   const double h = sys ? h1 : h0;
   const bool nJ = sys ? nJ1 : nJ0, nQ = sys ? nQ1 : nQ0;
   step_pre<NB, false, 32>(S, W, h, nJ, nQ, false, col);
-  sweep2<NB, false, true>(S0, S1, slots, ch, 0, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
+  sweep2<NB, false, true, FACT>(S0, S1, slots, ch, 0, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
   phisalpha_cols<NB, false, 32>(S, W, h, false, nQ, col);
   wsync();
   step_mid_rest<NB, false, 32>(S, W, nJ, nQ, col);
-  sweep2<NB, false, true>(S0, S1, slots, ch, 1, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
+  sweep2<NB, false, true, FACT>(S0, S1, slots, ch, 1, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
   step_post<NB, false, 32>(S, W, h, nJ, nQ, false, col);
 }
 
@@ -1455,7 +1491,10 @@ template <int NB, bool G> __device__ void load_system(Smem<NB, G>& S, const SysD
   }
   for (int q = lane; q < N7; q += WAVE) S.dqdt[q] = st[off_dqdt(NB) + q];
   for (int q = lane; q < NB * NB; q += WAVE) S.pair[q] = B.pair[D.off_pair + q];
-  if (lane == 0) S.t = st[off_t(NB)];
+  if (lane == 0) {
+    S.t = st[off_t(NB)];
+    S.rowupd = D.rowupd;
+  }
   wsync();
   if (lane == 0) {
     constexpr int NPS = Smem<NB>::NP > 0 ? Smem<NB>::NP : 1;
@@ -2054,6 +2093,7 @@ __device__ void run_pair(Duo<NB, HI, G>& P, const SysDesc& D0, const SysDesc& D1
     // -- a main step, a Newton iteration or a transit's gradient step, of any
     // mix -- as one combined step when both have one
     const bool k0 = D0.any_kick != 0, k1 = D1.any_kick != 0;
+    const bool fact = D0.rowupd != 0;  // the launch's form of the row update (both records)
     RunCursor c0, c1;
     c0.is = st0; c0.iend = e0; c0.act = st0 < e0 ? ACT_MAIN : ACT_DONE; c0.body = NB; c0.saved = 0;
     c1.is = st1; c1.iend = e1; c1.act = st1 < e1 ? ACT_MAIN : ACT_DONE; c1.body = NB; c1.saved = 0;
@@ -2069,13 +2109,20 @@ __device__ void run_pair(Duo<NB, HI, G>& P, const SysDesc& D0, const SysDesc& D1
         if constexpr (!G) {
           if (!k0 && !k1) {
             Swork<NB>& W1 = *reinterpret_cast<Swork<NB>*>(P.w + sizeof(Swork<NB>));
-            ahl21_step2h<NB>(S0, S1, W, W1, W.slot, duo_ch<NB, HI, G>(), h0, h1, nJ0, nQ0, nJ1, nQ1, lane);
+            if (fact)
+              ahl21_step2h<NB, true>(S0, S1, W, W1, W.slot, duo_ch<NB, HI, G>(), h0, h1, nJ0, nQ0, nJ1, nQ1, lane);
+            else
+              ahl21_step2h<NB, false>(S0, S1, W, W1, W.slot, duo_ch<NB, HI, G>(), h0, h1, nJ0, nQ0, nJ1, nQ1, lane);
             done = true;
           }
         }
-        if (!done)
-          ahl21_step2<NB, G, duo_phi2<NB, HI, G>()>(S0, S1, W, duo_ch<NB, HI, G>(), h0, h1, nJ0, nQ0, nJ1, nQ1, k0,
-                                                     k1, lane);
+        if (!done) {
+          constexpr bool PHI2 = duo_phi2<NB, HI, G>();
+          if (fact)
+            ahl21_step2<NB, G, PHI2, true>(S0, S1, W, duo_ch<NB, HI, G>(), h0, h1, nJ0, nQ0, nJ1, nQ1, k0, k1, lane);
+          else
+            ahl21_step2<NB, G, PHI2, false>(S0, S1, W, duo_ch<NB, HI, G>(), h0, h1, nJ0, nQ0, nJ1, nQ1, k0, k1, lane);
+        }
       } else if (a0) {
         ahl21_step<NB>(S0, W, h0, nJ0, nQ0, k0, lane);
       } else {
@@ -2390,7 +2437,8 @@ __global__ __launch_bounds__(2 * WAVE) void nbg_run_coop(GroupList gl, SysBufs B
       __syncthreads();  // thread 0 writes `sys` again
       continue;
     }
-    const SysDesc D = gl.descs[g][idx];
+    SysDesc D = gl.descs[g][idx];
+    D.rowupd = gl.rowupd;
     const int64_t* ints = B.ints + D.off_int;
     const int64_t before = ints[D.n];
     const int64_t tr0 = ints[D.n + 1], it0 = ints[D.n + 2];
@@ -2437,7 +2485,8 @@ __global__ __launch_bounds__(WAVE) void nbg_run(GroupList gl, SysBufs B, int64_t
       ++g;
       continue;
     }
-    const SysDesc D = gl.descs[g][idx];
+    SysDesc D = gl.descs[g][idx];
+    D.rowupd = gl.rowupd;
     const int64_t* ints = B.ints + D.off_int;
     const int64_t before = ints[D.n];  // step cursor (lane 0 reads and writes it)
     const int64_t tr0 = ints[D.n + 1], it0 = ints[D.n + 2];
@@ -2468,8 +2517,10 @@ __global__ __launch_bounds__(WAVE) void nbg_run_duo(GroupList gl, SysBufs B, int
     }
     const int idx = __builtin_amdgcn_readfirstlane(gl.ustart[g][u]);
     const bool two = __builtin_amdgcn_readfirstlane(gl.ustart[g][u + 1]) - idx == 2;
-    const SysDesc D0 = gl.descs[g][idx];
-    const SysDesc D1 = gl.descs[g][two ? idx + 1 : idx];
+    SysDesc D0 = gl.descs[g][idx];
+    SysDesc D1 = gl.descs[g][two ? idx + 1 : idx];
+    D0.rowupd = gl.rowupd;
+    D1.rowupd = gl.rowupd;
     const int64_t* ints0 = B.ints + D0.off_int;
     const int64_t* ints1 = B.ints + D1.off_int;
     const int64_t before0 = ints0[D0.n], tr00 = ints0[D0.n + 1], it00 = ints0[D0.n + 2];

@@ -46,6 +46,8 @@ struct SysDesc {
   int32_t any_kick;  // some pair[i,j] (i<j) is true
   int32_t tag;       // reported in SysBufs.done_list when the system finishes
   int32_t traj;      // MODE_CARTESIAN: TRAJ_* flags of the optional records
+  int32_t rowupd;    // Jacobian row update of the pair sweeps: 0 the reference's order, 1 factored (set per launch)
+  int32_t pad_;
   int64_t nsteps;    // full steps in the run
   int64_t chunk;     // > 0: at most this many steps per visit (overrides the launch's chunk)
   int64_t ntt;
@@ -69,6 +71,7 @@ struct GroupList {
   // two systems; unit u is descs[g][ustart[g][u] .. ustart[g][u + 1])
   int32_t nunits[MAX_GROUPS];
   const int* ustart[MAX_GROUPS];
+  int32_t rowupd;  // SysDesc::rowupd of every system of the launch
 };
 
 struct SysBufs {

@@ -89,6 +89,7 @@ def lib():
     L.nbg_server_footprint.restype = C.c_int64
     L.nbg_copy_calibrate.argtypes = [VP, C.c_int64, C.c_int32, D]
     L.nbg_set_systems_per_wave.argtypes = [VP, C.c_int32]
+    L.nbg_set_row_update.argtypes = [VP, C.c_int32]
     L.nbg_batch_autotune_systems_per_wave.argtypes = [C.c_int32]
     L.nbg_batch_autotune_systems_per_wave.restype = C.c_int32
     L.nbg_probe_row_update.argtypes = [VP, C.c_int32, C.c_int32, C.c_int32, D, D]
@@ -165,6 +166,13 @@ class Context:
         the wave's lanes; results are the same bits either way (C ABI:
         nbg_set_systems_per_wave)."""
         self.check(lib().nbg_set_systems_per_wave(self._h, int(k)), "nbg_set_systems_per_wave")
+
+    def set_row_update(self, mode=0):
+        """Jacobian row update of the pair sweeps: 0 the reference's order
+        (bitwise), 1 factored (Jacobians and their gradients to ~1e-13
+        relative; states and transit times bitwise).  C ABI:
+        nbg_set_row_update."""
+        self.check(lib().nbg_set_row_update(self._h, int(mode)), "nbg_set_row_update")
 
     def set_coop(self, mode=1, max_systems=128):
         """Two-wave workgroups for the longest transit-search systems of batch

@@ -1,0 +1,118 @@
+"""The factored Jacobian row update (nbg_set_row_update(1)): the state, dq/dt
+and the transit times, which never read the Jacobian, stay bit for bit the
+oracle's; the Jacobian and the transit-time gradients agree with the oracle
+within the tolerance asked of Jacobian entries (1e-10 relative to each
+array's largest entry, the reference's maxabs norm) for one system per call
+(C0, and C1's whole 66,667-step run against its golden file), and for
+batches with one and with two systems per wave.  Each test prints the
+largest relative difference it saw."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib as O
+import nbodygradient_amd as nb
+from nbodygradient_amd import workloads as W
+from nbodygradient_amd.batch import Batch
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+REL = 1e-10
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def _tol(a, b, rel=REL):
+    """max |a - b| <= rel * max |b| (printed)."""
+    r = _rel(a, b)
+    print("  max relative difference %.3g" % r, flush=True)
+    return r <= rel
+
+
+def _factored(fn):
+    ctx = nb.default_context()
+    ctx.set_row_update(1)
+    try:
+        return fn()
+    finally:
+        ctx.set_row_update(0)
+
+
+def test_factored_single_system_c0():
+    print("gpu: factored row update, C0", flush=True)
+    spec = W.c0()
+
+    def run():
+        ic = nb.ElementsIC(spec.t0, spec.n, spec.elements)
+        s = nb.State(ic)
+        tt = nb.TransitTiming(spec.tmax, ic)
+        nb.Integrator(spec.h, spec.t0, spec.tmax)(s, tt, grad=True)
+        return s, tt
+
+    s, tt = _factored(run)
+    x, v, jinit = W.initial_states([spec], processes=1)[0]
+    os_ = O.OState(x, v, spec.elements[:, 0], t=spec.t0, jac_init=jinit)
+    ott, odq, ode, oc = O.integrate_transits(os_, spec.h, spec.tmax, spec.ntt(), grad=True)
+    assert np.array_equal(tt.count, oc) and oc.sum() > 100
+    assert np.array_equal(tt.tt, ott)
+    assert np.array_equal(s.x, os_.x) and np.array_equal(s.v, os_.v)
+    assert _tol(tt.dtdq0, odq) and _tol(tt.dtdelements, ode)
+    assert _tol(s.jac_step, os_.jac_step)
+    assert not np.array_equal(s.jac_step, os_.jac_step)  # the factored form is in use
+
+
+def test_factored_c1_full_run():
+    print("gpu: factored row update, C1 full run", flush=True)
+    g = np.load(os.path.join(HERE, "golden", "c1_oracle.npz"))
+    spec = W.c1()
+
+    def run():
+        ic = nb.ElementsIC(spec.t0, spec.n, spec.elements)
+        s = nb.State(ic)
+        tt = nb.TransitTiming(spec.tmax, ic)
+        nb.Integrator(spec.h, spec.t0, spec.tmax)(s, tt, grad=True)
+        return s, tt
+
+    s, tt = _factored(run)
+    assert np.array_equal(tt.count, g["count"])
+    assert np.array_equal(tt.tt, g["tt"])
+    assert np.array_equal(s.x, g["x"]) and np.array_equal(s.v, g["v"])
+    assert _tol(tt.dtdelements[:, ::int(g["every"])], g["dtdelements_sel"])
+
+
+@pytest.mark.parametrize("spw", [1, 2])
+def test_factored_batch(spw):
+    print("gpu: factored row update, batch spw=%d" % spw, flush=True)
+    rng = np.random.default_rng(61)
+    specs = []
+    for q in range(18):
+        sp = W.c2_system(rng, 3 + q % 7)
+        sp.tmax = min(sp.tmax, 400 * sp.h)
+        specs.append(sp)
+    states = W.initial_states(specs, processes=1)
+    ctx = nb.default_context()
+    ctx.set_systems_per_wave(spw)
+    ctx.set_grid_cap(2)
+    try:
+        def run():
+            b = Batch(specs, states)
+            st = b.run()
+            r = b.download()
+            b.close()
+            return st, r
+        st, res = _factored(run)
+    finally:
+        ctx.set_systems_per_wave(1)
+        ctx.set_grid_cap(0)
+    assert (st["paired_systems"] > 0) == (spw == 2)
+    for sp, (x, v, jinit), r in zip(specs, states, res):
+        os_ = O.OState(x, v, sp.elements[:, 0], t=sp.t0, jac_init=jinit)
+        ott, odq, ode, oc = O.integrate_transits(os_, sp.h, sp.tmax, sp.ntt(), grad=True)
+        assert np.array_equal(r["count"], oc)
+        assert np.array_equal(r["tt"], ott)
+        assert np.array_equal(r["x"], os_.x) and np.array_equal(r["v"], os_.v)
+        if oc.sum() > 0:
+            assert _tol(r["dtdq0"], odq) and _tol(r["dtdelements"], ode)
