@@ -9,10 +9,16 @@ resident in HBM.  One benchmark "step" is one ROUND over the whole batch
 every step, calc_dtdelements! when its run ends) in one persistent launch;
 systems whose run has ended restart from their initial records, so every
 round carries the same mix.  `value` = system-steps (main steps + one per
-recorded transit) of the timed rounds / their wall time.
+recorded transit) of the timed rounds / their wall time.  The timed rounds use
+the factored Jacobian row update (nbg_set_row_update(1): Jacobians and their
+gradients within 1e-10 relative of the reference's order, the tolerance asked
+of Jacobian entries; states and transit times bit for bit) unless
+--row-update 0; the same rounds in the reference's order are reported beside.
 
 Reported beside it (side legs, each dropped -- never the headline -- when the
-time budget would not fit it):
+time budget would not fit it; they run the library's default, reference-order
+row update):
+  rounds_reference_order  the headline's rounds in the reference's order;
   c2_makespan  the same batch run once to completion (every system to its tmax);
   plain_gpu    the one-system-per-launch form (nbg_integrate_transits), per-n
                step latency on a sample, summed over the batch's step counts;
@@ -46,11 +52,12 @@ import numpy as np  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6
 # HBM bytes per system-step of the dominant launch in rounds of the 4096-system
-# batch, from rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the same
-# counters' reading of nbg_copy_kernel of known size (the figures and the
-# commands are in profiles/nbg_run_c2_round_counters.md and
-# profiles/nbg_run_duo_c2_round_counters.md)
-HBM_BYTES_PER_UNIT = {"nbg_run<9>": 373.3e3, "nbg_run_duo<9>": 480.5e3}
+# batch, by kernel and row update (0 reference order, 1 factored), from
+# rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the same counters'
+# reading of nbg_copy_kernel of known size (the figures and the commands are
+# in profiles/nbg_run_c2_round_counters.md, nbg_run_duo_c2_round_counters.md
+# and nbg_run_duo_c2_round_factored_counters.md)
+HBM_BYTES_PER_UNIT = {("nbg_run<9>", 0): 373.3e3, ("nbg_run_duo<9>", 0): 480.5e3, ("nbg_run_duo<9>", 1): 425.8e3}
 METRIC = "system-steps/s w/ Jacobian, batched vs plain GPU; p50/p99 request latency"
 T_START = time.perf_counter()
 
@@ -240,6 +247,11 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--systems-per-wave", type=int, default=0, choices=[0, 1, 2],
                     help="0: the autotuner's choice (1 without it); 1 or 2: forced")
+    ap.add_argument("--row-update", type=int, default=1, choices=[0, 1],
+                    help="Jacobian row update of the timed rounds: 1 factored (Jacobians within 1e-10 relative of "
+                         "the reference's order, states and transit times bitwise), 0 the reference's order")
+    ap.add_argument("--reference-rounds", type=int, default=5,
+                    help="rounds also timed in the reference's order of the row update (side leg; 0: skip)")
     ap.add_argument("--no-makespan", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -269,6 +281,7 @@ def main():
     specs = Wk.c2(args.systems)
     states = Wk.initial_states(specs)
     ctx = nb.default_context()
+    ctx.set_row_update(args.row_update)
     name, cus = ctx.info()
     log("device %s, %d CUs; %d systems prepared" % (name, cus, len(specs)))
 
@@ -331,7 +344,7 @@ def main():
         log("launch %s: grid %d, %.1f ms per round, %d units %s, %.3f TFLOP/s" %
             (L["kernel"], L["grid"], L["ms_per_round"], L["units"], L["units_by_n"], L["tflops"]))
     traffic = None
-    bytes_unit = HBM_BYTES_PER_UNIT.get(D["kernel"])
+    bytes_unit = HBM_BYTES_PER_UNIT.get((D["kernel"], args.row_update))
     if bytes_unit is not None:
         traffic = bytes_unit * D["units"] / args.steps
     out = {
@@ -355,13 +368,17 @@ def main():
                    "parallelism": "1 GPU, one persistent launch per body-count class (n <= 9 in one), "
                                   "%d system%s per wave" % (spw, "s" if spw > 1 else ""),
                    "systems_per_wave": spw,
+                   "row_update": "factored (Jacobians and their gradients within 1e-10 relative of the reference "
+                                 "order, states and transit times bit for bit)" if args.row_update == 1 else
+                                 "reference order (every output bit for bit the oracle's)",
                    "autotune": tune},
         "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": D["kernel"], "launch_ms": D["ms_per_round"], "units_per_launch": D["units"] / args.steps,
                      "note": "FP64 vector (VALU) peak 78.6 TFLOP/s; no MFMA in the kernel (a measured pair-update "
                              "probe ran 28 % slower on the FP64 matrix cores); W(n) = 2016 n^2 (n-1) + 126 n^3 flops "
-                             "per system-step; traffic = " +
+                             "per system-step, the reference formulation's chain work whatever the row update's form "
+                             "(the factored form needs fewer operations for it); traffic = " +
                              ("%.1f KB per system-step (calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE of these rounds) "
                               "x the launch's system-steps per round" % (bytes_unit / 1e3) if bytes_unit else
                               "not measured for this kernel")},
@@ -381,6 +398,23 @@ def main():
             return False
         return True
 
+    if args.row_update == 1 and args.reference_rounds > 0 and fits(8, "rounds_reference_order"):
+        # the same rounds with the row update in the reference's order (bit for bit the oracle's results)
+        ctx.set_row_update(0)
+        batch.round(K)
+        rms, runits = [], 0
+        for k in range(args.reference_rounds):
+            t = time.perf_counter()
+            st = batch.round(K)
+            rms.append((time.perf_counter() - t) * 1e3)
+            runits += st["system_steps"]
+        out["rounds_reference_order"] = {"value": runits / (sum(rms) / 1e3), "unit": "system-steps/s",
+                                         "ms_per_round": float(np.mean(rms)), "rounds": args.reference_rounds,
+                                         "what": "the headline's rounds with the Jacobian row update in the reference's "
+                                                 "order: every output bit for bit the oracle's"}
+        log("rounds, reference order of the row update: %.4g system-steps/s (%.1f ms per round)" %
+            (out["rounds_reference_order"]["value"], np.mean(rms)))
+    ctx.set_row_update(0)  # the side legs below run the library's default form
     if not args.no_makespan and fits(45, "c2_makespan"):
         ctx.set_coop(1, 128)  # whole runs: the automatic two-wave selection, four one-wave groups per CU
         ctx.set_launch(0)
