@@ -90,7 +90,7 @@ template <int NB, bool G = (NB >= JGLOBAL_MIN)> struct Smem {
 template <int NB> struct Swork {
   static constexpr int N3 = 3 * NB, N4 = 4 * NB;
   static constexpr int NP = NB * (NB - 1) / 2;
-  static constexpr int PQ = 17;  // per-pair factors of phisalpha!, see phi_build
+  static constexpr int PQ = 13;  // per-pair factors of phisalpha!, see phisalpha_cols
   // pair slots share the space of the force-gradient arrays
   static constexpr int CH_FIT = (int)(((2 * N3 * N4 + (NP > 0 ? NP : 1) * PQ) * sizeof(double)) / sizeof(PairSlot));
   static constexpr int CH_MIN = CH_FIT < 4 ? 4 : CH_FIT;
@@ -375,7 +375,10 @@ __device__ __attribute__((always_inline)) inline void phisalpha_cols(Smem<NB, G>
       for (int k = 0; k < 3; ++k) S.a[3 * b + k] = ab[k];
   }
   wsync();
-  // loop 2 factors (ahl21.jl:609-640), needing the accelerations above
+  // loop 2 factors (ahl21.jl:609-640), needing the accelerations above.
+  // Products of a pair scalar with a component (f5 = fac * (5 / r2) and
+  // ff = (3 * fac1) * rij[k]) are stored as the scalar and formed where they
+  // are used, with the same operations.
   for (int p = col; p < NP; p += STEP) {
     int i = 0, q = p;
     while (q >= NB - 1 - i) { q -= NB - 1 - i; ++i; }
@@ -393,17 +396,16 @@ __device__ __attribute__((always_inline)) inline void phisalpha_cols(Smem<NB, G>
     const double fac2 = ((2.0 * NBG_GNEWT) * (mi + mj)) / r1 + 3.0 * ardot;
     double* o = pq + p * PQ;
     for (int k = 0; k < 3; ++k) {
-      const double fac = fac1 * (rij[k] * fac2 - r2 * aij[k]);
-      o[k] = fac;                                                  // fac
-      o[3 + k] = fac * (5.0 / r2);                                 // f5
-      o[6 + k] = (((2.0 * NBG_GNEWT) * fac1) * rij[k]) / r1;       // fm
-      o[9 + k] = ((-2.0) * fac1) * ((((rij[k] * NBG_GNEWT) * (mi + mj)) / (r2 * r1)) + aij[k]);  // fr
-      // (fac1 * 3) * rij[k] multiplies aij[p] in fac3 and is ff = (3 * fac1) * rij[k]
-      // as well (IEEE multiplication commutes)
-      o[12 + k] = (fac1 * 3.0) * rij[k];
+      o[k] = fac1 * (rij[k] * fac2 - r2 * aij[k]);                                             // fac
+      o[4 + k] = (((2.0 * NBG_GNEWT) * fac1) * rij[k]) / r1;                                   // fm
+      o[7 + k] = ((-2.0) * fac1) * ((((rij[k] * NBG_GNEWT) * (mi + mj)) / (r2 * r1)) + aij[k]);  // fr
     }
-    o[15] = fac1 * fac2;   // fd
-    o[16] = (-fac1) * r2;  // fa
+    o[3] = 5.0 / r2;       // f5 = fac * o[3]
+    // (fac1 * 3) * rij[k] multiplies aij[p] in fac3 and is ff = (3 * fac1) * rij[k]
+    // as well (IEEE multiplication commutes)
+    o[10] = fac1 * 3.0;
+    o[11] = fac1 * fac2;   // fd
+    o[12] = (-fac1) * r2;  // fa
   }
   wsync();
   // loop 2: velocity corrections, dqdt_phi and jac_phi (ahl21.jl:607-698),
@@ -440,10 +442,10 @@ __device__ __attribute__((always_inline)) inline void phisalpha_cols(Smem<NB, G>
       const double m = bi ? S.m[j] : S.m[i];  // the partner's mass
       const bool self_pos = mpos && myd == b, other_pos = mpos && myd == c;
       const bool self_mass = mmass && myd == b, other_mass = mmass && myd == c;
-      const double fd = o[15], fa = o[16];
+      const double fd = o[11], fa = o[12];
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const double fac = o[k], f5 = o[3 + k], fm = o[6 + k], fr = o[9 + k], ff = o[12 + k];
+        const double fac = o[k], f5 = fac * o[3], fm = o[4 + k], fr = o[7 + k], ff = o[10] * rij[k];
         const double mf = m * fac;
         kahan(vb[k], veb[k], bi ? mf : -mf);
         if (needQ) {
@@ -455,7 +457,7 @@ __device__ __attribute__((always_inline)) inline void phisalpha_cols(Smem<NB, G>
         const double y = fm * m;
         const double sy = bi ? y : -y, sfac = bi ? fac : -fac;
         const double fdm = fd * m;
-        const double fac3 = fr * rmy + o[12 + k] * amy;
+        const double fac3 = fr * rmy + ff * amy;
         const double mf3 = m * fac3;
         const bool pk = myp == k;
         const double t1 = self_pos ? -x5 : (other_pos ? x5 : (self_mass ? sy : (other_mass ? sfac : -0.0)));
@@ -1798,16 +1800,22 @@ template <int N> constexpr size_t lds_bytes_upto() {
 // layouts (duo_fits), so the launch keeps its occupancy; other systems, and
 // pairs that cannot share a schedule, run one after the other as in nbg_run.
 // ---------------------------------------------------------------------------
+// LDS of a two-systems-per-wave launch of class HI: the CU's 160 KB over as
+// many workgroups as the one-system layouts allow, so occupancy is theirs.
+template <int HI> constexpr size_t duo_lds_bytes() {
+  constexpr size_t CU = 163840, k = CU / lds_bytes_upto<HI>();
+  return (CU / k) / 256 * 256;
+}
 template <int N, int HI, bool G = true> struct Duo {
   static constexpr size_t REC = sizeof(Smem<N, G>);
-  static constexpr size_t FREE = lds_bytes_upto<HI>() > 2 * REC ? lds_bytes_upto<HI>() - 2 * REC : 0;
+  static constexpr size_t FREE = duo_lds_bytes<HI>() > 2 * REC ? duo_lds_bytes<HI>() - 2 * REC : 0;
   // the work area takes what the launch's LDS leaves (at least one Swork<N>)
   static constexpr size_t WBYTES = (FREE > sizeof(Swork<N>) ? FREE : sizeof(Swork<N>)) / 8 * 8;
   Smem<N, G> s0, s1;  // G: Jacobians in HBM
   alignas(8) unsigned char w[WBYTES];
 };
 template <int N, int HI, bool G = true> constexpr bool duo_fits() {
-  return sizeof(Duo<N, HI, G>) <= lds_bytes_upto<HI>();
+  return sizeof(Duo<N, HI, G>) <= duo_lds_bytes<HI>();
 }
 // slots per system in a Duo's shared work area
 template <int N, int HI, bool G = true> constexpr int duo_ch() {
@@ -2504,7 +2512,8 @@ __global__ __launch_bounds__(WAVE) void nbg_run(GroupList gl, SysBufs B, int64_t
 template <int HI>
 __global__ __launch_bounds__(WAVE) void nbg_run_duo(GroupList gl, SysBufs B, int64_t chunk,
                                                     unsigned long long* __restrict__ units) {
-  __shared__ __attribute__((aligned(16))) unsigned char raw[lds_bytes_upto<HI>()];
+  static_assert(duo_lds_bytes<HI>() >= lds_bytes_upto<HI>(), "the one-system layouts must fit");
+  __shared__ __attribute__((aligned(16))) unsigned char raw[duo_lds_bytes<HI>()];
   const int lane = threadIdx.x;
   int g = 0;
   while (g < gl.ng) {
