@@ -300,9 +300,10 @@ __device__ __attribute__((always_inline)) inline void phisalpha_cols(Smem<NB, G>
   double* D = W.f.dadq;
   // phisalpha! (ahl21.jl:558-700, alpha = 2) for the Kepler pairs.
   // First one lane per pair evaluates the pair's factors (the divisions and
-  // square roots) into pq; then, one row block (the three rows of body b) at
-  // a time, a lane keeps the block's rows of its dadq and jac_phi columns in
-  // registers and accumulates them from pq.  Body b's rows receive
+  // square roots) into pq; then each lane forms its dadq column (loop 1),
+  // and, one row block (the three rows of body b) at a time, keeps the
+  // block's rows of its jac_phi column in registers and accumulates them
+  // from pq (loop 2).  Body b's rows receive
   // contributions only from the pairs that contain b, and in the reference's
   // lexicographic pair order those come partner by partner in increasing
   // order; looping over the partner c in increasing order therefore repeats
@@ -331,48 +332,66 @@ __device__ __attribute__((always_inline)) inline void phisalpha_cols(Smem<NB, G>
     }
   }
   wsync();
-  // loop 1: accelerations and dadq (ahl21.jl:567-602).  For the lane's
-  // column (body myd, coordinate myp; myp = 3 for the mass) the reference's
-  // conditional updates of pair (i, j) reduce, per role of myd in the pair,
-  // to at most two additions in a fixed order: t1 then t2, each -0.0 (which
-  // leaves any sum unchanged) when it does not apply.  "self" is the body b
-  // of the row block, "other" its partner c.
+  // loop 1: accelerations and dadq (ahl21.jl:567-602), column by column.
+  // For the lane's column (body myd, coordinate myp; myp = 3 for the mass)
+  // the reference's conditional updates of pair (i, j) reduce, per role of
+  // myd in the pair, to at most two additions in a fixed order, t1 then t2
+  // (-0.0, which leaves any sum unchanged, when one does not apply).  Row
+  // block b != myd receives them from one pair only, (b, myd) -- myd is b's
+  // partner there -- so its entry is (0 + t1) + t2.  Row block myd receives
+  // them from every partner c in increasing order, the order of the
+  // reference's pair loop, summed in registers; so does the acceleration
+  // a[myp, myd] on the lane whose coordinate is myp.  Every entry therefore
+  // gets exactly the reference's additions.
   const bool mpos = myp >= 0 && myp < 3, mmass = myp == 3;
   const int ry = mpos ? myp : 0;
+  if (own) {
 #pragma unroll 1
-  for (int b = 0; b < NB; ++b) {
-    double ab[3] = {0.0, 0.0, 0.0};
+    for (int b = 0; b < NB; ++b) {
+      if (b == myd) continue;
+      const int i = b < myd ? b : myd, j = b < myd ? myd : b;
+      const bool bi = (b == i);
+      double t[3] = {0.0, 0.0, 0.0};
+      if (!(anyKick && S.pair[i + NB * j])) {
+        const double* o = pq + pidx(i, j) * PQ;
+        const double m = S.m[myd];  // b's partner's mass
+        const double f2m = o[0] * m;
+        const double rmy = S.x[3 * i + ry] - S.x[3 * j + ry];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const double fac = o[1 + k];
+          const double x3 = (o[4 + k] * m) * rmy;
+          const double t1 = mmass ? (bi ? -fac : fac) : -x3;
+          const double t2 = (mpos && myp == k) ? f2m : -0.0;
+          t[k] = (0.0 + t1) + t2;
+        }
+      }
+      for (int k = 0; k < 3; ++k) D[(3 * b + k) * N4 + col] = t[k];
+    }
     double db[3] = {0.0, 0.0, 0.0};
+    double ab = 0.0;
 #pragma unroll 1
     for (int c = 0; c < NB; ++c) {
-      if (c == b) continue;
-      const int i = c < b ? c : b, j = c < b ? b : c;
+      if (c == myd) continue;
+      const int i = c < myd ? c : myd, j = c < myd ? myd : c;
       if (anyKick && S.pair[i + NB * j]) continue;
       const double* o = pq + pidx(i, j) * PQ;
-      const double fac2 = o[0];
+      const bool bi = (myd == i);
+      const double m = S.m[c];  // the partner's mass
+      const double f2m = o[0] * m;
       const double rmy = S.x[3 * i + ry] - S.x[3 * j + ry];
-      const bool bi = (b == i);
-      const double m = bi ? S.m[j] : S.m[i];  // the partner's mass
-      const bool self_pos = mpos && myd == b, other_pos = mpos && myd == c;
-      const bool other_mass = mmass && myd == c;
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        const double fac = o[1 + k];
-        const double f3 = o[4 + k];
-        const double mf = m * fac;
-        ab[k] += bi ? -mf : mf;
-        const double x3 = (f3 * m) * rmy;
-        const double f2m = fac2 * m;
-        const bool pk = myp == k;
-        const double t1 = other_mass ? (bi ? -fac : fac) : (self_pos ? x3 : (other_pos ? -x3 : -0.0));
-        const double t2 = (pk && self_pos) ? -f2m : ((pk && other_pos) ? f2m : -0.0);
+        const double x3 = (o[4 + k] * m) * rmy;
+        const double t1 = mpos ? x3 : -0.0;
+        const double t2 = (mpos && myp == k) ? -f2m : -0.0;
         db[k] = (db[k] + t1) + t2;
       }
+      const double mf = m * o[1 + ry];
+      ab += bi ? -mf : mf;
     }
-    if (own)
-      for (int k = 0; k < 3; ++k) D[(3 * b + k) * N4 + col] = db[k];
-    if (col == 0)
-      for (int k = 0; k < 3; ++k) S.a[3 * b + k] = ab[k];
+    for (int k = 0; k < 3; ++k) D[(3 * myd + k) * N4 + col] = db[k];
+    if (mpos) S.a[3 * myd + myp] = ab;
   }
   wsync();
   // loop 2 factors (ahl21.jl:609-640), needing the accelerations above.
