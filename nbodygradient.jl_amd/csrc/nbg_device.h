@@ -289,7 +289,7 @@ template <int NB, bool G> __device__ void kick_build(Smem<NB, G>& S, Swork<NB>& 
 // loops striding by STEP lanes: one system on a whole wave (STEP = WAVE,
 // col = lane) or two on its halves (STEP = 32, phi_build2).  col 0 writes
 // the system's uniform values (a, v, dqdt_phi).
-template <int NB, bool G, int STEP>
+template <int NB, bool G, int STEP, bool FACT = false>
 __device__ __attribute__((always_inline)) inline void phisalpha_cols(Smem<NB, G>& S, Swork<NB>& W, double h, bool anyKick,
                                                                      bool needQ, int col) {
   constexpr int N3 = 3 * NB, N4 = 4 * NB;
@@ -431,6 +431,86 @@ __device__ __attribute__((always_inline)) inline void phisalpha_cols(Smem<NB, G>
   // with the same role reduction as loop 1: per k, at most three
   // role-dependent additions (t1, t2, t3), then the dadq terms.
   const double q3h = 3.0 / h;
+  if constexpr (FACT) {
+    // Factored (rounds differently from the reference's order): for pair
+    // (i, j) the rows of body i receive mj * X and those of body j -mi * X
+    // of the same column term X, plus the first mass derivative on the
+    // partner's mass column, so X is formed once per pair.  Pairs run in the
+    // reference's lexicographic order, so body i's rows (and its uniform
+    // velocity and dqdt_phi, summed exactly as in the reference) stay in
+    // registers across the partners j > i; a body's rows as a partner j are
+    // read and written back in LDS.  jac_phi starts at phic!'s terms.
+    if (own && !anyKick)
+      for (int r = 0; r < N3; ++r) PHI[r * N4 + col] = 0.0;
+    int p = 0;
+#pragma unroll 1
+    for (int i = 0; i < NB - 1; ++i) {
+      double vi[3], vie[3], qi[3], pi[3], xi[3], Di[3];
+      for (int k = 0; k < 3; ++k) {
+        vi[k] = S.v[3 * i + k];
+        vie[k] = S.ve[3 * i + k];
+        qi[k] = needQ ? S.dqdt_phi[7 * i + 3 + k] : 0.0;
+        pi[k] = own ? PHI[(3 * i + k) * N4 + col] : 0.0;
+        xi[k] = S.x[3 * i + k];
+        Di[k] = own ? D[(3 * i + k) * N4 + col] : 0.0;
+      }
+      const double mi = S.m[i], ai = S.a[3 * i + ry];
+      const bool di = myd == i;
+#pragma unroll 1
+      for (int j = i + 1; j < NB; ++j, ++p) {
+        if (anyKick && S.pair[i + NB * j]) continue;
+        const double* o = pq + p * PQ;
+        const double mj = S.m[j];
+        double rij[3], Dj[3], vj[3], vje[3], qj[3], pj[3];
+        for (int k = 0; k < 3; ++k) {
+          rij[k] = xi[k] - S.x[3 * j + k];
+          Dj[k] = own ? D[(3 * j + k) * N4 + col] : 0.0;
+          vj[k] = S.v[3 * j + k];
+          vje[k] = S.ve[3 * j + k];
+          qj[k] = needQ ? S.dqdt_phi[7 * j + 3 + k] : 0.0;
+          pj[k] = own ? PHI[(3 * j + k) * N4 + col] : 0.0;
+        }
+        double dot = 0.0;
+        for (int k = 0; k < 3; ++k) dot += rij[k] * (Di[k] - Dj[k]);
+        const double rmy = ry == 0 ? rij[0] : (ry == 1 ? rij[1] : rij[2]);
+        const double amy = ai - S.a[3 * j + ry];
+        const bool dj = myd == j;
+        const double fd = o[11], fa = o[12];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          const double fac = o[k];
+          kahan(vi[k], vie[k], mj * fac);
+          kahan(vj[k], vje[k], -(mi * fac));
+          if (needQ) {
+            qi[k] += (q3h * mj) * fac;
+            qj[k] += -((q3h * mi) * fac);
+          }
+          const double ff = o[10] * rij[k];
+          const double base = (((myp == k) ? fd : 0.0) - (fac * o[3]) * rmy) + (o[7 + k] * rmy + ff * amy);
+          const double role = mpos ? (di ? base : (dj ? -base : 0.0)) : ((mmass && (di || dj)) ? o[4 + k] : 0.0);
+          const double X = (role + fa * (Di[k] - Dj[k])) + ff * dot;
+          pi[k] += mj * X + ((mmass && dj) ? fac : 0.0);
+          pj[k] += (-mi) * X + ((mmass && di) ? -fac : 0.0);
+        }
+        for (int k = 0; k < 3; ++k) {
+          if (own) PHI[(3 * j + k) * N4 + col] = pj[k];
+          if (col == 0) {
+            S.v[3 * j + k] = vj[k];
+            S.ve[3 * j + k] = vje[k];
+            if (needQ) S.dqdt_phi[7 * j + 3 + k] = qj[k];
+          }
+        }
+      }
+      for (int k = 0; k < 3; ++k) {
+        if (own) PHI[(3 * i + k) * N4 + col] = pi[k];
+        if (col == 0) {
+          S.v[3 * i + k] = vi[k];
+          S.ve[3 * i + k] = vie[k];
+          if (needQ) S.dqdt_phi[7 * i + 3 + k] = qi[k];
+        }
+      }
+    }
+  } else {
 #pragma unroll 1
   for (int b = 0; b < NB; ++b) {
     double vb[3], veb[3], qb[3], pb[3];
@@ -495,6 +575,7 @@ __device__ __attribute__((always_inline)) inline void phisalpha_cols(Smem<NB, G>
         if (needQ) S.dqdt_phi[7 * b + 3 + k] = qb[k];
       }
   }
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -502,7 +583,8 @@ __device__ __attribute__((always_inline)) inline void phisalpha_cols(Smem<NB, G>
 // Kepler pairs, alpha = 2) with Jacobian.  Phi and dadq columns are owned by
 // lanes; a, v and dqdt_phi are uniform (lane 0 stores).
 // ---------------------------------------------------------------------------
-template <int NB, bool G> __device__ void phi_build(Smem<NB, G>& S, Swork<NB>& W, double h, bool anyKick, bool needQ, int lane) {
+template <int NB, bool G, bool FACT = false>
+__device__ void phi_build(Smem<NB, G>& S, Swork<NB>& W, double h, bool anyKick, bool needQ, int lane) {
   constexpr int N3 = 3 * NB, N4 = 4 * NB;
   const bool own = lane < N4;
   int myd = -1, myp = -1;
@@ -645,7 +727,7 @@ template <int NB, bool G> __device__ void phi_build(Smem<NB, G>& S, Swork<NB>& W
         wsync();
       }
   }
-  phisalpha_cols<NB, G, WAVE>(S, W, h, anyKick, needQ, lane);
+  phisalpha_cols<NB, G, WAVE, FACT>(S, W, h, anyKick, needQ, lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -1102,10 +1184,10 @@ template <int NB, bool G, int STEP = WAVE>
 __device__ __attribute__((always_inline)) inline void step_mid_rest(Smem<NB, G>& S, Swork<NB>& W, bool needJ, bool needQ,
                                                                     int col);
 
-template <int NB, bool G>
+template <int NB, bool G, bool FACT = false>
 __device__ __attribute__((always_inline)) inline void step_mid(Smem<NB, G>& S, Swork<NB>& W, double h, bool needJ, bool needQ, bool anyKick, int lane) {
   // ---- phic!, phisalpha!
-  phi_build<NB>(S, W, h, anyKick, needQ, lane);
+  phi_build<NB, G, FACT>(S, W, h, anyKick, needQ, lane);
   wsync();
   step_mid_rest<NB>(S, W, needJ, needQ, lane);
 }
@@ -1219,7 +1301,7 @@ __device__ void ahl21_step_form(Smem<NB, G>& S, Swork<NB>& W, double h, bool nee
                                 int lane) {
   step_pre<NB>(S, W, h, needJ, needQ, anyKick, lane);
   sweep<NB, G, FACT>(S, W, 0, 0.5 * h, needJ, needQ, lane);  // forward pair sweep (drift first)
-  step_mid<NB>(S, W, h, needJ, needQ, anyKick, lane);
+  step_mid<NB, G, FACT>(S, W, h, needJ, needQ, anyKick, lane);
   sweep<NB, G, FACT>(S, W, 1, 0.5 * h, needJ, needQ, lane);  // backward pair sweep (drift after)
   step_post<NB>(S, W, h, needJ, needQ, anyKick, lane);
 }
@@ -1244,13 +1326,13 @@ __device__ void ahl21_step(Smem<NB, G>& S, Swork<NB>& W, double h, bool needJ, b
 // system's columns and pair factors, [32, 64) the second's, each lane with
 // exactly the operations phi_build gives it; each system builds into its own
 // work area.  Systems without kicked pairs only.
-template <int NB, bool G>
+template <int NB, bool G, bool FACT = false>
 __device__ __attribute__((always_inline)) inline void phi_build2(Smem<NB, G>& S0, Smem<NB, G>& S1, Swork<NB>& W0,
                                                                  Swork<NB>& W1, double h0, double h1, bool nQ0, bool nQ1,
                                                                  int lane) {
   static_assert(4 * NB <= 32 && Swork<NB>::NP <= 32, "a system's columns and pairs must fit half a wave");
   const bool sys = lane >= 32;
-  phisalpha_cols<NB, G, 32>(sys ? S1 : S0, sys ? W1 : W0, sys ? h1 : h0, false, sys ? nQ1 : nQ0, lane & 31);
+  phisalpha_cols<NB, G, 32, FACT>(sys ? S1 : S0, sys ? W1 : W0, sys ? h1 : h0, false, sys ? nQ1 : nQ0, lane & 31);
 }
 
 template <int NB, bool G, bool PHI2, bool FACT>
@@ -1263,7 +1345,7 @@ __device__ void ahl21_step2(Smem<NB, G>& S0, Smem<NB, G>& S1, Swork<NB>& W, int 
   if constexpr (PHI2) {
     if (!k0 && !k1) {  // the second work area follows the first
       Swork<NB>& W1 = *reinterpret_cast<Swork<NB>*>(reinterpret_cast<unsigned char*>(&W) + sizeof(Swork<NB>));
-      phi_build2<NB>(S0, S1, W, W1, h0, h1, nQ0, nQ1, lane);
+      phi_build2<NB, G, FACT>(S0, S1, W, W1, h0, h1, nQ0, nQ1, lane);
       wsync();
       step_mid_rest<NB>(S0, W, nJ0, nQ0, lane);
       step_mid_rest<NB>(S1, W1, nJ1, nQ1, lane);
@@ -1271,8 +1353,8 @@ __device__ void ahl21_step2(Smem<NB, G>& S0, Smem<NB, G>& S1, Swork<NB>& W, int 
     }
   }
   if (!shared) {
-    step_mid<NB>(S0, W, h0, nJ0, nQ0, k0, lane);
-    step_mid<NB>(S1, W, h1, nJ1, nQ1, k1, lane);
+    step_mid<NB, G, FACT>(S0, W, h0, nJ0, nQ0, k0, lane);
+    step_mid<NB, G, FACT>(S1, W, h1, nJ1, nQ1, k1, lane);
   }
   sweep2<NB, G, false, FACT>(S0, S1, W.slot, ch, 1, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
   step_post<NB>(S0, W, h0, nJ0, nQ0, k0, lane);
@@ -1296,7 +1378,7 @@ __device__ void ahl21_step2h(Smem<NB, false>& S0, Smem<NB, false>& S1, Swork<NB>
   const bool nJ = sys ? nJ1 : nJ0, nQ = sys ? nQ1 : nQ0;
   step_pre<NB, false, 32>(S, W, h, nJ, nQ, false, col);
   sweep2<NB, false, true, FACT>(S0, S1, slots, ch, 0, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
-  phisalpha_cols<NB, false, 32>(S, W, h, false, nQ, col);
+  phisalpha_cols<NB, false, 32, FACT>(S, W, h, false, nQ, col);
   wsync();
   step_mid_rest<NB, false, 32>(S, W, nJ, nQ, col);
   sweep2<NB, false, true, FACT>(S0, S1, slots, ch, 1, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);

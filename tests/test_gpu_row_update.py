@@ -1,9 +1,11 @@
-"""The factored Jacobian row update (nbg_set_row_update(1)): the state, dq/dt
-and the transit times, which never read the Jacobian, stay bit for bit the
-oracle's; the Jacobian and the transit-time gradients agree with the oracle
-within the tolerance asked of Jacobian entries (1e-10 relative to each
-array's largest entry, the reference's maxabs norm) for one system per call
-(C0, and C1's whole 66,667-step run against its golden file), and for
+"""The factored Jacobian updates (nbg_set_row_update(1)): the state (x, v)
+and the transit counts, which never read the Jacobian or its force-gradient
+block, stay bit for bit the oracle's; the transit times agree within 1e-12
+relative per transit (they read dq/dt, whose jac_phi term rounds
+differently); the Jacobian and the transit-time gradients agree with the
+oracle within the tolerance asked of Jacobian entries (1e-10 relative to
+each array's largest entry, the reference's maxabs norm) for one system per
+call (C0, and C1's whole 66,667-step run against its golden file), and for
 batches with one and with two systems per wave.  Each test prints the
 largest relative difference it saw."""
 import os
@@ -19,6 +21,7 @@ from nbodygradient_amd.batch import Batch
 pytestmark = pytest.mark.gpu
 HERE = os.path.dirname(os.path.abspath(__file__))
 REL = 1e-10
+TT_REL = 1e-12
 
 
 def _rel(a, b):
@@ -30,6 +33,15 @@ def _tol(a, b, rel=REL):
     r = _rel(a, b)
     print("  max relative difference %.3g" % r, flush=True)
     return r <= rel
+
+
+def _tt_close(a, b):
+    """Transit times: |a - b| <= TT_REL * |b| for every transit (printed)."""
+    assert a.shape == b.shape
+    m = b != 0
+    r = float(np.max(np.abs(a[m] - b[m]) / np.abs(b[m]))) if m.any() else 0.0
+    print("  transit times: max relative difference %.3g" % r, flush=True)
+    return r <= TT_REL and np.array_equal(a[~m], b[~m])
 
 
 def _factored(fn):
@@ -57,10 +69,11 @@ def test_factored_single_system_c0():
     os_ = O.OState(x, v, spec.elements[:, 0], t=spec.t0, jac_init=jinit)
     ott, odq, ode, oc = O.integrate_transits(os_, spec.h, spec.tmax, spec.ntt(), grad=True)
     assert np.array_equal(tt.count, oc) and oc.sum() > 100
-    assert np.array_equal(tt.tt, ott)
+    assert _tt_close(tt.tt, ott)
     assert np.array_equal(s.x, os_.x) and np.array_equal(s.v, os_.v)
     assert _tol(tt.dtdq0, odq) and _tol(tt.dtdelements, ode)
     assert _tol(s.jac_step, os_.jac_step)
+    assert _tol(s.dqdt, os_.dqdt, rel=1e-12)
     assert not np.array_equal(s.jac_step, os_.jac_step)  # the factored form is in use
 
 
@@ -78,7 +91,7 @@ def test_factored_c1_full_run():
 
     s, tt = _factored(run)
     assert np.array_equal(tt.count, g["count"])
-    assert np.array_equal(tt.tt, g["tt"])
+    assert _tt_close(tt.tt, g["tt"])
     assert np.array_equal(s.x, g["x"]) and np.array_equal(s.v, g["v"])
     assert _tol(tt.dtdelements[:, ::int(g["every"])], g["dtdelements_sel"])
 
@@ -112,7 +125,7 @@ def test_factored_batch(spw):
         os_ = O.OState(x, v, sp.elements[:, 0], t=sp.t0, jac_init=jinit)
         ott, odq, ode, oc = O.integrate_transits(os_, sp.h, sp.tmax, sp.ntt(), grad=True)
         assert np.array_equal(r["count"], oc)
-        assert np.array_equal(r["tt"], ott)
+        assert _tt_close(r["tt"], ott)
         assert np.array_equal(r["x"], os_.x) and np.array_equal(r["v"], os_.v)
         if oc.sum() > 0:
             assert _tol(r["dtdq0"], odq) and _tol(r["dtdelements"], ode)
