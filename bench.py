@@ -10,9 +10,9 @@ every step, calc_dtdelements! when its run ends) in one persistent launch;
 systems whose run has ended restart from their initial records, so every
 round carries the same mix.  `value` = system-steps (main steps + one per
 recorded transit) of the timed rounds / their wall time.  The timed rounds use
-the factored Jacobian row update (nbg_set_row_update(1): Jacobians and their
-gradients within 1e-10 relative of the reference's order, the tolerance asked
-of Jacobian entries; states and transit times bit for bit) unless
+the factored Jacobian updates (nbg_set_row_update(1): Jacobians and their
+gradients within 1e-10 relative of the reference's order and transit times
+within 1e-12, the tolerances asked of them; states bit for bit) unless
 --row-update 0; the same rounds in the reference's order are reported beside.
 
 Reported beside it (side legs, each dropped -- never the headline -- when the
@@ -55,9 +55,10 @@ FP64_PEAK_TFLOPS = 78.6
 # batch, by kernel and row update (0 reference order, 1 factored), from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the same counters'
 # reading of nbg_copy_kernel of known size (the figures and the commands are
-# in profiles/nbg_run_c2_round_counters.md, nbg_run_duo_c2_round_counters.md
-# and nbg_run_duo_c2_round_factored_counters.md)
-HBM_BYTES_PER_UNIT = {("nbg_run<9>", 0): 373.3e3, ("nbg_run_duo<9>", 0): 480.5e3, ("nbg_run_duo<9>", 1): 425.8e3}
+# in profiles/nbg_run_c2_round_factored_counters.md, nbg_run_duo_c2_round_counters.md
+# and nbg_run_duo_c2_round_factored_counters.md); a configuration without a
+# measurement reports traffic null
+HBM_BYTES_PER_UNIT = {("nbg_run<9>", 1): 353.3e3, ("nbg_run_duo<9>", 0): 470.3e3, ("nbg_run_duo<9>", 1): 427.6e3}
 METRIC = "system-steps/s w/ Jacobian, batched vs plain GPU; p50/p99 request latency"
 T_START = time.perf_counter()
 
@@ -248,8 +249,8 @@ def main():
     ap.add_argument("--systems-per-wave", type=int, default=0, choices=[0, 1, 2],
                     help="0: the autotuner's choice (1 without it); 1 or 2: forced")
     ap.add_argument("--row-update", type=int, default=1, choices=[0, 1],
-                    help="Jacobian row update of the timed rounds: 1 factored (Jacobians within 1e-10 relative of "
-                         "the reference's order, states and transit times bitwise), 0 the reference's order")
+                    help="Jacobian updates of the timed rounds: 1 factored (Jacobians within 1e-10 relative of "
+                         "the reference's order, transit times within 1e-12, states bitwise), 0 the reference's order")
     ap.add_argument("--reference-rounds", type=int, default=5,
                     help="rounds also timed in the reference's order of the row update (side leg; 0: skip)")
     ap.add_argument("--no-makespan", action="store_true")
@@ -369,7 +370,7 @@ def main():
                                   "%d system%s per wave" % (spw, "s" if spw > 1 else ""),
                    "systems_per_wave": spw,
                    "row_update": "factored (Jacobians and their gradients within 1e-10 relative of the reference "
-                                 "order, states and transit times bit for bit)" if args.row_update == 1 else
+                                 "order, transit times within 1e-12, states bit for bit)" if args.row_update == 1 else
                                  "reference order (every output bit for bit the oracle's)",
                    "autotune": tune},
         "roofline": {"bound": "fp64_valu", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
@@ -377,7 +378,7 @@ def main():
                      "kernel": D["kernel"], "launch_ms": D["ms_per_round"], "units_per_launch": D["units"] / args.steps,
                      "note": "FP64 vector (VALU) peak 78.6 TFLOP/s; no MFMA in the kernel (a measured pair-update "
                              "probe ran 28 % slower on the FP64 matrix cores); W(n) = 2016 n^2 (n-1) + 126 n^3 flops "
-                             "per system-step, the reference formulation's chain work whatever the row update's form "
+                             "per system-step, the reference formulation's chain work whatever the updates' form "
                              "(the factored form needs fewer operations for it); traffic = " +
                              ("%.1f KB per system-step (calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE of these rounds) "
                               "x the launch's system-steps per round" % (bytes_unit / 1e3) if bytes_unit else

@@ -311,16 +311,18 @@ int nbg_set_coop(nbg_ctx* ctx, int32_t mode, int32_t max_systems);
  * Results are bitwise those of k = 1.  Serving rounds keep one system per
  * wave. */
 int nbg_set_systems_per_wave(nbg_ctx* ctx, int32_t k);
-/* Form of the Jacobian row update of the pair sweeps (every launch of ctx;
- * the chain product J += jac_ij * J of ahl21.jl:31-42,60-72).  mode 0
- * (default): the reference's mul! order, results bitwise equal to the CPU
- * restatement.  mode 1: factored -- the Kepler block of jac_ij gives body i's
- * rows +mj*kep and body j's -mi*kep times the same difference of the two
- * bodies' rows, so kep*(a_i - a_j) is formed once (36 instead of 144 fused
- * multiply-adds per column and pair).  It rounds differently: jac_step,
- * jac_error, dtdq0 and dtdelements agree with mode 0 to ~1e-13 relative (the
- * tolerance asked of Jacobian entries is 1e-10); x, v, dq/dt and transit
- * times, which never read the Jacobian, stay bitwise. */
+/* Form of the Jacobian updates (every launch of ctx): the chain product
+ * J += jac_ij * J of the pair sweeps (ahl21.jl:31-42,60-72) and phisalpha!'s
+ * jac_phi (ahl21.jl:607-698).  mode 0 (default): the reference's order,
+ * results bitwise equal to the CPU restatement.  mode 1: factored -- a pair
+ * (i, j) contributes +mj and -mi times one shared term to the rows of body i
+ * and of body j, so that term is formed once per pair: kep*(a_i - a_j) in the
+ * sweeps (36 instead of 144 fused multiply-adds per column and pair) and
+ * jac_phi's column term.  It rounds differently: jac_step, jac_error, dtdq0
+ * and dtdelements agree with mode 0 to ~1e-11 relative (the tolerance asked
+ * of Jacobian entries is 1e-10), dq/dt and the transit times, whose Newton
+ * iteration reads dq/dt, within 1e-12 (measured equal); x and v, which read
+ * neither, stay bitwise. */
 int nbg_set_row_update(nbg_ctx* ctx, int32_t mode);
 /* Diagnostic only (no counterpart in the reference): bound every wait between
  * the two waves of a cooperative workgroup to `polls` polls (0: the defaults,

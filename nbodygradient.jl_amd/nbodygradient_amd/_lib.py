@@ -168,10 +168,10 @@ class Context:
         self.check(lib().nbg_set_systems_per_wave(self._h, int(k)), "nbg_set_systems_per_wave")
 
     def set_row_update(self, mode=0):
-        """Jacobian row update of the pair sweeps: 0 the reference's order
-        (bitwise), 1 factored (Jacobians and their gradients to ~1e-13
-        relative; states and transit times bitwise).  C ABI:
-        nbg_set_row_update."""
+        """Form of the Jacobian updates (pair sweeps and phisalpha!'s
+        jac_phi): 0 the reference's order (bitwise), 1 factored (Jacobians and
+        their gradients to ~1e-11 relative, dq/dt and transit times within
+        1e-12, states bitwise).  C ABI: nbg_set_row_update."""
         self.check(lib().nbg_set_row_update(self._h, int(mode)), "nbg_set_row_update")
 
     def set_coop(self, mode=1, max_systems=128):
