@@ -1111,8 +1111,11 @@ __device__ __attribute__((always_inline)) inline void sweep2(Smem<NB, G>& S0, Sm
 // ---------------------------------------------------------------------------
 // The stages of one step between the pair sweeps: before the forward sweep
 // (kickfast! and the first drift), between the sweeps (phic!/phisalpha!),
-// after the backward sweep (the second drift and kickfast!).
-template <int NB, bool G, int STEP = WAVE>
+// after the backward sweep (the second drift and kickfast!).  The
+// zero-addend compensated passes of comp_sum_matrix! over the rows an
+// operation leaves unchanged (zero_add_rows) re-normalise pairs that their
+// last update already normalised; the factored form (FACT) leaves them out.
+template <int NB, bool G, int STEP = WAVE, bool FACT = false>
 __device__ __attribute__((always_inline)) inline void step_pre(Smem<NB, G>& S, Swork<NB>& W, double h, bool needJ, bool needQ, bool anyKick, int col) {
   constexpr int N3 = 3 * NB, N4 = 4 * NB, N7 = 7 * NB;
   const double h2 = 0.5 * h;
@@ -1160,7 +1163,7 @@ __device__ __attribute__((always_inline)) inline void step_pre(Smem<NB, G>& S, S
       double P[N3];
       load_pos<NB>(S, c, P);
       drift_cols<NB>(S, c, h2);
-      zero_add_rows<NB>(S, c, true, !anyKick);
+      if constexpr (!FACT) zero_add_rows<NB>(S, c, true, !anyKick);
       if (anyKick) vel_product_add<NB>(S, W.f.dadq, c, P);
     }
   // ---- drift of the state (drift_grad! / drift!): entries are independent
@@ -1180,7 +1183,7 @@ __device__ __attribute__((always_inline)) inline void step_pre(Smem<NB, G>& S, S
   wsync();
 }
 
-template <int NB, bool G, int STEP = WAVE>
+template <int NB, bool G, int STEP = WAVE, bool FACT = false>
 __device__ __attribute__((always_inline)) inline void step_mid_rest(Smem<NB, G>& S, Swork<NB>& W, bool needJ, bool needQ,
                                                                     int col);
 
@@ -1189,12 +1192,12 @@ __device__ __attribute__((always_inline)) inline void step_mid(Smem<NB, G>& S, S
   // ---- phic!, phisalpha!
   phi_build<NB, G, FACT>(S, W, h, anyKick, needQ, lane);
   wsync();
-  step_mid_rest<NB>(S, W, needJ, needQ, lane);
+  step_mid_rest<NB, G, WAVE, FACT>(S, W, needJ, needQ, lane);
 }
 
 // The rest of the stage between the sweeps, once jac_phi is built:
 // J_vel += jac_phi * J_pos, and dq/dt.
-template <int NB, bool G, int STEP>
+template <int NB, bool G, int STEP, bool FACT>
 __device__ __attribute__((always_inline)) inline void step_mid_rest(Smem<NB, G>& S, Swork<NB>& W, bool needJ, bool needQ,
                                                                     int col) {
   constexpr int N3 = 3 * NB, N4 = 4 * NB, N7 = 7 * NB;
@@ -1203,7 +1206,7 @@ __device__ __attribute__((always_inline)) inline void step_mid_rest(Smem<NB, G>&
       double P[N3];
       load_pos<NB>(S, c, P);
       vel_product_add<NB>(S, W.f.phi, c, P);
-      zero_add_rows<NB>(S, c, true, false);
+      if constexpr (!FACT) zero_add_rows<NB>(S, c, true, false);
     }
   if (needQ) {
     // tmp7n = jac_phi*dqdt; dqdt .+= dqdt_phi .+ tmp7n
@@ -1230,7 +1233,7 @@ __device__ __attribute__((always_inline)) inline void step_mid_rest(Smem<NB, G>&
   wsync();
 }
 
-template <int NB, bool G, int STEP = WAVE>
+template <int NB, bool G, int STEP = WAVE, bool FACT = false>
 __device__ __attribute__((always_inline)) inline void step_post(Smem<NB, G>& S, Swork<NB>& W, double h, bool needJ, bool needQ, bool anyKick, int col) {
   constexpr int N3 = 3 * NB, N4 = 4 * NB, N7 = 7 * NB;
   const double h2 = 0.5 * h;
@@ -1288,9 +1291,9 @@ __device__ __attribute__((always_inline)) inline void step_post(Smem<NB, G>& S, 
         double P[N3];
         load_pos<NB>(S, c, P);
         vel_product_add<NB>(S, W.f.dadq, c, P);
-        zero_add_rows<NB>(S, c, true, false);
+        if constexpr (!FACT) zero_add_rows<NB>(S, c, true, false);
       } else {
-        zero_add_rows<NB>(S, c, true, true);
+        if constexpr (!FACT) zero_add_rows<NB>(S, c, true, true);
       }
     }
   wsync();
@@ -1299,11 +1302,11 @@ __device__ __attribute__((always_inline)) inline void step_post(Smem<NB, G>& S, 
 template <int NB, bool G, bool FACT>
 __device__ void ahl21_step_form(Smem<NB, G>& S, Swork<NB>& W, double h, bool needJ, bool needQ, bool anyKick,
                                 int lane) {
-  step_pre<NB>(S, W, h, needJ, needQ, anyKick, lane);
+  step_pre<NB, G, WAVE, FACT>(S, W, h, needJ, needQ, anyKick, lane);
   sweep<NB, G, FACT>(S, W, 0, 0.5 * h, needJ, needQ, lane);  // forward pair sweep (drift first)
   step_mid<NB, G, FACT>(S, W, h, needJ, needQ, anyKick, lane);
   sweep<NB, G, FACT>(S, W, 1, 0.5 * h, needJ, needQ, lane);  // backward pair sweep (drift after)
-  step_post<NB>(S, W, h, needJ, needQ, anyKick, lane);
+  step_post<NB, G, WAVE, FACT>(S, W, h, needJ, needQ, anyKick, lane);
 }
 
 // The step in the record's form of the Jacobian row update (SysDesc::rowupd):
@@ -1338,8 +1341,8 @@ __device__ __attribute__((always_inline)) inline void phi_build2(Smem<NB, G>& S0
 template <int NB, bool G, bool PHI2, bool FACT>
 __device__ void ahl21_step2(Smem<NB, G>& S0, Smem<NB, G>& S1, Swork<NB>& W, int ch, double h0, double h1,
                             bool nJ0, bool nQ0, bool nJ1, bool nQ1, bool k0, bool k1, int lane) {
-  step_pre<NB>(S0, W, h0, nJ0, nQ0, k0, lane);
-  step_pre<NB>(S1, W, h1, nJ1, nQ1, k1, lane);
+  step_pre<NB, G, WAVE, FACT>(S0, W, h0, nJ0, nQ0, k0, lane);
+  step_pre<NB, G, WAVE, FACT>(S1, W, h1, nJ1, nQ1, k1, lane);
   sweep2<NB, G, false, FACT>(S0, S1, W.slot, ch, 0, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
   bool shared = false;
   if constexpr (PHI2) {
@@ -1347,8 +1350,8 @@ __device__ void ahl21_step2(Smem<NB, G>& S0, Smem<NB, G>& S1, Swork<NB>& W, int 
       Swork<NB>& W1 = *reinterpret_cast<Swork<NB>*>(reinterpret_cast<unsigned char*>(&W) + sizeof(Swork<NB>));
       phi_build2<NB, G, FACT>(S0, S1, W, W1, h0, h1, nQ0, nQ1, lane);
       wsync();
-      step_mid_rest<NB>(S0, W, nJ0, nQ0, lane);
-      step_mid_rest<NB>(S1, W1, nJ1, nQ1, lane);
+      step_mid_rest<NB, G, WAVE, FACT>(S0, W, nJ0, nQ0, lane);
+      step_mid_rest<NB, G, WAVE, FACT>(S1, W1, nJ1, nQ1, lane);
       shared = true;
     }
   }
@@ -1357,8 +1360,8 @@ __device__ void ahl21_step2(Smem<NB, G>& S0, Smem<NB, G>& S1, Swork<NB>& W, int 
     step_mid<NB, G, FACT>(S1, W, h1, nJ1, nQ1, k1, lane);
   }
   sweep2<NB, G, false, FACT>(S0, S1, W.slot, ch, 1, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
-  step_post<NB>(S0, W, h0, nJ0, nQ0, k0, lane);
-  step_post<NB>(S1, W, h1, nJ1, nQ1, k1, lane);
+  step_post<NB, G, WAVE, FACT>(S0, W, h0, nJ0, nQ0, k0, lane);
+  step_post<NB, G, WAVE, FACT>(S1, W, h1, nJ1, nQ1, k1, lane);
 }
 
 // One step of both systems of a Duo entirely at once, for body counts whose
@@ -1376,13 +1379,13 @@ __device__ void ahl21_step2h(Smem<NB, false>& S0, Smem<NB, false>& S1, Swork<NB>
   Swork<NB>& W = sys ? W1 : W0;
   const double h = sys ? h1 : h0;
   const bool nJ = sys ? nJ1 : nJ0, nQ = sys ? nQ1 : nQ0;
-  step_pre<NB, false, 32>(S, W, h, nJ, nQ, false, col);
+  step_pre<NB, false, 32, FACT>(S, W, h, nJ, nQ, false, col);
   sweep2<NB, false, true, FACT>(S0, S1, slots, ch, 0, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
   phisalpha_cols<NB, false, 32, FACT>(S, W, h, false, nQ, col);
   wsync();
-  step_mid_rest<NB, false, 32>(S, W, nJ, nQ, col);
+  step_mid_rest<NB, false, 32, FACT>(S, W, nJ, nQ, col);
   sweep2<NB, false, true, FACT>(S0, S1, slots, ch, 1, 0.5 * h0, 0.5 * h1, nJ0, nQ0, nJ1, nQ1, lane);
-  step_post<NB, false, 32>(S, W, h, nJ, nQ, false, col);
+  step_post<NB, false, 32, FACT>(S, W, h, nJ, nQ, false, col);
 }
 
 // ---------------------------------------------------------------------------
