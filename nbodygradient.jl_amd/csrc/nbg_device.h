@@ -1160,8 +1160,8 @@ __device__ __attribute__((always_inline)) inline void step_pre(Smem<NB, G>& S, S
   // ---- Jacobian: jac_copy = K*J (pre-drift), drift_grad!, J += jac_copy
   if (needJ)
     for (int c = col; c < N7; c += STEP) {
-      double P[N3];
-      load_pos<NB>(S, c, P);
+      double P[N3];  // the pre-drift position rows, the operand of K*J
+      if (anyKick) load_pos<NB>(S, c, P);
       drift_cols<NB>(S, c, h2);
       if constexpr (!FACT) zero_add_rows<NB>(S, c, true, !anyKick);
       if (anyKick) vel_product_add<NB>(S, W.f.dadq, c, P);
