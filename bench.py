@@ -55,10 +55,10 @@ FP64_PEAK_TFLOPS = 78.6
 # batch, by kernel and row update (0 reference order, 1 factored), from
 # rocprofv3 FETCH_SIZE and WRITE_SIZE passes scaled by the same counters'
 # reading of nbg_copy_kernel of known size (the figures and the commands are
-# in profiles/nbg_run_c2_round_factored_counters.md, nbg_run_duo_c2_round_counters.md
-# and nbg_run_duo_c2_round_factored_counters.md); a configuration without a
-# measurement reports traffic null
-HBM_BYTES_PER_UNIT = {("nbg_run<9>", 1): 353.3e3, ("nbg_run_duo<9>", 0): 470.3e3, ("nbg_run_duo<9>", 1): 427.6e3}
+# in profiles/nbg_run_duo_c2_round_counters.md and
+# nbg_run_duo_c2_round_factored_counters.md); a configuration without a
+# measurement of the current kernels reports traffic null
+HBM_BYTES_PER_UNIT = {("nbg_run_duo<9>", 0): 470.3e3, ("nbg_run_duo<9>", 1): 354.7e3}
 METRIC = "system-steps/s w/ Jacobian, batched vs plain GPU; p50/p99 request latency"
 T_START = time.perf_counter()
 

@@ -318,7 +318,10 @@ int nbg_set_systems_per_wave(nbg_ctx* ctx, int32_t k);
  * (i, j) contributes +mj and -mi times one shared term to the rows of body i
  * and of body j, so that term is formed once per pair: kep*(a_i - a_j) in the
  * sweeps (36 instead of 144 fused multiply-adds per column and pair) and
- * jac_phi's column term.  It rounds differently: jac_step, jac_error, dtdq0
+ * jac_phi's column term; and the zero-addend compensated passes of
+ * comp_sum_matrix! over rows an operation leaves unchanged, which only
+ * re-normalise pairs their last update normalised, are left out.  It rounds
+ * differently: jac_step, jac_error, dtdq0
  * and dtdelements agree with mode 0 to ~1e-11 relative (the tolerance asked
  * of Jacobian entries is 1e-10), dq/dt and the transit times, whose Newton
  * iteration reads dq/dt, within 1e-12 (measured equal); x and v, which read

@@ -169,9 +169,10 @@ class Context:
 
     def set_row_update(self, mode=0):
         """Form of the Jacobian updates (pair sweeps and phisalpha!'s
-        jac_phi): 0 the reference's order (bitwise), 1 factored (Jacobians and
-        their gradients to ~1e-11 relative, dq/dt and transit times within
-        1e-12, states bitwise).  C ABI: nbg_set_row_update."""
+        jac_phi): 0 the reference's order (bitwise), 1 factored, without the
+        zero-addend compensated passes (Jacobians and their gradients to
+        ~1e-11 relative, dq/dt and transit times within 1e-12, states
+        bitwise).  C ABI: nbg_set_row_update."""
         self.check(lib().nbg_set_row_update(self._h, int(mode)), "nbg_set_row_update")
 
     def set_coop(self, mode=1, max_systems=128):
