@@ -287,8 +287,9 @@ template <int NB, bool G> __device__ void kick_build(Smem<NB, G>& S, Swork<NB>& 
 // phisalpha! (ahl21.jl:558-700, alpha = 2) for one system, on the lanes
 // whose column index within the system is col = 0, 1, ..., its pair-factor
 // loops striding by STEP lanes: one system on a whole wave (STEP = WAVE,
-// col = lane) or two on its halves (STEP = 32, phi_build2).  col 0 writes
-// the system's uniform values (a, v, dqdt_phi).
+// col = lane) or two on its halves (STEP = 32, phi_build2).  The lane of
+// body d's coordinate p writes the acceleration a[p, d]; col 0 writes the
+// other uniform values (v, dqdt_phi).  FACT: jac_phi in the factored form.
 template <int NB, bool G, int STEP, bool FACT = false>
 __device__ __attribute__((always_inline)) inline void phisalpha_cols(Smem<NB, G>& S, Swork<NB>& W, double h, bool anyKick,
                                                                      bool needQ, int col) {
@@ -303,7 +304,8 @@ __device__ __attribute__((always_inline)) inline void phisalpha_cols(Smem<NB, G>
   // square roots) into pq; then each lane forms its dadq column (loop 1),
   // and, one row block (the three rows of body b) at a time, keeps the
   // block's rows of its jac_phi column in registers and accumulates them
-  // from pq (loop 2).  Body b's rows receive
+  // from pq (loop 2; the factored form goes pair by pair instead, see
+  // there).  Body b's rows receive
   // contributions only from the pairs that contain b, and in the reference's
   // lexicographic pair order those come partner by partner in increasing
   // order; looping over the partner c in increasing order therefore repeats
@@ -581,7 +583,8 @@ __device__ __attribute__((always_inline)) inline void phisalpha_cols(Smem<NB, G>
 // ---------------------------------------------------------------------------
 // phic! (ahl21.jl:392-552, kicked pairs) and phisalpha! (ahl21.jl:558-700,
 // Kepler pairs, alpha = 2) with Jacobian.  Phi and dadq columns are owned by
-// lanes; a, v and dqdt_phi are uniform (lane 0 stores).
+// lanes; v and dqdt_phi are uniform (lane 0 stores), and phisalpha!'s
+// accelerations are stored by the lane of each body and coordinate.
 // ---------------------------------------------------------------------------
 template <int NB, bool G, bool FACT = false>
 __device__ void phi_build(Smem<NB, G>& S, Swork<NB>& W, double h, bool anyKick, bool needQ, int lane) {
