@@ -129,3 +129,72 @@ def test_factored_batch(spw):
         assert np.array_equal(r["x"], os_.x) and np.array_equal(r["v"], os_.v)
         if oc.sum() > 0:
             assert _tol(r["dtdq0"], odq) and _tol(r["dtdelements"], ode)
+
+
+def _check_factored(r, sp, x, v, jinit):
+    os_ = O.OState(x, v, sp.elements[:, 0], t=sp.t0, jac_init=jinit)
+    ott, odq, ode, oc = O.integrate_transits(os_, sp.h, sp.tmax, sp.ntt(), grad=True)
+    assert np.array_equal(r["count"], oc)
+    assert _tt_close(r["tt"], ott)
+    assert np.array_equal(r["x"], os_.x)
+    if oc.sum() > 0:
+        assert _tol(r["dtdq0"], odq) and _tol(r["dtdelements"], ode)
+
+
+def test_factored_two_wave_workgroups():
+    """The factored updates in two-wave workgroups (transit search on the
+    second wave), forced for every system of a small batch."""
+    print("gpu: factored row update, two-wave workgroups", flush=True)
+    rng = np.random.default_rng(67)
+    specs = []
+    for q in range(8):
+        sp = W.c2_system(rng, 3 + q % 7)
+        sp.tmax = min(sp.tmax, 500 * sp.h)
+        specs.append(sp)
+    states = W.initial_states(specs, processes=1)
+    ctx = nb.default_context()
+    ctx.set_coop(2, 128)
+    try:
+        def run():
+            b = Batch(specs, states)
+            b.run()
+            r = b.download()
+            b.close()
+            return r
+        res = _factored(run)
+    finally:
+        ctx.set_coop(1, 128)
+    for sp, (x, v, jinit), r in zip(specs, states, res):
+        _check_factored(r, sp, x, v, jinit)
+
+
+def test_factored_serving_rounds():
+    """Serving rounds take the context's form of the updates as well."""
+    print("gpu: factored row update, serving rounds", flush=True)
+    from nbodygradient_amd.server import Server
+    rng = np.random.default_rng(71)
+    specs = []
+    for q in range(10):
+        sp = W.c2_system(rng, 2 + q % 9)
+        sp.tmax = min(sp.tmax, 300 * sp.h)
+        specs.append(sp)
+    states = W.initial_states(specs, processes=1)
+
+    def run():
+        sv = Server(max_bytes=1 << 30, chunk=41)
+        ids = [sv.submit(s, st) for s, st in zip(specs, states)]
+        out = {}
+        for _ in range(100000):
+            sv.step()
+            for rid in sv.finished():
+                out[rid] = sv.fetch(rid)
+            st = sv.status()
+            if st["pending"] == 0 and st["active"] == 0:
+                break
+        sv.close()
+        return ids, out
+
+    ids, out = _factored(run)
+    assert sorted(out) == sorted(ids)
+    for rid, sp, (x, v, jinit) in zip(ids, specs, states):
+        _check_factored(out[rid], sp, x, v, jinit)
